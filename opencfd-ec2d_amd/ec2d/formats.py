@@ -1,0 +1,269 @@
+"""Readers and writers for the OpenCFD-EC2D case files.
+
+The formats are the reference's own, so that a case directory can be run by
+either the reference Fortran or this package:
+
+* ``control.in``  six list-directed records, each preceded by a comment line
+  (opencfd_ec2d_v1.5.4.f90:197-212).
+* ``Mesh2d.dat``  block count, (NI, NJ) per block, then x(i,j) and y(i,j)
+  per block with i fastest (sub_init.f90:122-127, 176).
+* ``bc2d_1.in``   two comment lines, block count, then per block two comment
+  lines, the subface count, a comment line and one 9-integer record per
+  subface ``f_no face ist iend jst jend neighb subface orient``
+  (sub_init.f90:228-249).
+* ``flow2d.dat``  restart file: a title line, then per block a zone line and
+  (nx+1)(ny+1) records ``x y d u v T ...`` over j=0..ny, i=0..nx
+  (sub_init.f90:509-523).
+* ``FDM.in``      optional list of blocks computed by finite differences
+  (sub_Finite_Difference.f90:287-305).
+"""
+from __future__ import annotations
+
+import os
+from dataclasses import dataclass, field, fields
+from typing import List, Sequence
+
+import numpy as np
+
+# Codes of the reference (opencfd_ec2d_v1.5.4.f90:42-47, 50).
+SCHEME_UD1, SCHEME_NND2, SCHEME_UD3, SCHEME_WENO3, SCHEME_MUSCL2, SCHEME_MUSCL3, SCHEME_OMUSCL2 = range(7)
+FLUX_SW, FLUX_HLL, FLUX_HLLC, FLUX_ROE, FLUX_VANLEER, FLUX_AUSM = 1, 2, 3, 4, 5, 6
+RECON_ORIGINAL, RECON_CONSERVATIVE, RECON_CHARACTERISTIC = 0, 1, 2
+BC_WALL, BC_FARFIELD, BC_SYMMETRY, BC_INLET, BC_OUTLET = -10, -20, -30, -40, -50
+TIME_LU_SGS, TIME_EULER1, TIME_RK3, TIME_DUAL_LU_SGS = 0, 1, 3, -1
+TURB_NONE, TURB_BL, TURB_SA, TURB_SST = 0, 1, 2, 3
+LAP = 2
+PI = 3.1415926535897932
+
+
+def _fnum(tok: str) -> float:
+    """Parse one Fortran list-directed real (``1.d-6``, ``10.``, ``2.31d0``)."""
+    return float(tok.replace("d", "e").replace("D", "e"))
+
+
+def _ftok(x: float) -> str:
+    """Format a double so that a Fortran list-directed read recovers it exactly."""
+    return repr(float(x)).replace("e", "E")
+
+
+@dataclass
+class Control:
+    """The records of ``control.in`` (names follow opencfd_ec2d_v1.5.4.f90:203-211)."""
+
+    Ma: float = 0.5
+    Re: float = 1.0e6
+    gamma: float = 1.4
+    AoA: float = 0.0            # degrees, as written in the file
+    Pr: float = 0.72
+    t_end: float = 1.0
+    Kstep_save: int = 1000000
+    If_viscous: int = 0
+    Iflag_turbulence_model: int = TURB_NONE
+    Iflag_init: int = 0
+    Iflag_local_dt: int = 1
+    dt_global: float = 2.0 ** -10
+    CFL: float = 5.0
+    dtmax: float = 1.0
+    dtmin: float = 1.0e-10
+    Time_Method: int = TIME_LU_SGS
+    P_outlet: float = -1.0
+    T_inf: float = 288.15
+    Twall: float = -1.0
+    vt_inf: float = 3.0
+    Kt_inf: float = 1.0e-8
+    Wt_inf: float = 1.0e-6
+    Iflag_Scheme: int = SCHEME_MUSCL3
+    Iflag_Flux: int = FLUX_ROE
+    IFlag_Reconstruction: int = RECON_ORIGINAL
+    Kstep_show: int = 1000000
+    Num_Mesh: int = 1
+    Num_Threads: int = 1
+    Nstep_Inner_Limit: int = 20
+    Res_Inner_Limit: float = 1.0e-6
+    Pre_Step_Mesh: List[int] = field(default_factory=lambda: [0, 0, 0])
+
+    # -- derived quantities, as read_parameter computes them (:222-233) ----
+    @property
+    def nvar(self) -> int:
+        if self.Iflag_turbulence_model == TURB_SST:
+            return 6
+        if self.Iflag_turbulence_model == TURB_SA:
+            return 5
+        return 4
+
+    @property
+    def aoa_rad(self) -> float:
+        return self.AoA * PI / 180.0
+
+    @property
+    def Cv(self) -> float:
+        return 1.0 / (self.gamma * (self.gamma - 1.0) * self.Ma * self.Ma)
+
+    @property
+    def Cp(self) -> float:
+        return self.Cv * self.gamma
+
+    @property
+    def Twall_nd(self) -> float:
+        return self.Twall / self.T_inf
+
+    def iterations(self) -> int:
+        """Outer iterations the reference main loop makes: tt += dt_global while tt < t_end."""
+        tt, n = 0.0, 0
+        while tt < self.t_end:
+            tt = tt + self.dt_global
+            n += 1
+        return n
+
+
+_RECORDS = [
+    ("Ma Re gamma AoA Pr t_end Kstep_save If_viscous Iflag_turbulence_model Iflag_init",
+     ["Ma", "Re", "gamma", "AoA", "Pr", "t_end", "Kstep_save", "If_viscous", "Iflag_turbulence_model", "Iflag_init"]),
+    ("Iflag_local_dt dt_global CFL dtmax dtmin Time_Method P_outlet T_inf Twall vt_inf Kt_inf Wt_inf",
+     ["Iflag_local_dt", "dt_global", "CFL", "dtmax", "dtmin", "Time_Method", "P_outlet", "T_inf", "Twall",
+      "vt_inf", "Kt_inf", "Wt_inf"]),
+    ("Iflag_Scheme Iflag_Flux IFlag_Reconstruction Kstep_show",
+     ["Iflag_Scheme", "Iflag_Flux", "IFlag_Reconstruction", "Kstep_show"]),
+    ("Num_Mesh Num_Threads Nstep_Inner_Limit Res_Inner_Limit",
+     ["Num_Mesh", "Num_Threads", "Nstep_Inner_Limit", "Res_Inner_Limit"]),
+]
+
+
+def write_control(path: str, c: Control) -> None:
+    types = {f.name: f.type for f in fields(Control)}
+    lines = []
+    for header, names in _RECORDS:
+        lines.append(" " + header)
+        vals = []
+        for n in names:
+            v = getattr(c, n)
+            vals.append(str(int(v)) if types[n] in ("int", int) else _ftok(v))
+        lines.append(" " + "  ".join(vals))
+    lines.append(" Pre_Step_Mesh(1:Num_Mesh)")
+    lines.append(" " + "  ".join(str(int(v)) for v in c.Pre_Step_Mesh[: c.Num_Mesh]))
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+def read_control(path: str) -> Control:
+    with open(path) as f:
+        raw = f.read().splitlines()
+    c = Control()
+    types = {f.name: f.type for f in fields(Control)}
+    rec = 1  # record k lives on line 2k (a comment line precedes each one)
+    for _, names in _RECORDS:
+        toks = raw[rec].split()
+        for n, t in zip(names, toks):
+            v = _fnum(t)
+            setattr(c, n, int(v) if types[n] in ("int", int) else v)
+        rec += 2
+    toks = raw[rec].split()
+    pre = [int(_fnum(t)) for t in toks[: c.Num_Mesh]]
+    c.Pre_Step_Mesh = (pre + [0, 0, 0])[:3]
+    return c
+
+
+def write_mesh2d(path: str, xs: Sequence[np.ndarray], ys: Sequence[np.ndarray]) -> None:
+    """xs[m], ys[m] are (ny, nx) arrays of node coordinates (index [j-1, i-1])."""
+    with open(path, "w") as f:
+        f.write(f"{len(xs)}\n")
+        f.write(" ".join(f"{x.shape[1]} {x.shape[0]}" for x in xs) + "\n")
+        for x, y in zip(xs, ys):
+            for arr in (x, y):
+                flat = np.ascontiguousarray(arr).ravel()  # i fastest, then j
+                for k in range(0, flat.size, 4):
+                    f.write(" ".join(_ftok(v) for v in flat[k:k + 4]) + "\n")
+
+
+def _tokens(path: str) -> List[str]:
+    with open(path) as f:
+        return f.read().split()
+
+
+def read_mesh2d(path: str):
+    tok = _tokens(path)
+    nb = int(tok[0])
+    dims = [(int(tok[1 + 2 * m]), int(tok[2 + 2 * m])) for m in range(nb)]
+    pos = 1 + 2 * nb
+    xs, ys = [], []
+    for nx, ny in dims:
+        n = nx * ny
+        x = np.array([_fnum(t) for t in tok[pos:pos + n]]).reshape(ny, nx)
+        pos += n
+        y = np.array([_fnum(t) for t in tok[pos:pos + n]]).reshape(ny, nx)
+        pos += n
+        xs.append(x)
+        ys.append(y)
+    return xs, ys
+
+
+def write_bcin(path: str, bcs: Sequence[Sequence[Sequence[int]]], title: str = "ec2d case") -> None:
+    """bcs[m] is the list of 9-integer subface records of block m+1."""
+    out = [f" 2D BC file for multi-block mesh:  {title}", " # Blocks", f"  {len(bcs)}"]
+    for m, subs in enumerate(bcs):
+        out.append(f" Block  {m + 1}")
+        out.append(" Subface")
+        out.append(f"  {len(subs)}")
+        out.append(" f_no, face, istart,iend, jstart, jend,  neighb, subface orient")
+        for s in subs:
+            out.append(" " + " ".join(f"{int(v):6d}" for v in s))
+    with open(path, "w") as f:
+        f.write("\n".join(out) + "\n")
+
+
+def read_bcin(path: str) -> List[List[List[int]]]:
+    with open(path) as f:
+        lines = f.read().splitlines()
+    nb = int(lines[2].split()[0])
+    pos = 3
+    bcs = []
+    for _ in range(nb):
+        pos += 2
+        nsub = int(lines[pos].split()[0])
+        pos += 2
+        subs = []
+        for _ in range(nsub):
+            subs.append([int(v) for v in lines[pos].split()[:9]])
+            pos += 1
+        bcs.append(subs)
+    return bcs
+
+
+def write_fdm_in(path: str, blocks: Sequence[int]) -> None:
+    lines = [" Blocks using the finite-difference method", " (hybrid FVM/FDM)", " ---", " NB_FDM",
+             f" {len(blocks)}", " block numbers"] + [f" {b}" for b in blocks]
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+def read_fdm_in(path: str) -> List[int]:
+    if not os.path.exists(path):
+        return []
+    with open(path) as f:
+        lines = f.read().splitlines()
+    nb = int(lines[4].split()[0])
+    return [int(lines[6 + k].split()[0]) for k in range(nb)]
+
+
+def write_flow2d(path: str, x1s, y1s, prims) -> None:
+    """Restart file read by init_flow_read (sub_init.f90:509-523).
+
+    prims[m] is an array (ny+1, nx+1, 4) of (d, u, v, T) over j=0..ny, i=0..nx
+    (the interior plus one ghost ring), x1s/y1s the matching cell centres.
+    """
+    with open(path, "w") as f:
+        f.write("variables=x,y,d,u,v,T,p\n")
+        for x1, y1, q in zip(x1s, y1s, prims):
+            nyp, nxp = q.shape[:2]
+            f.write(f" zone  i= {nxp} j= {nyp}\n")
+            for j in range(nyp):
+                for i in range(nxp):
+                    f.write(" ".join(_ftok(v) for v in (x1[j, i], y1[j, i], *q[j, i, :4])) + "\n")
+
+
+def fortran_pow_int(x, n: int):
+    """x**n for a constant integer n as flang evaluates it: ((x*x)*x)*... ."""
+    r = x
+    for _ in range(n - 1):
+        r = r * x
+    return r
