@@ -1,0 +1,159 @@
+/* ec2d.h - C-ABI of the MI355X residual / implicit-update path of OpenCFD-EC2D.
+ *
+ * Plain C types only (int, double, pointers); every entry point returns 0 on
+ * success and a nonzero status otherwise, with the message available from
+ * ec2d_last_error().  Arrays cross the boundary in the reference's own
+ * Fortran layout (column-major, the reference's bounds, the equation index
+ * fastest for U), so a Fortran caller passes c_loc() of its module arrays.
+ *
+ * Model: the library holds a batch of cases (one reference run each); every
+ * solver entry point below acts on all cases of the batch at once, so a
+ * single-case caller gets exactly the reference's subroutine and an ensemble
+ * caller gets one launch per stage for all of its cases.  Per-case results
+ * (residual norms, errors, time counters) are read back by case id.
+ *
+ * Reference interfaces replaced (file:line in OpenCFD-EC2D v1.5.4):
+ *   see the comment of each function.
+ */
+#ifndef EC2D_H
+#define EC2D_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* control.in after read_parameter's conversions (opencfd_ec2d_v1.5.4.f90:197-235):
+ * AoA in radians, Twall divided by T_inf, Cv and Cp derived. */
+typedef struct {
+  double Ma, Re, gamma, AoA, Pr, Cv, Cp;
+  double t_end, dt_global, CFL, dtmax, dtmin, P_outlet, T_inf, Twall, vt_inf, Kt_inf, Wt_inf;
+  int If_viscous, Iflag_turbulence_model, Iflag_init, Iflag_local_dt, Time_Method;
+  int Iflag_Scheme, Iflag_Flux, IFlag_Reconstruction, Num_Mesh, Nvar;
+  int Pre_Step_Mesh[3];
+} ec2d_control;
+
+/* array selectors for ec2d_put_array / ec2d_get_array (reference bounds) */
+enum {
+  EC2D_U = 1,      /* U(NV,-1:nx+1,-1:ny+1) */
+  EC2D_RES = 2,    /* Res(NV,1:nx-1,1:ny-1) */
+  EC2D_DT = 3,     /* dt(1:nx-1,1:ny-1) */
+  EC2D_DU = 4,     /* dU(NV,1:nx,1:ny) */
+  EC2D_LCI = 5, EC2D_LCJ = 6, EC2D_LVI = 7, EC2D_LVJ = 8, /* (1:nx,1:ny) */
+  EC2D_AMU = 9, EC2D_AMU_T = 10,                          /* (-1:nx+1,-1:ny+1), interior filled */
+  EC2D_UN = 23,    /* Un(NVun,0:nx,0:ny) */
+  EC2D_QF = 24,    /* QF(4,1:nx-1,1:ny-1) */
+  EC2D_DELTU = 25  /* deltU(4,-1:nx+1,-1:ny+1) */
+};
+
+/* ---- life cycle --------------------------------------------------------- */
+int ec2d_init(void);     /* select the GPU, create the stream; no reference counterpart */
+int ec2d_finalize(void); /* free everything */
+int ec2d_last_error(char *buf, int len);
+
+/* A new case with its control parameters (read_parameter,
+ * opencfd_ec2d_v1.5.4.f90:197-235, and set_control_para, :239-260: the
+ * coarse levels use UD1, no turbulence model and 4 equations).
+ * Returns the case id (>= 0) or a negative status. */
+int ec2d_case_create(const ec2d_control *ctl);
+
+/* Geometry of block mBlock (1-based) of mesh level nMesh (1 = finest) of a
+ * case, as the reference holds it after its set-up (Creat_Mesh1 /
+ * Creat_Mesh, sub_init.f90:111-380; Update_coordinate_buffer,
+ * sub_boundary.f90:159-301; comput_dw, sub_turbulence_SST.f90:274-365;
+ * comput_Jacobian, sub_Finite_Difference.f90:314-382).
+ *   bc_msg  9 ints per subface: f_no face ist iend jst jend neighb subface orient
+ *   x, y    (-1:nx+2,-1:ny+2) nodes incl. ghosts
+ *   x1, y1  (-1:nx+1,-1:ny+1) cell centres
+ *   vol, si, sj, ni1, ni2, nj1, nj2  (1:nx,1:ny)
+ *   dw      (0:nx,0:ny) wall distance, NULL unless SA / SST
+ *   fd      5 arrays Akx,Aky,Aix,Aiy,Ajac of (1:nx-1,1:ny-1) back to back, NULL unless fvm_fdm = 1 */
+int ec2d_define_block(int case_id, int nMesh, int mBlock, int nx, int ny, int nsub, const int *bc_msg, int fvm_fdm,
+                      const double *x, const double *y, const double *x1, const double *y1, const double *vol,
+                      const double *si, const double *sj, const double *ni1, const double *ni2, const double *nj1,
+                      const double *nj2, const double *dw, const double *fd);
+
+/* Allocate the device arenas of the batch and upload geometry; U starts at
+ * the reference's allocation defaults (sub_init.f90:160-174, 310-318). */
+int ec2d_commit(void);
+
+int ec2d_put_array(int case_id, int nMesh, int mBlock, int which, const double *src);
+int ec2d_get_array(int case_id, int nMesh, int mBlock, int which, double *dst);
+
+/* ---- the per-iteration path (acts on all cases of the batch) ------------ */
+
+/* Init_flow (sub_init.f90:383-495): freestream start with coarse-level
+ * pre-iterations and prolongation (Iflag_init = 0), or ghost fill of a state
+ * put with ec2d_put_array (Iflag_init = 1, as init_flow_read :499-560). */
+int ec2d_init_flow(void);
+
+/* Comput_Residual_one_mesh (sub_Residual.f90:13-107): Res, dt, Lci..Lvj and
+ * the residual norms of every case. */
+int ec2d_comput_residual_one_mesh(int nMesh);
+
+/* du_LU_SGS_2D (sub_LU_SGS.f90:4-108); mBlock = 0 sweeps every block. */
+int ec2d_du_lu_sgs_2d(int nMesh, int mBlock, double alfa1);
+
+/* the update loop U = U + dU of NS_Time_advance_LU_SGS (sub_NS_singlegrid.f90:203-211) */
+int ec2d_update_u(int nMesh);
+
+/* Boundary_condition_onemesh (sub_boundary.f90:10-34) */
+int ec2d_boundary_condition_onemesh(int nMesh);
+
+/* update_buffer_onemesh (sub_boundary.f90:45-153) */
+int ec2d_update_buffer_onemesh(int nMesh);
+
+/* force_vt_kw (sub_NS_singlegrid.f90:30-60) */
+int ec2d_force_vt_kw(int nMesh);
+
+/* NS_Time_advance (sub_NS_singlegrid.f90:8-26): one LU-SGS, Euler or RK3
+ * step including ghost fill and the nu_t / k / omega clip. */
+int ec2d_ns_time_advance(int nMesh);
+
+/* NS_2stge_multigrid / NS_3stge_multigrid (sub_NS_multigrid.f90:14-142) */
+int ec2d_ns_multigrid(void);
+
+/* multigrid pieces (sub_NS_multigrid.f90:156-398) */
+int ec2d_interpolation2h(int m1, int m2, int flag);
+int ec2d_prolong_u(int m1, int m2, int flag);
+int ec2d_comput_force_function(int nMesh);
+int ec2d_add_force_function(int nMesh);
+int ec2d_comput_delt_u(int nMesh);
+int ec2d_set_un(int nMesh);
+int ec2d_comput_new_u(int nMesh);
+
+/* n passes of the main-loop body (opencfd_ec2d_v1.5.4.f90:172-186) as one
+ * HIP graph replayed n times; no host synchronisation inside. */
+int ec2d_run(int n_iter);
+
+/* Res_max(1:6), Res_rms(1:6) of mesh nMesh of a case (after a residual);
+ * synchronises the stream.  Returns the T < 1e-5 error of the reference
+ * (sub_Residual.f90:143-146) as status 2 with the cell in ec2d_last_error. */
+int ec2d_res_norms(int case_id, int nMesh, double *res_max, double *res_rms);
+
+/* Kstep and tt of a case's mesh level (Mesh_TYPE, opencfd_ec2d_v1.5.4.f90:88-90) */
+int ec2d_time(int case_id, int nMesh, int *kstep, double *tt);
+
+int ec2d_synchronize(void);
+
+/* Cells of the batch, and of one case, on a mesh level (Num_Cell, sub_init.f90:134). */
+long long ec2d_num_cells(int case_id, int nMesh);
+
+/* Device time of the last ec2d_run call in milliseconds (HIP events on the
+ * library stream), and the mean time per launch of the residual and LU-SGS
+ * kernels measured with events around them during that call (0 if not
+ * measured).  No reference counterpart: measurement support. */
+int ec2d_timing(double *run_ms, double *residual_ms, double *lusgs_ms, double *ghost_ms);
+
+/* 1: ec2d_run launches eagerly and times every residual / LU-SGS / ghost
+ * launch with HIP events (see ec2d_timing); 0 (default): one HIP graph.
+ * No reference counterpart: measurement support. */
+int ec2d_set_profile(int on);
+
+/* Tile shape of the fused residual kernel (rows x 64 cells); no reference
+ * counterpart.  Call before ec2d_commit.  Valid rows: 1, 2, 4, 8. */
+int ec2d_set_tile_rows(int rows);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,495 @@
+// Fused residual kernel: Comput_Residual_one_mesh (sub_Residual.f90:13-107)
+// for all blocks of a batch in one launch.
+//
+// One workgroup computes one TX x TY tile of cells of one block:
+//   1. stage U of the tile plus a 2-cell halo from HBM into LDS and turn it
+//      into primitives (comput_duvtpckw, sub_Residual.f90:114-152), Sutherland
+//      viscosity (get_viscous, :670-689) and, for SA, nu_t
+//      (sub_turbulence_SA.f90:24-36); the wall ghost override of Amut_boundary
+//      (:180-220) is applied in LDS;
+//   2. every i-face and j-face of the tile: reconstruction, rotation, flux,
+//      back-rotation, viscous flux (Residual_FVM, :200-455) and the SA face
+//      fluxes (sub_turbulence_SA.f90:43-115); results stay in LDS;
+//   3. per cell: Res = Fi(i+1)-Fi(i)+Fj(j+1)-Fj(j) (:459-469), the SA
+//      residual and source (sub_turbulence_SA.f90:77-168), spectral radii
+//      (comput_Lvc, sub_NS_singlegrid.f90:352-388), local time step (comput_dt,
+//      :324-348) and this tile's share of Res_max / sum(Res^2).
+// HBM traffic per cell: U read once (+halo), Res, dt, Lci, Lcj (Lvi, Lvj)
+// written once; faces shared by two tiles are recomputed, never re-read.
+#include "ec2d_device.h"
+#include "ec2d_physics.h"
+
+namespace ec2d {
+
+struct ResArgs {
+  const TileDesc* tiles;
+  const int* tile_perm;       // launch order -> logical tile
+  const BlockDesc* blocks;
+  const CaseParams* cps;
+  double* arena;
+  double* partials;           // [logical tile][2*NV]: max |Res|, sum Res^2
+  unsigned long long* err;    // first cell with T < 1e-5 (block<<40 | j<<20 | i), ~0 if none
+  MeshOpts o;
+  PlaneMap pm;
+  int clip;                   // apply force_vt_kw to interior nu_t on load
+  int write_visc;             // store Amu / Amu_t planes
+};
+
+template <int TX, int TY, int NV, bool VISC, bool SA>
+__global__ __launch_bounds__(TX* TY) void k_residual(ResArgs a) {
+  constexpr int RW = TX + 4, RH = TY + 4, NT = TX * TY;
+  constexpr int NFI = (TX + 1) * TY, NFJ = TX * (TY + 1);
+  __shared__ double sd[RH][RW], su[RH][RW], sv[RH][RW], sp[RH][RW];
+  __shared__ double sT[VISC ? RH : 1][VISC ? RW : 1];
+  __shared__ double sAm[VISC ? RH : 1][VISC ? RW : 1], sAt[VISC ? RH : 1][VISC ? RW : 1];
+  __shared__ double sx[VISC ? RH : 1][VISC ? RW : 1], sy[VISC ? RH : 1][VISC ? RW : 1];
+  __shared__ double svt[SA ? RH : 1][SA ? RW : 1];
+  __shared__ double sFi[4][TY][TX + 1], sFj[4][TY + 1][TX];
+  __shared__ double sFv[SA ? 2 : 1][SA ? TY : 1][SA ? TX + 1 : 1], sFw[SA ? 2 : 1][SA ? TY + 1 : 1][SA ? TX : 1];
+  __shared__ double sred[2 * NV][NT / 64];
+
+  const int lt = a.tile_perm[blockIdx.x];
+  const TileDesc t = a.tiles[lt];
+  const BlockDesc B = a.blocks[t.block];
+  const CaseParams cp = a.cps[B.case_id];
+  const PlaneMap pm = a.pm;
+  const int nx = B.nx, ny = B.ny;
+  const double g = cp.gamma;
+  double* base = a.arena + B.base;
+  auto PL = [&](int slot) { return base + (int64_t)slot * B.plane; };
+  const int tid = threadIdx.x;
+  const int i0 = t.i0, j0 = t.j0;
+  const double* U[NV];
+#pragma unroll
+  for (int m = 0; m < NV; m++) U[m] = PL(pm.U + m);
+
+  // ---- 1. stage tile + halo ------------------------------------------------
+  for (int idx = tid; idx < RW * RH; idx += NT) {
+    int li = idx % RW, lj = idx / RW;
+    int i = i0 - 2 + li, j = j0 - 2 + lj;
+    if (i < -1 || i > nx + 1 || j < -1 || j > ny + 1) continue;
+    int64_t q = pidx(B, i, j);
+    double d = U[0][q];
+    double uu = U[1][q] / d;
+    double vv = U[2][q] / d;
+    double T = (U[3][q] - 0.5 * d * (uu * uu + vv * vv)) / (cp.Cv * d);
+    sd[lj][li] = d;
+    su[lj][li] = uu;
+    sv[lj][li] = vv;
+    sp[lj][li] = cp.p00 * d * T;
+    bool interior = (i >= 1 && i <= nx - 1 && j >= 1 && j <= ny - 1);
+    if (interior && T < 1.0e-5) {
+      unsigned long long key = ((unsigned long long)t.block << 40) | ((unsigned long long)j << 20) | (unsigned)i;
+      atomicMin(a.err, key);
+    }
+    if constexpr (VISC) {
+      sT[lj][li] = T;
+      bool ring = (i >= 0 && i <= nx && j >= 0 && j <= ny);
+      double am = 0.0;
+      if (ring) am = 1.0 / cp.Re * (1.0 + cp.Tsb) * sqrt(T * T * T) / (cp.Tsb + T);
+      sAm[lj][li] = am;
+      sx[lj][li] = PL(pm.x1)[q];
+      sy[lj][li] = PL(pm.y1)[q];
+      double at = 0.0;
+      if constexpr (SA) {
+        double vt = U[4][q];
+        if (a.clip && interior && vt < 0) {
+          vt = 1.0e-10;
+          const_cast<double*>(U[4])[q] = vt;
+        }
+        svt[lj][li] = vt;
+        if (ring) {
+          const double Cv1_3 = 7.1 * 7.1 * 7.1;
+          double X = d * vt / am;
+          double X3 = X * X * X;
+          double fv1 = X3 / (X3 + Cv1_3);
+          at = fv1 * d * vt;
+        }
+      }
+      sAt[lj][li] = at;
+    }
+  }
+  __syncthreads();
+  if constexpr (SA) {
+    // Amut_boundary: wall ghost cells of layer 0 take -Amu_t of the adjacent interior cell
+    const double* flag = PL(pm.flag);
+    for (int idx = tid; idx < RW * RH; idx += NT) {
+      int li = idx % RW, lj = idx / RW;
+      int i = i0 - 2 + li, j = j0 - 2 + lj;
+      if (i < 0 || i > nx || j < 0 || j > ny) continue;
+      bool gi = (i == 0 || i == nx) && j >= 1 && j <= ny - 1;
+      bool gj = (j == 0 || j == ny) && i >= 1 && i <= nx - 1;
+      if (!(gi || gj)) continue;
+      if (flag[pidx(B, i, j)] == 0.0) continue;
+      int ii = li, jj = lj;
+      if (i == 0 && gi) ii = li + 1;
+      else if (gi) ii = li - 1;
+      else if (j == 0) jj = lj + 1;
+      else jj = lj - 1;
+      sAt[lj][li] = -sAt[jj][ii];
+    }
+    __syncthreads();
+  }
+
+  const bool fdm = B.fdm != 0;
+  const int LF = 4;
+  // ---- 2a. i-faces ---------------------------------------------------------
+  for (int f = tid; f < NFI; f += NT) {
+    int fi = f % (TX + 1), fj = f / (TX + 1);
+    int i = i0 + fi, j = j0 + fj;
+    if (i > nx || j > ny - 1) continue;
+    if (fdm && (i >= LF + 2 && i <= nx - LF - 1) && (j >= LF + 2 && j <= ny - LF - 1)) continue;
+    int64_t q = pidx(B, i, j);
+    double si = PL(pm.si)[q], ni1 = PL(pm.ni1)[q], ni2 = PL(pm.ni2)[q];
+    int lj = fj + 2, li = fi + 2;  // local index of cell (i, j); stencil li-2 .. li+1
+    double UL[4], UR[4];
+    if (a.o.recon == 0) {
+#define RECON(ARR, m)                                                                                           \
+  UL[m] = scheme_fP(ARR[lj][li - 2], ARR[lj][li - 1], ARR[lj][li], ARR[lj][li + 1], a.o.scheme);              \
+  UR[m] = scheme_fm(ARR[lj][li - 2], ARR[lj][li - 1], ARR[lj][li], ARR[lj][li + 1], a.o.scheme);
+      RECON(sd, 0) RECON(su, 1) RECON(sv, 2) RECON(sp, 3)
+#undef RECON
+    } else {
+      double U0[4][4];
+      for (int k = 0; k < 4; k++) {
+        int64_t qk = pidx(B, i - 2 + k, j);
+        for (int m = 0; m < 4; m++) U0[k][m] = U[m][qk];
+      }
+      if (a.o.recon == 1) {
+        double QL[4], QR[4];
+        for (int m = 0; m < 4; m++) {
+          QL[m] = scheme_fP(U0[0][m], U0[1][m], U0[2][m], U0[3][m], a.o.scheme);
+          QR[m] = scheme_fm(U0[0][m], U0[1][m], U0[2][m], U0[3][m], a.o.scheme);
+        }
+        cons_to_prim(QL, UL, g);
+        cons_to_prim(QR, UR, g);
+      } else {
+        recon_characteristic(U0, UL, UR, g, a.o.scheme);
+      }
+    }
+    double QL[4], QR[4], F0[4];
+    QL[0] = UL[0]; QL[1] = UL[1] * ni1 + UL[2] * ni2; QL[2] = -UL[1] * ni2 + UL[2] * ni1; QL[3] = UL[3];
+    QR[0] = UR[0]; QR[1] = UR[1] * ni1 + UR[2] * ni2; QR[2] = -UR[1] * ni2 + UR[2] * ni1; QR[3] = UR[3];
+    face_flux(a.o.flux, QL, QR, F0, g);
+    double F1 = -F0[0] * si;
+    double F2 = -(F0[1] * ni1 - F0[2] * ni2) * si;
+    double F3 = -(F0[1] * ni2 + F0[2] * ni1) * si;
+    double F4 = -F0[3] * si;
+    if constexpr (VISC) {
+      double Amu1, Amk1;
+      if (i == 1) {
+        Amu1 = sAm[lj][li] + sAt[lj][li];
+        Amk1 = cp.Cp * (sAm[lj][li] / cp.Pr + sAt[lj][li] / EC2D_PRT);
+      } else if (i == nx) {
+        Amu1 = sAm[lj][li - 1] + sAt[lj][li - 1];
+        Amk1 = cp.Cp * (sAm[lj][li - 1] / cp.Pr + sAt[lj][li - 1] / EC2D_PRT);
+      } else {
+        Amu1 = (sAm[lj][li - 1] + sAm[lj][li] + sAt[lj][li - 1] + sAt[lj][li]) * 0.5;
+        Amk1 = cp.Cp * ((sAm[lj][li - 1] + sAm[lj][li]) / cp.Pr + (sAt[lj][li - 1] + sAt[lj][li]) / EC2D_PRT) * 0.5;
+      }
+      double Dix = sx[lj][li] - sx[lj][li - 1];
+      double Diy = sy[lj][li] - sy[lj][li - 1];
+      double Djx = (sx[lj + 1][li - 1] + sx[lj + 1][li] - sx[lj - 1][li - 1] - sx[lj - 1][li]) * 0.25;
+      double Djy = (sy[lj + 1][li - 1] + sy[lj + 1][li] - sy[lj - 1][li - 1] - sy[lj - 1][li]) * 0.25;
+      double Ds = 1.0 / (Dix * Djy - Djx * Diy);
+      double Diu = su[lj][li] - su[lj][li - 1];
+      double Div = sv[lj][li] - sv[lj][li - 1];
+      double DiT = sT[lj][li] - sT[lj][li - 1];
+      double Dju = (su[lj + 1][li - 1] + su[lj + 1][li] - su[lj - 1][li - 1] - su[lj - 1][li]) * 0.25;
+      double Djv = (sv[lj + 1][li - 1] + sv[lj + 1][li] - sv[lj - 1][li - 1] - sv[lj - 1][li]) * 0.25;
+      double DjT = (sT[lj + 1][li - 1] + sT[lj + 1][li] - sT[lj - 1][li - 1] - sT[lj - 1][li]) * 0.25;
+      double ux = (Diu * Djy - Dju * Diy) * Ds, vx = (Div * Djy - Djv * Diy) * Ds, Tx = (DiT * Djy - DjT * Diy) * Ds;
+      double uy = (-Diu * Djx + Dju * Dix) * Ds, vy = (-Div * Djx + Djv * Dix) * Ds, Ty = (-DiT * Djx + DjT * Dix) * Ds;
+      double u1 = (su[lj][li] + su[lj][li - 1]) * 0.5;
+      double v1 = (sv[lj][li] + sv[lj][li - 1]) * 0.5;
+      double t11 = ((4.0 / 3.0) * ux - (2.0 / 3.0) * vy) * Amu1;
+      double t22 = ((4.0 / 3.0) * vy - (2.0 / 3.0) * ux) * Amu1;
+      double t12 = (uy + vx) * Amu1;
+      double E1 = u1 * t11 + v1 * t12 + Amk1 * Tx;
+      double E2 = u1 * t12 + v1 * t22 + Amk1 * Ty;
+      F2 = F2 + (t11 * ni1 + t12 * ni2) * si;
+      F3 = F3 + (t12 * ni1 + t22 * ni2) * si;
+      F4 = F4 + (E1 * ni1 + E2 * ni2) * si;
+      if constexpr (SA) {
+        const double Cb2 = 0.622, SA_sigma = 2.0 / 3.0;
+        const double c_diff = 0.5 * (1.0 + Cb2) / SA_sigma;
+        double vn1 = su[lj][li - 1] * ni1 + sv[lj][li - 1] * ni2;
+        double vn2 = su[lj][li] * ni1 + sv[lj][li] * ni2;
+        double vfi = -0.5 * ((vn1 + fabs(vn1)) * svt[lj][li - 1] + (vn2 - fabs(vn2)) * svt[lj][li]) * si;
+        double v0 = c_diff * (svt[lj][li] + sAm[lj][li] / sd[lj][li] + svt[lj][li - 1] + sAm[lj][li - 1] / sd[lj][li - 1]);
+        double Dv = svt[lj][li] - svt[lj][li - 1];
+        double Djvt = (svt[lj + 1][li - 1] + svt[lj + 1][li] - svt[lj - 1][li - 1] - svt[lj - 1][li]) * 0.25;
+        double vtx = (Dv * Djy - Djvt * Diy) * Ds;
+        double vty = (-Dv * Djx + Djvt * Dix) * Ds;
+        sFv[0][fj][fi] = vfi + v0 * (vtx * ni1 + vty * ni2) * si;
+        sFv[1][fj][fi] = (vtx * ni1 + vty * ni2) * si;
+      }
+    }
+    sFi[0][fj][fi] = F1;
+    sFi[1][fj][fi] = F2;
+    sFi[2][fj][fi] = F3;
+    sFi[3][fj][fi] = F4;
+  }
+  // ---- 2b. j-faces ---------------------------------------------------------
+  for (int f = tid; f < NFJ; f += NT) {
+    int fi = f % TX, fj = f / TX;
+    int i = i0 + fi, j = j0 + fj;
+    if (i > nx - 1 || j > ny) continue;
+    if (fdm && (i >= LF + 2 && i <= nx - LF - 1) && (j >= LF + 2 && j <= ny - LF - 1)) continue;
+    int64_t q = pidx(B, i, j);
+    double sj = PL(pm.sj)[q], nj1 = PL(pm.nj1)[q], nj2 = PL(pm.nj2)[q];
+    int lj = fj + 2, li = fi + 2;
+    double UL[4], UR[4];
+    if (a.o.recon == 0) {
+      // the j-direction original-variable reconstruction uses the global
+      // scheme option, not the mesh's own (sub_Residual.f90:367)
+#define RECON(ARR, m)                                                                                              \
+  UL[m] = scheme_fP(ARR[lj - 2][li], ARR[lj - 1][li], ARR[lj][li], ARR[lj + 1][li], a.o.global_scheme);          \
+  UR[m] = scheme_fm(ARR[lj - 2][li], ARR[lj - 1][li], ARR[lj][li], ARR[lj + 1][li], a.o.global_scheme);
+      RECON(sd, 0) RECON(su, 1) RECON(sv, 2) RECON(sp, 3)
+#undef RECON
+    } else {
+      double U0[4][4];
+      for (int k = 0; k < 4; k++) {
+        int64_t qk = pidx(B, i, j - 2 + k);
+        for (int m = 0; m < 4; m++) U0[k][m] = U[m][qk];
+      }
+      if (a.o.recon == 1) {
+        double QL[4], QR[4];
+        for (int m = 0; m < 4; m++) {
+          QL[m] = scheme_fP(U0[0][m], U0[1][m], U0[2][m], U0[3][m], a.o.scheme);
+          QR[m] = scheme_fm(U0[0][m], U0[1][m], U0[2][m], U0[3][m], a.o.scheme);
+        }
+        cons_to_prim(QL, UL, g);
+        cons_to_prim(QR, UR, g);
+      } else {
+        recon_characteristic(U0, UL, UR, g, a.o.scheme);
+      }
+    }
+    double QL[4], QR[4], F0[4];
+    QL[0] = UL[0]; QL[1] = UL[1] * nj1 + UL[2] * nj2; QL[2] = -UL[1] * nj2 + UL[2] * nj1; QL[3] = UL[3];
+    QR[0] = UR[0]; QR[1] = UR[1] * nj1 + UR[2] * nj2; QR[2] = -UR[1] * nj2 + UR[2] * nj1; QR[3] = UR[3];
+    face_flux(a.o.flux, QL, QR, F0, g);
+    double F1 = -F0[0] * sj;
+    double F2 = -(F0[1] * nj1 - F0[2] * nj2) * sj;
+    double F3 = -(F0[1] * nj2 + F0[2] * nj1) * sj;
+    double F4 = -F0[3] * sj;
+    if constexpr (VISC) {
+      double Amu1, Amk1;
+      if (j == 1) {
+        Amu1 = sAm[lj][li] + sAt[lj][li];
+        Amk1 = cp.Cp * (sAm[lj][li] / cp.Pr + sAt[lj][li] / EC2D_PRT);
+      } else if (j == ny) {
+        Amu1 = sAm[lj - 1][li] + sAt[lj - 1][li];
+        Amk1 = cp.Cp * (sAm[lj - 1][li] / cp.Pr + sAt[lj - 1][li] / EC2D_PRT);
+      } else {
+        Amu1 = (sAm[lj][li] + sAm[lj - 1][li] + sAt[lj][li] + sAt[lj - 1][li]) * 0.5;
+        Amk1 = cp.Cp * ((sAm[lj - 1][li] + sAm[lj][li]) / cp.Pr + (sAt[lj - 1][li] + sAt[lj][li]) / EC2D_PRT) * 0.5;
+      }
+      double Dix = (sx[lj - 1][li + 1] + sx[lj][li + 1] - sx[lj - 1][li - 1] - sx[lj][li - 1]) * 0.25;
+      double Diy = (sy[lj - 1][li + 1] + sy[lj][li + 1] - sy[lj - 1][li - 1] - sy[lj][li - 1]) * 0.25;
+      double Djx = sx[lj][li] - sx[lj - 1][li];
+      double Djy = sy[lj][li] - sy[lj - 1][li];
+      double Ds = 1.0 / (Dix * Djy - Djx * Diy);
+      double Diu = (su[lj - 1][li + 1] + su[lj][li + 1] - su[lj - 1][li - 1] - su[lj][li - 1]) * 0.25;
+      double Div = (sv[lj - 1][li + 1] + sv[lj][li + 1] - sv[lj - 1][li - 1] - sv[lj][li - 1]) * 0.25;
+      double DiT = (sT[lj - 1][li + 1] + sT[lj][li + 1] - sT[lj - 1][li - 1] - sT[lj][li - 1]) * 0.25;
+      double Dju = su[lj][li] - su[lj - 1][li];
+      double Djv = sv[lj][li] - sv[lj - 1][li];
+      double DjT = sT[lj][li] - sT[lj - 1][li];
+      double ux = (Diu * Djy - Dju * Diy) * Ds, vx = (Div * Djy - Djv * Diy) * Ds, Tx = (DiT * Djy - DjT * Diy) * Ds;
+      double uy = (-Diu * Djx + Dju * Dix) * Ds, vy = (-Div * Djx + Djv * Dix) * Ds, Ty = (-DiT * Djx + DjT * Dix) * Ds;
+      double t11 = ((4.0 / 3.0) * ux - (2.0 / 3.0) * vy) * Amu1;
+      double t22 = ((4.0 / 3.0) * vy - (2.0 / 3.0) * ux) * Amu1;
+      double t12 = (uy + vx) * Amu1;
+      double u1 = (su[lj][li] + su[lj - 1][li]) * 0.5;
+      double v1 = (sv[lj][li] + sv[lj - 1][li]) * 0.5;
+      double E1 = u1 * t11 + v1 * t12 + Amk1 * Tx;
+      double E2 = u1 * t12 + v1 * t22 + Amk1 * Ty;
+      F2 = F2 + (t11 * nj1 + t12 * nj2) * sj;
+      F3 = F3 + (t12 * nj1 + t22 * nj2) * sj;
+      F4 = F4 + (E1 * nj1 + E2 * nj2) * sj;
+      if constexpr (SA) {
+        const double Cb2 = 0.622, SA_sigma = 2.0 / 3.0;
+        const double c_diff = 0.5 * (1.0 + Cb2) / SA_sigma;
+        double vn1 = su[lj - 1][li] * nj1 + sv[lj - 1][li] * nj2;
+        double vn2 = su[lj][li] * nj1 + sv[lj][li] * nj2;
+        double vfi = -0.5 * ((vn1 + fabs(vn1)) * svt[lj - 1][li] + (vn2 - fabs(vn2)) * svt[lj][li]) * sj;
+        double v0 = c_diff * (svt[lj][li] + sAm[lj][li] / sd[lj][li] + svt[lj - 1][li] + sAm[lj - 1][li] / sd[lj - 1][li]);
+        double Div2 = (svt[lj - 1][li + 1] + svt[lj][li + 1] - svt[lj - 1][li - 1] - svt[lj][li - 1]) * 0.25;
+        double Djv2 = svt[lj][li] - svt[lj - 1][li];
+        double vtx = (Div2 * Djy - Djv2 * Diy) * Ds;
+        double vty = (-Div2 * Djx + Djv2 * Dix) * Ds;
+        sFw[0][fj][fi] = vfi + v0 * (vtx * nj1 + vty * nj2) * sj;
+        sFw[1][fj][fi] = (vtx * nj1 + vty * nj2) * sj;
+      }
+    }
+    sFj[0][fj][fi] = F1;
+    sFj[1][fj][fi] = F2;
+    sFj[2][fj][fi] = F3;
+    sFj[3][fj][fi] = F4;
+  }
+  __syncthreads();
+
+  // ---- 3. cells ------------------------------------------------------------
+  double rmax[NV], rsum[NV];
+#pragma unroll
+  for (int m = 0; m < NV; m++) { rmax[m] = 0.0; rsum[m] = 0.0; }
+  {
+    int ci = tid % TX, cj = tid / TX;
+    int i = i0 + ci, j = j0 + cj;
+    if (i <= nx - 1 && j <= ny - 1) {
+      int64_t q = pidx(B, i, j);
+      int li = ci + 2, lj = cj + 2;
+      double R[NV];
+      bool fdm_cell = fdm && (i >= LF + 1 && i <= nx - LF - 1) && (j >= LF + 1 && j <= ny - LF - 1);
+      if (!fdm_cell) {
+#pragma unroll
+        for (int m = 0; m < 4; m++) R[m] = sFi[m][cj][ci + 1] - sFi[m][cj][ci] + sFj[m][cj + 1][ci] - sFj[m][cj][ci];
+      }
+      if constexpr (SA) {
+        const double SA_sigma = 2.0 / 3.0, Cv1 = 7.1, Cb1 = 0.1355, Cb2 = 0.622, SA_k = 0.41;
+        const double Cw1 = Cb1 / (SA_k * SA_k) + (1.0 + Cb2) / SA_sigma, Cw2 = 0.3, Ct3 = 1.3, Ct4 = 0.5;
+        const double Cv1_3 = Cv1 * Cv1 * Cv1, Cw3_6 = 2.0 * 2.0 * 2.0 * 2.0 * 2.0 * 2.0;
+        double vtc = svt[lj][li];
+        double v0 = (vtc + sAm[lj][li] / sd[lj][li]) * Cb2 / SA_sigma;
+        double r5 = sFv[0][cj][ci + 1] - sFv[0][cj][ci] - v0 * (sFv[1][cj][ci + 1] - sFv[1][cj][ci]);
+        r5 = r5 + sFw[0][cj + 1][ci] - sFw[0][cj][ci] - v0 * (sFw[1][cj + 1][ci] - sFw[1][cj][ci]);
+        double Dix = (sx[lj][li + 1] - sx[lj][li - 1]) * 0.5;
+        double Diy = (sy[lj][li + 1] - sy[lj][li - 1]) * 0.5;
+        double Djx = (sx[lj + 1][li] - sx[lj - 1][li]) * 0.5;
+        double Djy = (sy[lj + 1][li] - sy[lj - 1][li]) * 0.5;
+        double Ds = 1.0 / (Dix * Djy - Djx * Diy);
+        double Diu = (su[lj][li + 1] - su[lj][li - 1]) * 0.5;
+        double Div = (sv[lj][li + 1] - sv[lj][li - 1]) * 0.5;
+        double Dju = (su[lj + 1][li] - su[lj - 1][li]) * 0.5;
+        double Djv = (sv[lj + 1][li] - sv[lj - 1][li]) * 0.5;
+        double vx = (Div * Djy - Djv * Diy) * Ds;
+        double uy = (-Diu * Djx + Dju * Dix) * Ds;
+        double S = fabs(vx - uy);
+        double dwc = PL(pm.dw)[q];
+        double X = sd[lj][li] * vtc / sAm[lj][li];
+        double X3 = X * X * X;
+        double fv1 = X3 / (X3 + Cv1_3);
+        double fv2 = 1.0 - X / (1.0 + X * fv1);
+        double kd = SA_k * dwc;
+        double S1 = fmax(S + fv2 * vtc / (kd * kd), EC2D_SP(0.3) * S);
+        double r = fmin(vtc / (S1 * SA_k * SA_k * dwc * dwc), 10.0);
+        double gg = r + Cw2 * (r * r * r * r * r * r - r);
+        double fw = gg * pow((1.0 + Cw3_6) / (gg * gg * gg * gg * gg * gg + Cw3_6), 1.0 / 6.0);
+        double ft2 = Ct3 * exp(-(Ct4 * X * X));
+        double qv = vtc / dwc;
+        double Q_SA = Cb1 * (1.0 - ft2) * S1 * vtc - (Cw1 * fw - Cb1 * ft2 / (SA_k * SA_k)) * (qv * qv);
+        r5 = r5 + Q_SA * PL(pm.vol)[q];
+        R[4] = r5;
+      }
+      if (!fdm_cell) {
+#pragma unroll
+        for (int m = 0; m < 4; m++) PL(pm.Res + m)[q] = R[m];
+      }
+      if constexpr (SA) PL(pm.Res + 4)[q] = R[4];
+      // comput_Lvc
+      int64_t qi1 = pidx(B, i + 1, j), qj1 = pidx(B, i, j + 1);
+      double s0 = PL(pm.vol)[q];
+      double si = 0.5 * (PL(pm.si)[q] + PL(pm.si)[qi1]);
+      double ni1 = 0.5 * (PL(pm.ni1)[q] + PL(pm.ni1)[qi1]);
+      double ni2 = 0.5 * (PL(pm.ni2)[q] + PL(pm.ni2)[qi1]);
+      double sj = 0.5 * (PL(pm.sj)[q] + PL(pm.sj)[qj1]);
+      double nj1 = 0.5 * (PL(pm.nj1)[q] + PL(pm.nj1)[qj1]);
+      double nj2 = 0.5 * (PL(pm.nj2)[q] + PL(pm.nj2)[qj1]);
+      double uni = su[lj][li] * ni1 + sv[lj][li] * ni2;
+      double unj = su[lj][li] * nj1 + sv[lj][li] * nj2;
+      double T = (U[3][q] - 0.5 * sd[lj][li] * (su[lj][li] * su[lj][li] + sv[lj][li] * sv[lj][li])) / (cp.Cv * sd[lj][li]);
+      double cc = sqrt(T) / cp.Ma;
+      double Lci = (fabs(uni) + cc) * si;
+      double Lcj = (fabs(unj) + cc) * sj;
+      double Lvi = 0.0, Lvj = 0.0;
+      if constexpr (VISC) {
+        double tmp1 = g / sd[lj][li] * (sAm[lj][li] / cp.Pr + sAt[lj][li] / EC2D_PRT);
+        Lvi = tmp1 * si * si / s0;
+        Lvj = tmp1 * sj * sj / s0;
+        PL(pm.Lvi)[q] = Lvi;
+        PL(pm.Lvj)[q] = Lvj;
+      }
+      PL(pm.Lci)[q] = Lci;
+      PL(pm.Lcj)[q] = Lcj;
+      double dt;
+      if (cp.Iflag_local_dt == 0) {
+        dt = cp.dt_global;
+      } else {
+        dt = cp.CFL * s0 / (Lci + Lcj + 1.0 * (Lvi + Lvj));
+        if (dt > cp.dtmax) dt = cp.dtmax;
+        if (dt < cp.dtmin) dt = cp.dtmin;
+      }
+      PL(pm.dt)[q] = dt;
+      if (!fdm_cell) {
+#pragma unroll
+        for (int m = 0; m < NV; m++) {
+          rmax[m] = fabs(R[m]);
+          rsum[m] = R[m] * R[m];
+        }
+      } else if constexpr (SA) {
+        rmax[4] = fabs(R[4]);
+        rsum[4] = R[4] * R[4];
+      }
+      if constexpr (VISC) {
+        if (a.write_visc) {
+          PL(pm.Amu)[q] = sAm[lj][li];
+          PL(pm.Amu_t)[q] = sAt[lj][li];
+        }
+      }
+    }
+  }
+  // ---- tile reduction (fixed tree: wave butterfly, then waves in order) ----
+#pragma unroll
+  for (int m = 0; m < NV; m++) {
+    double mx = rmax[m], sm = rsum[m];
+    for (int off = 32; off > 0; off >>= 1) {
+      mx = fmax(mx, __shfl_xor(mx, off));
+      sm = sm + __shfl_xor(sm, off);
+    }
+    if ((tid & 63) == 0) {
+      sred[2 * m][tid >> 6] = mx;
+      sred[2 * m + 1][tid >> 6] = sm;
+    }
+  }
+  __syncthreads();
+  if (tid < 2 * NV) {
+    int m = tid >> 1;
+    double acc = sred[tid][0];
+    for (int w = 1; w < NT / 64; w++) acc = (tid & 1) ? acc + sred[tid][w] : fmax(acc, sred[tid][w]);
+    a.partials[(int64_t)lt * 2 * NV + tid] = acc;
+    (void)m;
+  }
+}
+
+// Deterministic second stage: one workgroup per case reduces its tiles in
+// logical order into Res_max(1:NV) and Res_rms(1:NV) = sqrt(sum/Num_Cell).
+template <int NV>
+__global__ void k_norms(const double* partials, const int* case_tile0, const double* ncell, double* norms) {
+  int c = blockIdx.x;
+  int t0 = case_tile0[c], t1 = case_tile0[c + 1];
+  __shared__ double sh[256];
+  for (int k = 0; k < 2 * NV; k++) {
+    double acc = 0.0;
+    bool is_sum = k & 1;
+    for (int t = t0 + threadIdx.x; t < t1; t += blockDim.x) {
+      double v = partials[(int64_t)t * 2 * NV + k];
+      acc = is_sum ? acc + v : fmax(acc, v);
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+      if (threadIdx.x < s) sh[threadIdx.x] = is_sum ? sh[threadIdx.x] + sh[threadIdx.x + s] : fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      int m = k >> 1;
+      if (is_sum) norms[c * 12 + 6 + m] = sqrt(sh[0] / ncell[c]);
+      else norms[c * 12 + m] = sh[0];
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace ec2d

@@ -1,0 +1,250 @@
+"""Host-side mirror of the OpenCFD-EC2D driver on top of the C-ABI.
+
+`Batch` is the set of cases one process solves; each `Case` is one reference
+run (one case directory in the reference's file formats).  The methods keep
+the reference's subroutine names and argument meaning, so a parity test reads
+like the reference's main loop:
+
+    b = Batch([Case.from_dir(d)])
+    b.init_flow()                           # Init_flow
+    b.ns_time_advance(1)                    # NS_Time_advance(1)
+    rmax, rrms = b.res_norms(0)             # Mesh(1)%Res_max, Res_rms
+
+All numerics run in libec2d_hip.so; this module only reads files, builds the
+reference's geometry on the host (ec2d.geometry) and moves arrays.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import List, Optional
+
+import numpy as np
+
+from . import capi
+from . import formats as F
+from . import geometry as G
+
+_dp = C.POINTER(C.c_double)
+
+
+def _ptr(a: Optional[np.ndarray]):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_dp)
+
+
+class Case:
+    """One reference case: control parameters plus host geometry of every level."""
+
+    def __init__(self, control: F.Control, levels: List[List[G.BlockGeom]], case_dir: Optional[str] = None):
+        self.ctl = control
+        self.levels = levels
+        self.dir = case_dir
+        self.id = -1
+
+    @classmethod
+    def from_dir(cls, case_dir: str) -> "Case":
+        ctl = F.read_control(os.path.join(case_dir, "control.in"))
+        xs, ys = F.read_mesh2d(os.path.join(case_dir, "Mesh2d.dat"))
+        bcs = F.read_bcin(os.path.join(case_dir, "bc2d_1.in"))
+        fdm = F.read_fdm_in(os.path.join(case_dir, "FDM.in"))
+        need_dw = ctl.Iflag_turbulence_model in (F.TURB_SA, F.TURB_SST)
+        levels = G.build_levels(xs, ys, bcs, ctl.Num_Mesh, fdm, need_dw)
+        return cls(ctl, levels, case_dir)
+
+    @classmethod
+    def from_arrays(cls, ctl: F.Control, xs, ys, bcs, fdm=(), case_dir=None) -> "Case":
+        need_dw = ctl.Iflag_turbulence_model in (F.TURB_SA, F.TURB_SST)
+        return cls(ctl, G.build_levels(xs, ys, bcs, ctl.Num_Mesh, fdm, need_dw), case_dir)
+
+    @property
+    def num_mesh(self) -> int:
+        return len(self.levels)
+
+    def c_control(self) -> capi.Control:
+        c = self.ctl
+        k = capi.Control()
+        k.Ma, k.Re, k.gamma, k.AoA, k.Pr, k.Cv, k.Cp = c.Ma, c.Re, c.gamma, c.aoa_rad, c.Pr, c.Cv, c.Cp
+        k.t_end, k.dt_global, k.CFL, k.dtmax, k.dtmin = c.t_end, c.dt_global, c.CFL, c.dtmax, c.dtmin
+        k.P_outlet, k.T_inf, k.Twall = c.P_outlet, c.T_inf, c.Twall_nd
+        k.vt_inf, k.Kt_inf, k.Wt_inf = c.vt_inf, c.Kt_inf, c.Wt_inf
+        k.If_viscous, k.Iflag_turbulence_model, k.Iflag_init = c.If_viscous, c.Iflag_turbulence_model, c.Iflag_init
+        k.Iflag_local_dt, k.Time_Method = c.Iflag_local_dt, c.Time_Method
+        k.Iflag_Scheme, k.Iflag_Flux, k.IFlag_Reconstruction = c.Iflag_Scheme, c.Iflag_Flux, c.IFlag_Reconstruction
+        k.Num_Mesh, k.Nvar = self.num_mesh, c.nvar
+        for n in range(3):
+            k.Pre_Step_Mesh[n] = int(c.Pre_Step_Mesh[n])
+        return k
+
+    def read_restart(self) -> List[np.ndarray]:
+        """U on the finest level from flow2d.dat (init_flow_read, sub_init.f90:509-523)."""
+        c = self.ctl
+        out = []
+        with open(os.path.join(self.dir, "flow2d.dat")) as f:
+            f.readline()
+            for B in self.levels[0]:
+                f.readline()
+                n = (B.nx + 1) * (B.ny + 1)
+                q = np.array([[F._fnum(t) for t in f.readline().split()[2:6]] for _ in range(n)])
+                q = q.reshape(B.ny + 1, B.nx + 1, 4)
+                d0, u0, v0, T0 = q[..., 0], q[..., 1], q[..., 2], q[..., 3]
+                U = np.zeros((B.ny + 3, B.nx + 3, c.nvar))
+                U[..., 0] = 1.0
+                U[..., 3] = 1.0
+                sl = (slice(1, B.ny + 2), slice(1, B.nx + 2))
+                U[sl + (0,)] = d0
+                U[sl + (1,)] = d0 * u0
+                U[sl + (2,)] = d0 * v0
+                U[sl + (3,)] = d0 * c.Cv * T0 + 0.5 * d0 * (u0 * u0 + v0 * v0)
+                out.append(U)
+        return out
+
+
+class Batch:
+    """All cases of one process; every solver call acts on all of them."""
+
+    def __init__(self, cases: List[Case], tile_rows: int = 4):
+        self.L = capi.load()
+        capi.call("ec2d_init")
+        if tile_rows != 4:
+            capi.call("ec2d_set_tile_rows", tile_rows)
+        self.cases = cases
+        for c in cases:
+            cid = self.L.ec2d_case_create(C.byref(c.c_control()))
+            if cid < 0:
+                raise capi.Ec2dError(capi.last_error())
+            c.id = cid
+            for lev, blocks in enumerate(c.levels):
+                for mb, B in enumerate(blocks):
+                    bc = np.ascontiguousarray(np.array(B.bc, dtype=np.int32).ravel())
+                    fd = None
+                    if B.fvm_fdm == 1 and lev == 0:
+                        fd = np.ascontiguousarray(np.stack([B.fd[k] for k in ("Akx", "Aky", "Aix", "Aiy", "Ajac")]))
+                    arrs = [np.ascontiguousarray(getattr(B, n)) for n in
+                            ("x", "y", "x1", "y1", "vol", "si", "sj", "ni1", "ni2", "nj1", "nj2")]
+                    dw = np.ascontiguousarray(B.dw) if (B.dw is not None and lev == 0) else None
+                    capi.call("ec2d_define_block", cid, lev + 1, mb + 1, B.nx, B.ny, len(B.bc),
+                              bc.ctypes.data_as(C.POINTER(C.c_int)), int(B.fvm_fdm if lev == 0 else 0),
+                              *[_ptr(a) for a in arrs], _ptr(dw), _ptr(fd))
+        capi.call("ec2d_commit")
+        self.num_mesh = cases[0].num_mesh
+
+    # ---- arrays (reference layout, numpy C order = Fortran memory order) ---
+    _SHAPES = {
+        capi.EC2D_U: lambda B, nv: (B.ny + 3, B.nx + 3, nv),
+        capi.EC2D_RES: lambda B, nv: (B.ny - 1, B.nx - 1, nv),
+        capi.EC2D_DT: lambda B, nv: (B.ny - 1, B.nx - 1),
+        capi.EC2D_DU: lambda B, nv: (B.ny, B.nx, nv),
+        capi.EC2D_LCI: lambda B, nv: (B.ny, B.nx),
+        capi.EC2D_LCJ: lambda B, nv: (B.ny, B.nx),
+        capi.EC2D_LVI: lambda B, nv: (B.ny, B.nx),
+        capi.EC2D_LVJ: lambda B, nv: (B.ny, B.nx),
+        capi.EC2D_AMU: lambda B, nv: (B.ny + 3, B.nx + 3),
+        capi.EC2D_AMU_T: lambda B, nv: (B.ny + 3, B.nx + 3),
+        capi.EC2D_UN: lambda B, nv: (B.ny + 1, B.nx + 1, nv),
+        capi.EC2D_QF: lambda B, nv: (B.ny - 1, B.nx - 1, 4),
+        capi.EC2D_DELTU: lambda B, nv: (B.ny + 3, B.nx + 3, 4),
+    }
+    NAMES = {"U": capi.EC2D_U, "Res": capi.EC2D_RES, "dt": capi.EC2D_DT, "dU": capi.EC2D_DU, "Lci": capi.EC2D_LCI,
+             "Lcj": capi.EC2D_LCJ, "Lvi": capi.EC2D_LVI, "Lvj": capi.EC2D_LVJ, "Amu": capi.EC2D_AMU,
+             "Amu_t": capi.EC2D_AMU_T, "Un": capi.EC2D_UN, "QF": capi.EC2D_QF, "deltU": capi.EC2D_DELTU}
+
+    def _nv(self, case: Case, nmesh: int) -> int:
+        return case.ctl.nvar if nmesh == 1 else 4
+
+    def get(self, name: str, case: int = 0, nmesh: int = 1, mblock: int = 1) -> np.ndarray:
+        c = self.cases[case]
+        B = c.levels[nmesh - 1][mblock - 1]
+        w = self.NAMES[name]
+        out = np.zeros(self._SHAPES[w](B, self._nv(c, nmesh)))
+        capi.call("ec2d_get_array", c.id, nmesh, mblock, w, _ptr(out))
+        return out
+
+    def put(self, name: str, arr: np.ndarray, case: int = 0, nmesh: int = 1, mblock: int = 1):
+        c = self.cases[case]
+        B = c.levels[nmesh - 1][mblock - 1]
+        w = self.NAMES[name]
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        assert a.shape == self._SHAPES[w](B, self._nv(c, nmesh)), (name, a.shape)
+        capi.call("ec2d_put_array", c.id, nmesh, mblock, w, _ptr(a))
+
+    # ---- the reference's subroutines ------------------------------------
+    def init_flow(self):
+        """Init_flow (sub_init.f90:383-414)."""
+        if self.cases[0].ctl.Iflag_init != 0:
+            for c in self.cases:
+                for mb, U in enumerate(c.read_restart()):
+                    self.put("U", U, c.id, 1, mb + 1)
+        capi.call("ec2d_init_flow")
+
+    def comput_residual_one_mesh(self, nmesh=1):
+        capi.call("ec2d_comput_residual_one_mesh", nmesh)
+
+    def du_lu_sgs_2d(self, nmesh=1, mblock=0, alfa1=0.0):
+        capi.call("ec2d_du_lu_sgs_2d", nmesh, mblock, alfa1)
+
+    def update_u(self, nmesh=1):
+        capi.call("ec2d_update_u", nmesh)
+
+    def boundary_condition_onemesh(self, nmesh=1):
+        capi.call("ec2d_boundary_condition_onemesh", nmesh)
+
+    def update_buffer_onemesh(self, nmesh=1):
+        capi.call("ec2d_update_buffer_onemesh", nmesh)
+
+    def force_vt_kw(self, nmesh=1):
+        capi.call("ec2d_force_vt_kw", nmesh)
+
+    def ns_time_advance(self, nmesh=1):
+        capi.call("ec2d_ns_time_advance", nmesh)
+
+    def ns_multigrid(self):
+        capi.call("ec2d_ns_multigrid")
+
+    def main_step(self):
+        """One pass of the main loop body (opencfd_ec2d_v1.5.4.f90:174-180)."""
+        if self.num_mesh == 1:
+            self.ns_time_advance(1)
+        else:
+            self.ns_multigrid()
+
+    def run(self, n_iter: int):
+        capi.call("ec2d_run", int(n_iter))
+
+    def res_norms(self, case: int = 0, nmesh: int = 1):
+        rmax = np.zeros(6)
+        rrms = np.zeros(6)
+        capi.call("ec2d_res_norms", self.cases[case].id, nmesh, _ptr(rmax), _ptr(rrms))
+        return rmax, rrms
+
+    def time(self, case: int = 0, nmesh: int = 1):
+        k = C.c_int()
+        t = C.c_double()
+        capi.call("ec2d_time", self.cases[case].id, nmesh, C.byref(k), C.byref(t))
+        return k.value, t.value
+
+    def num_cells(self, case: int = -1, nmesh: int = 1) -> int:
+        return int(self.L.ec2d_num_cells(case, nmesh))
+
+    def synchronize(self):
+        capi.call("ec2d_synchronize")
+
+    def set_profile(self, on: bool):
+        capi.call("ec2d_set_profile", 1 if on else 0)
+
+    def timing(self):
+        v = [C.c_double() for _ in range(4)]
+        capi.call("ec2d_timing", *[C.byref(x) for x in v])
+        return {"run_ms": v[0].value, "residual_ms": v[1].value, "lusgs_ms": v[2].value, "ghost_ms": v[3].value}
+
+    def close(self):
+        capi.call("ec2d_finalize")
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+

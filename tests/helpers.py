@@ -1,0 +1,48 @@
+"""Shared helpers: small synthetic cases and GPU-vs-oracle comparison."""
+import os
+import tempfile
+
+import numpy as np
+
+from ec2d import cases
+from ec2d import formats as F
+
+CASE_ROOT = os.environ.get("EC2D_TEST_CASES", os.path.join(tempfile.gettempdir(), "ec2d_test_cases"))
+
+
+def case_dir(name: str) -> str:
+    d = os.path.join(CASE_ROOT, name)
+    os.makedirs(d, exist_ok=True)
+    return d
+
+
+def naca_case(name, ctl: F.Control, nx=65, ny=17, ds1=1e-3, fdm=(), **kw):
+    x, y, bc, _ = cases.naca_cgrid(nx, ny, ds1=ds1, **kw)
+    return cases.write_case(case_dir(name), ctl, [x], [y], [bc], fdm_blocks=fdm)
+
+
+def vortex_case(name, ctl: F.Control, seed=3, nbx=3, nby=2, wmin=9, wmax=21, hmin=9, hmax=17, h=0.25, bc_edit=None):
+    from ec2d import geometry as G
+
+    xs, ys, bcs, field, _ = cases.vortex_multiblock(seed=seed, nbx=nbx, nby=nby, wmin=wmin, wmax=wmax, hmin=hmin,
+                                                   hmax=hmax, h=h)
+    if bc_edit:
+        bc_edit(bcs)
+    lev = G.build_levels(xs, ys, bcs, 1)
+    flow = ([b.x1[1:-1, 1:-1] for b in lev[0]], [b.y1[1:-1, 1:-1] for b in lev[0]],
+            [field(b.x1[1:-1, 1:-1], b.y1[1:-1, 1:-1]) for b in lev[0]])
+    return cases.write_case(case_dir(name), ctl, xs, ys, bcs, flow=flow)
+
+
+def first_diff(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    if a.shape != b.shape:
+        return f"shape {a.shape} vs {b.shape}"
+    bad = np.argwhere(~((a == b) | (np.isnan(a) & np.isnan(b))))
+    if bad.size == 0:
+        return None
+    k = tuple(bad[0])
+    with np.errstate(divide="ignore", invalid="ignore"):
+        rel = np.nanmax(np.abs(a - b) / np.maximum(np.abs(b), 1e-300))
+    return f"{len(bad)} differ, first at {k}: {a[k]!r} vs {b[k]!r}, max rel {rel:.3e}"

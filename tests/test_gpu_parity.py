@@ -1,0 +1,168 @@
+"""GPU path vs the C oracle, on identical inputs.
+
+Bar: bit-exact FP64 state (U), residual, time step and spectral radii for the
+Euler, laminar and multigrid paths; the residual norms (whose summation order
+differs: a fixed tree on the GPU, a serial loop in the reference) within
+1e-12 relative.  The SA path evaluates exp and pow with the GPU's math library
+instead of glibc, so it is held to 1e-12 relative per iteration instead of
+bit equality.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+import oracle as O
+from ec2d import cases
+from ec2d import formats as F
+from ec2d import solver
+from helpers import first_diff, naca_case, vortex_case
+
+pytestmark = pytest.mark.gpu
+
+
+def _compare_steps(d, nsteps, arrays=("U", "Res", "dt", "Lci", "Lcj"), exact=True, rtol=1e-12, option=None,
+                   tile_rows=4):
+    ref = O.OracleCase(d)
+    ref.init_flow()
+    case = solver.Case.from_dir(d)
+    with solver.Batch([case], tile_rows=tile_rows) as b:
+        b.init_flow()
+        for mb in range(1, len(case.levels[0]) + 1):
+            diff = first_diff(b.get("U", 0, 1, mb), ref.block(1, mb).U)
+            assert diff is None, f"initial U block {mb}: {diff}"
+        for k in range(nsteps):
+            ref.main_step()
+            b.main_step()
+            for mb in range(1, len(case.levels[0]) + 1):
+                rb = ref.block(1, mb)
+                for a in arrays:
+                    g = b.get(a, 0, 1, mb)
+                    r = getattr(rb, a)
+                    if a in ("Lci", "Lcj", "Lvi", "Lvj"):
+                        g, r = g[:-1, :-1], r[:-1, :-1]
+                    if exact:
+                        diff = first_diff(g, r)
+                        assert diff is None, f"step {k} block {mb} {a}: {diff}"
+                    else:
+                        np.testing.assert_allclose(g, r, rtol=rtol, atol=1e-300, err_msg=f"step {k} block {mb} {a}")
+            gmax, grms = b.res_norms()
+            rmax, rrms = ref.norms()
+            nv = case.ctl.nvar
+            np.testing.assert_allclose(grms[:nv], rrms[:nv], rtol=1e-12, atol=0)
+            np.testing.assert_allclose(gmax[:nv], rmax[:nv], rtol=1e-12 if not exact else 0, atol=0)
+
+
+def test_euler_cgrid_roe_muscl3():
+    d = naca_case("g_euler", cases.control_euler_naca(iters=10))
+    _compare_steps(d, 6)
+
+
+@pytest.mark.parametrize("tile_rows", [1, 2, 8])
+def test_euler_tile_shapes(tile_rows):
+    d = naca_case("g_euler_t", cases.control_euler_naca(iters=10))
+    _compare_steps(d, 2, tile_rows=tile_rows)
+
+
+SCHEMES = range(7)
+FLUXES = range(1, 7)
+RECONS = range(3)
+
+
+@pytest.mark.parametrize("scheme,flux,recon", [(s, 4, 0) for s in SCHEMES] + [(5, f, 0) for f in FLUXES] +
+                         [(s, f, r) for s, f, r in itertools.product((1, 5), (2, 3, 6), (1, 2))])
+def test_scheme_flux_recon(scheme, flux, recon):
+    c = cases.control_euler_naca(iters=10)
+    c.Iflag_Scheme, c.Iflag_Flux, c.IFlag_Reconstruction = scheme, flux, recon
+    d = naca_case(f"g_sfr_{scheme}_{flux}_{recon}", c)
+    _compare_steps(d, 2)
+
+
+def test_laminar_isothermal_wall():
+    c = cases.control_sa_naca(iters=10, Re=1e5, Ma=0.5)
+    c.Iflag_turbulence_model = F.TURB_NONE
+    c.Twall = 350.0
+    d = naca_case("g_lam", c, ds1=1e-4)
+    _compare_steps(d, 4, arrays=("U", "Res", "dt", "Lci", "Lcj", "Lvi", "Lvj"))
+
+
+def test_sa_cgrid():
+    c = cases.control_sa_naca(iters=10, Re=1e5, Ma=0.5)
+    d = naca_case("g_sa", c, ds1=1e-4)
+    _compare_steps(d, 4, arrays=("U", "Res", "dt", "Lci", "Lcj", "Lvi", "Lvj"), exact=False, rtol=1e-11)
+
+
+def test_multiblock_vortex_ausm():
+    d = vortex_case("g_mb", cases.control_vortex(iters=10))
+    _compare_steps(d, 4)
+
+
+def _symm(bcs):
+    for subs in bcs:
+        for s in subs:
+            if s[6] == F.BC_FARFIELD and s[1] in (2, 4):
+                s[6] = F.BC_SYMMETRY
+
+
+def test_multiblock_symmetry():
+    d = vortex_case("g_mb_sym", cases.control_vortex(iters=10), bc_edit=_symm)
+    _compare_steps(d, 3)
+
+
+def test_multiblock_inlet_outlet():
+    c = cases.control_vortex(iters=10)
+    c.P_outlet = 1.0 / (1.4 * 0.5 * 0.5) * 0.98
+
+    def edit(bcs):
+        for subs in bcs:
+            for s in subs:
+                if s[6] == F.BC_FARFIELD and s[1] == 1:
+                    s[6] = F.BC_INLET
+                if s[6] == F.BC_FARFIELD and s[1] == 3:
+                    s[6] = F.BC_OUTLET
+
+    d = vortex_case("g_mb_io", c, bc_edit=edit)
+    _compare_steps(d, 3)
+
+
+@pytest.mark.parametrize("tm,nm", [(F.TIME_RK3, 2), (F.TIME_RK3, 3), (F.TIME_EULER1, 3)])
+def test_multigrid_euler(tm, nm):
+    c = cases.control_euler_naca(iters=10)
+    c.Time_Method, c.Num_Mesh, c.CFL = tm, nm, 0.3
+    d = naca_case(f"g_mg_{tm}_{nm}", c)
+    _compare_steps(d, 3, arrays=("U", "Res"))
+
+
+def test_graph_replay_matches_eager():
+    d = naca_case("g_graph", cases.control_euler_naca(iters=10))
+    with solver.Batch([solver.Case.from_dir(d)]) as b:
+        b.init_flow()
+        b.run(5)
+        U_graph = b.get("U")
+        k1, t1 = b.time()
+    ref = O.OracleCase(d)
+    ref.init_flow()
+    for _ in range(5):
+        ref.main_step()
+    assert first_diff(U_graph, ref.block().U) is None
+    assert k1 == 5 and t1 == ref.tt[0]
+
+
+def test_batch_of_cases_equals_separate_runs():
+    ds = []
+    for k, (ma, aoa) in enumerate([(0.5, 1.25), (0.6, -1.0), (0.4, 3.0)]):
+        c = cases.control_euler_naca(iters=10, Ma=ma, aoa=aoa)
+        ds.append(naca_case(f"g_batch_{k}", c, nx=65 if k != 1 else 97, ny=17 if k != 2 else 25))
+    cs = [solver.Case.from_dir(d) for d in ds]
+    with solver.Batch(cs) as b:
+        b.init_flow()
+        b.run(3)
+        Us = [b.get("U", k) for k in range(3)]
+        norms = [b.res_norms(k)[1] for k in range(3)]
+    for k, d in enumerate(ds):
+        ref = O.OracleCase(d)
+        ref.init_flow()
+        for _ in range(3):
+            ref.main_step()
+        assert first_diff(Us[k], ref.block().U) is None, k
+        np.testing.assert_allclose(norms[k][:4], ref.norms()[1][:4], rtol=1e-12)
