@@ -1,0 +1,175 @@
+"""Benchmark: cell-iterations per second of the MI355X residual + LU-SGS path.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2]
+
+One "step" is one pass of the reference main-loop body (for the multigrid
+workload one V-cycle) over every cell of the case; value = Num_Cell * K /
+(time of K steps), with the state resident in HBM.  The K steps run as one
+HIP graph replayed K times, bracketed by stream synchronisations.
+
+Workloads (see opencfd-ec2d_amd/ec2d/cases.py):
+  c1  NACA 0012 C-grid 257x65, Euler, Roe + MUSCL3 (van Albada), LU-SGS CFL 5
+  c2  NACA 0012 C-grid 513x129, SA-RANS Re 6.5e6 M 0.73 AoA 2, hybrid
+      FD-WENO5 interior / Roe edge band, LU-SGS smoothed 3-level V-cycle
+
+The JSON line also carries:
+  roofline      the dominant kernel's algorithmic bytes per launch divided by
+                its mean launch time (HIP events on the library stream),
+                against the 8 TB/s HBM peak
+  cpu_baseline  the reference Fortran (flang build in oracle/_ref) timed on
+                one host core over a bounded sample of the same workload
+"""
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "opencfd-ec2d_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def make_workload(name, d):
+    from ec2d import cases
+    from ec2d import formats as F
+
+    if name == "c1":
+        x, y, bc, info = cases.naca_cgrid(257, 65, ds1=1e-3, ratio=1.15)
+        ctl = cases.control_euler_naca(iters=2000)
+        cases.write_case(d, ctl, [x], [y], [bc])
+        desc = {"workload": "naca0012_cgrid_257x65_euler_roe_muscl3_lusgs", "grid": "257x65", "cells": 16384,
+                "mach": 0.5, "aoa_deg": 1.25, "cfl": 5.0, "multigrid_levels": 1}
+        return desc
+    if name == "c2":
+        x, y, bc, info = cases.naca_cgrid(513, 129, ds1=1e-6, ratio=1.15)
+        ctl = cases.control_sa_naca(iters=1000, Ma=0.73, aoa=2.0, Re=6.5e6, cfl=5.0, num_mesh=3,
+                                    time_method=F.TIME_LU_SGS, scheme=F.SCHEME_MUSCL3, flux=F.FLUX_ROE)
+        cases.write_case(d, ctl, [x], [y], [bc], fdm_blocks=(1,))
+        desc = {"workload": "naca0012_cgrid_513x129_sa_rans_weno5fd_roe_lusgs_mg3", "grid": "513x129",
+                "cells": 65536, "mach": 0.73, "aoa_deg": 2.0, "reynolds": 6.5e6, "cfl": 5.0,
+                "multigrid_levels": 3}
+        return desc
+    raise SystemExit(f"unknown workload {name}")
+
+
+# algorithmic bytes per cell of each kernel (FP64, 8 B), counting every datum
+# the kernel must move once: see DESIGN.md "Kernels and their rooflines"
+def kernel_bytes_per_cell(kernel, nv, viscous, sa):
+    if kernel == "lusgs":
+        return 8 * (5 * nv + 10 + (2 if viscous else 0))
+    if kernel == "residual":
+        return 8 * (nv + 7 + nv + 3 + (2 + 2 if viscous else 0) + (1 if sa else 0))
+    raise ValueError(kernel)
+
+
+def cpu_baseline(case_dir, cells, budget_s=20.0):
+    """Reference Fortran on one core: time n and 2n main-loop passes, difference."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import refpy
+
+    if not refpy.available():
+        return None
+    import contextlib
+    import io
+
+    with contextlib.redirect_stdout(io.StringIO()):
+        R = refpy.RefCase(case_dir)
+        t0 = time.perf_counter()
+        R.op("main_steps", 1)
+        t1 = time.perf_counter()
+    per = max(t1 - t0, 1e-4)
+    n = max(2, int(budget_s / 3.0 / per))
+    with contextlib.redirect_stdout(io.StringIO()):
+        t0 = time.perf_counter()
+        R.op("main_steps", n)
+        t1 = time.perf_counter()
+        R.op("main_steps", 2 * n)
+        t2 = time.perf_counter()
+    dt = (t2 - t1) - (t1 - t0)
+    if dt <= 0:
+        dt = (t2 - t1) / 2.0
+    return {"value": cells * n / dt, "unit": "cell-iters/s", "cores": 1, "kind": "reference",
+            "sample": f"{n} main-loop passes of the same case (difference of {2 * n} and {n} passes), "
+                      f"flang -O2 serial build of the unmodified reference"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default=os.environ.get("EC2D_WORKLOAD", "c1"))
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tile-rows", type=int, default=4)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one MI355X: this build has no multi-GPU path (blocks and ensemble cases would "
+              "shard one GPU each; see DESIGN.md).  Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    from ec2d import solver
+
+    with tempfile.TemporaryDirectory() as d:
+        desc = make_workload(args.workload, d)
+        case = solver.Case.from_dir(d)
+        nv = case.ctl.nvar
+        viscous = case.ctl.If_viscous == 1
+        sa = nv == 5
+        with solver.Batch([case], tile_rows=args.tile_rows) as b:
+            cells = b.num_cells(0, 1)
+            b.init_flow()
+            b.run(max(args.warmup, 1))
+            b.synchronize()
+            t0 = time.perf_counter()
+            b.run(args.steps)
+            b.synchronize()
+            t1 = time.perf_counter()
+            wall = t1 - t0
+            dev_ms = b.timing()["run_ms"]
+            rmax, rrms = b.res_norms(0, 1)
+            # per-kernel launch times, eager launches with HIP events
+            b.set_profile(True)
+            b.run(min(20, args.steps))
+            tm = b.timing()
+            b.set_profile(False)
+        cpu = None if args.no_cpu else cpu_baseline(d, cells)
+
+    ms_per_step = wall * 1000.0 / args.steps
+    value = cells * args.steps / wall
+    # dominant kernel = the one with the larger share of the step
+    lusgs_calls = 1 if desc["multigrid_levels"] == 1 else desc["multigrid_levels"]
+    share = {"lusgs": tm["lusgs_ms"], "residual": tm["residual_ms"]}
+    dom = max(share, key=share.get)
+    bpc = kernel_bytes_per_cell(dom, nv, viscous, sa)
+    per_launch_ms = tm[f"{dom}_ms"]
+    achieved = bpc * cells / (per_launch_ms * 1e-3) / 1e9 if per_launch_ms > 0 else 0.0
+    out = {
+        "metric": "cell-iters/s on 1 MI355X",
+        "value": value,
+        "unit": "cell-iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (generated NACA 0012 C-grid, freestream start)",
+        "config": dict(desc, device_ms_per_step=dev_ms / args.steps,
+                       residual_rms_last=[float(v) for v in rrms[:nv]],
+                       kernel_ms={"residual": tm["residual_ms"], "lusgs": tm["lusgs_ms"], "ghost": tm["ghost_ms"]},
+                       lusgs_launches_per_step=lusgs_calls),
+        "roofline": {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_cell": bpc, "cells_per_launch": cells},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

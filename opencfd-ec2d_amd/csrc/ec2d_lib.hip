@@ -12,6 +12,7 @@
 
 #include "../../include/ec2d.h"
 #include "ec2d_device.h"
+#include "ec2d_fdm.hip"
 #include "ec2d_ghost.hip"
 #include "ec2d_lusgs.hip"
 #include "ec2d_pointwise.hip"
@@ -62,6 +63,9 @@ struct DLevel {
   std::vector<int> case_first_block;
   int lusgs_threads = 64;
   std::vector<long long> case_cells;
+  bool fdm = false;
+  int2 *d_ilines = nullptr, *d_jlines = nullptr, *d_frows = nullptr;
+  int n_ilines = 0, n_jlines = 0, n_frows = 0;
 };
 
 struct Runtime {
@@ -111,8 +115,12 @@ PlaneMap make_planes(int nv, int nv_un, bool fdm_any) {
   p.deltU = k; k += 4;
   if (fdm_any) {
     p.Akx = k++; p.Aky = k++; p.Aix = k++; p.Aiy = k++; p.Ajac = k++;
+    p.fx = k; k += 4;
+    p.fy = k; k += 4;
+    p.ev = k; k += 6;
   } else {
     p.Akx = p.Aky = p.Aix = p.Aiy = p.Ajac = -1;
+    p.fx = p.fy = p.ev = -1;
   }
   p.nplanes = k;
   return p;
@@ -171,6 +179,31 @@ int launch_residual(int level, int clip) {
 #undef RBY_TY
 #undef RLAUNCH
   HIPCHK(hipGetLastError());
+  if (L.fdm) {
+    // finite-difference interior of FDM.in blocks, then norms over the final Res
+    FdmArgs f;
+    f.blocks = L.db;
+    f.cps = R.dcps;
+    f.arena = L.arena;
+    f.pm = L.pm;
+    f.fx = L.pm.fx;
+    f.fy = L.pm.fy;
+    f.ev = L.pm.ev;
+    f.viscous = visc ? 1 : 0;
+    f.dir = 0;
+    f.lines = L.d_ilines;
+    hipLaunchKernelGGL(k_fdm_lines, dim3(L.n_ilines), dim3(256), 0, R.s, f);
+    f.dir = 1;
+    f.lines = L.d_jlines;
+    hipLaunchKernelGGL(k_fdm_lines, dim3(L.n_jlines), dim3(256), 0, R.s, f);
+    f.lines = L.d_frows;
+    if (visc) hipLaunchKernelGGL(k_fdm_visc, dim3(L.n_frows), dim3(256), 0, R.s, f);
+    hipLaunchKernelGGL(k_fdm_res, dim3(L.n_frows), dim3(256), 0, R.s, f);
+    if (nv == 4) hipLaunchKernelGGL(k_tile_norms<4>, dim3(L.ntiles), dim3(256), 0, R.s, L.dt, L.db, L.arena, L.pm, L.dpartials, R.tile_rows);
+    else if (nv == 5) hipLaunchKernelGGL(k_tile_norms<5>, dim3(L.ntiles), dim3(256), 0, R.s, L.dt, L.db, L.arena, L.pm, L.dpartials, R.tile_rows);
+    else hipLaunchKernelGGL(k_tile_norms<6>, dim3(L.ntiles), dim3(256), 0, R.s, L.dt, L.db, L.arena, L.pm, L.dpartials, R.tile_rows);
+    HIPCHK(hipGetLastError());
+  }
   int ncase = (int)R.cases.size();
   if (nv == 4) hipLaunchKernelGGL(k_norms<4>, dim3(ncase), dim3(256), 0, R.s, L.dpartials, L.dcase_tile0, L.dncell, L.dnorms);
   else if (nv == 5) hipLaunchKernelGGL(k_norms<5>, dim3(ncase), dim3(256), 0, R.s, L.dpartials, L.dcase_tile0, L.dncell, L.dnorms);
@@ -192,10 +225,17 @@ int launch_lusgs(int level, const int* dlist, int nblk, double alfa1, int update
   bool visc = R.cases[0].cp.If_viscous == 1;
   dim3 grid(nblk), blk(L.lusgs_threads);
   int nv = L.o.nvar;
-  if (nv == 4 && !visc) hipLaunchKernelGGL((k_lusgs<4, false>), grid, blk, 0, R.s, a);
-  else if (nv == 4) hipLaunchKernelGGL((k_lusgs<4, true>), grid, blk, 0, R.s, a);
-  else if (nv == 5) hipLaunchKernelGGL((k_lusgs<5, true>), grid, blk, 0, R.s, a);
-  else if (nv == 6) hipLaunchKernelGGL((k_lusgs<6, true>), grid, blk, 0, R.s, a);
+  // up to 256 rows: 4-step prefetch ring (one wave per SIMD, registers to
+  // spare); larger blocks: up to 16 waves and no ring
+  const bool big = L.lusgs_threads > 256;
+#define LLAUNCH(NV, V)                                                                      \
+  if (big) hipLaunchKernelGGL((k_lusgs<NV, V, 1, 1024>), grid, blk, 0, R.s, a);             \
+  else hipLaunchKernelGGL((k_lusgs<NV, V, 4, 256>), grid, blk, 0, R.s, a);
+  if (nv == 4 && !visc) { LLAUNCH(4, false) }
+  else if (nv == 4) { LLAUNCH(4, true) }
+  else if (nv == 5) { LLAUNCH(5, true) }
+  else if (nv == 6) { LLAUNCH(6, true) }
+#undef LLAUNCH
   else return fail("bad equation count");
   HIPCHK(hipGetLastError());
   return 0;
@@ -377,7 +417,10 @@ int multigrid() {
   } else if (tm == 3) {
     if (advance_rk3(0)) return 1;
   } else {
+    // LU-SGS-smoothed cycle (extension): the fine step clips nu_t like
+    // NS_Time_advance does; the clip is applied by the next residual
     if (advance_lusgs(0, false)) return 1;
+    if (R.lv[0].o.nvar >= 5) R.clip_pending[0] = true;
   }
   if (residual(0)) return 1;
   if (launch_xfer(0, 1, 2, true) || launch_xfer(0, 1, 1, true)) return 1;
@@ -544,7 +587,7 @@ int ec2d_finalize(void) {
   for (auto& L : R.lv) {
     hipFree(L.arena); hipFree(L.db); hipFree(L.ds); hipFree(L.dnorm); hipFree(L.dt); hipFree(L.dperm);
     hipFree(L.dcase_tile0); hipFree(L.dncell); hipFree(L.dpartials); hipFree(L.dnorms); hipFree(L.drows);
-    hipFree(L.dblocks_all);
+    hipFree(L.dblocks_all); hipFree(L.d_ilines); hipFree(L.d_jlines); hipFree(L.d_frows);
   }
   hipFree(R.dcps);
   hipFree(R.derr);
@@ -663,7 +706,7 @@ int ec2d_commit(void) {
     L.o.flux = c0.Iflag_Flux;
     L.o.recon = c0.IFlag_Reconstruction;
     L.o.turb = (l == 0) ? c0.Iflag_turbulence_model : 0;
-    if (l == 0 && fdm_any) return fail("finite-difference (FDM.in) blocks are not supported by this build yet");
+    L.fdm = fdm_any;
     L.pm = make_planes(nv, nv, fdm_any);
     // blocks
     int64_t off = 0;
@@ -862,6 +905,27 @@ int ec2d_commit(void) {
       for (int j = 0; j <= L.hb[b].ny; j++) rows.push_back(make_int2((int)b, j));
     }
     L.nrows = (int)rows.size();
+    if (L.fdm) {
+      std::vector<int2> il, jl, fr;
+      for (size_t b = 0; b < L.hb.size(); b++) {
+        if (!L.hb[b].fdm) continue;
+        int nxp = L.hb[b].nx - 1, nyp = L.hb[b].ny - 1;
+        if (nxp > FDM_MAXN || nyp > FDM_MAXN) return fail("finite-difference block longer than 992 cells");
+        if (nxp < 2 * LFDM + 2 || nyp < 2 * LFDM + 2) return fail("finite-difference block too small");
+        for (int j = LFDM; j <= nyp - LFDM; j++) il.push_back(make_int2((int)b, j));
+        for (int i = LFDM; i <= nxp - LFDM; i++) jl.push_back(make_int2((int)b, i));
+        for (int j = 1; j <= nyp; j++) fr.push_back(make_int2((int)b, j));
+      }
+      L.n_ilines = (int)il.size();
+      L.n_jlines = (int)jl.size();
+      L.n_frows = (int)fr.size();
+      HIPCHK(hipMalloc(&L.d_ilines, sizeof(int2) * il.size()));
+      HIPCHK(hipMemcpy(L.d_ilines, il.data(), sizeof(int2) * il.size(), hipMemcpyHostToDevice));
+      HIPCHK(hipMalloc(&L.d_jlines, sizeof(int2) * jl.size()));
+      HIPCHK(hipMemcpy(L.d_jlines, jl.data(), sizeof(int2) * jl.size(), hipMemcpyHostToDevice));
+      HIPCHK(hipMalloc(&L.d_frows, sizeof(int2) * fr.size()));
+      HIPCHK(hipMemcpy(L.d_frows, fr.data(), sizeof(int2) * fr.size(), hipMemcpyHostToDevice));
+    }
     HIPCHK(hipMalloc(&L.drows, sizeof(int2) * rows.size()));
     HIPCHK(hipMemcpy(L.drows, rows.data(), sizeof(int2) * rows.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMalloc(&L.dblocks_all, sizeof(int) * all.size()));

@@ -138,11 +138,16 @@ __global__ __launch_bounds__(TX* TY) void k_residual(ResArgs a) {
     int fi = f % (TX + 1), fj = f / (TX + 1);
     int i = i0 + fi, j = j0 + fj;
     if (i > nx || j > ny - 1) continue;
-    if (fdm && (i >= LF + 2 && i <= nx - LF - 1) && (j >= LF + 2 && j <= ny - LF - 1)) continue;
+    // faces of the finite-difference interior carry no FVM flux, but the SA
+    // model's own face loops (sub_turbulence_SA.f90:43-73) cover every face
+    const bool skip_fvm = fdm && (i >= LF + 2 && i <= nx - LF - 1) && (j >= LF + 2 && j <= ny - LF - 1);
+    if (skip_fvm && !SA) continue;
     int64_t q = pidx(B, i, j);
     double si = PL(pm.si)[q], ni1 = PL(pm.ni1)[q], ni2 = PL(pm.ni2)[q];
     int lj = fj + 2, li = fi + 2;  // local index of cell (i, j); stencil li-2 .. li+1
     double UL[4], UR[4];
+    double F1 = 0.0, F2 = 0.0, F3 = 0.0, F4 = 0.0;
+    if (!skip_fvm) {
     if (a.o.recon == 0) {
 #define RECON(ARR, m)                                                                                           \
   UL[m] = scheme_fP(ARR[lj][li - 2], ARR[lj][li - 1], ARR[lj][li], ARR[lj][li + 1], a.o.scheme);              \
@@ -171,10 +176,11 @@ __global__ __launch_bounds__(TX* TY) void k_residual(ResArgs a) {
     QL[0] = UL[0]; QL[1] = UL[1] * ni1 + UL[2] * ni2; QL[2] = -UL[1] * ni2 + UL[2] * ni1; QL[3] = UL[3];
     QR[0] = UR[0]; QR[1] = UR[1] * ni1 + UR[2] * ni2; QR[2] = -UR[1] * ni2 + UR[2] * ni1; QR[3] = UR[3];
     face_flux(a.o.flux, QL, QR, F0, g);
-    double F1 = -F0[0] * si;
-    double F2 = -(F0[1] * ni1 - F0[2] * ni2) * si;
-    double F3 = -(F0[1] * ni2 + F0[2] * ni1) * si;
-    double F4 = -F0[3] * si;
+    F1 = -F0[0] * si;
+    F2 = -(F0[1] * ni1 - F0[2] * ni2) * si;
+    F3 = -(F0[1] * ni2 + F0[2] * ni1) * si;
+    F4 = -F0[3] * si;
+    }
     if constexpr (VISC) {
       double Amu1, Amk1;
       if (i == 1) {
@@ -207,9 +213,11 @@ __global__ __launch_bounds__(TX* TY) void k_residual(ResArgs a) {
       double t12 = (uy + vx) * Amu1;
       double E1 = u1 * t11 + v1 * t12 + Amk1 * Tx;
       double E2 = u1 * t12 + v1 * t22 + Amk1 * Ty;
-      F2 = F2 + (t11 * ni1 + t12 * ni2) * si;
-      F3 = F3 + (t12 * ni1 + t22 * ni2) * si;
-      F4 = F4 + (E1 * ni1 + E2 * ni2) * si;
+      if (!skip_fvm) {
+        F2 = F2 + (t11 * ni1 + t12 * ni2) * si;
+        F3 = F3 + (t12 * ni1 + t22 * ni2) * si;
+        F4 = F4 + (E1 * ni1 + E2 * ni2) * si;
+      }
       if constexpr (SA) {
         const double Cb2 = 0.622, SA_sigma = 2.0 / 3.0;
         const double c_diff = 0.5 * (1.0 + Cb2) / SA_sigma;
@@ -235,11 +243,14 @@ __global__ __launch_bounds__(TX* TY) void k_residual(ResArgs a) {
     int fi = f % TX, fj = f / TX;
     int i = i0 + fi, j = j0 + fj;
     if (i > nx - 1 || j > ny) continue;
-    if (fdm && (i >= LF + 2 && i <= nx - LF - 1) && (j >= LF + 2 && j <= ny - LF - 1)) continue;
+    const bool skip_fvm = fdm && (i >= LF + 2 && i <= nx - LF - 1) && (j >= LF + 2 && j <= ny - LF - 1);
+    if (skip_fvm && !SA) continue;
     int64_t q = pidx(B, i, j);
     double sj = PL(pm.sj)[q], nj1 = PL(pm.nj1)[q], nj2 = PL(pm.nj2)[q];
     int lj = fj + 2, li = fi + 2;
     double UL[4], UR[4];
+    double F1 = 0.0, F2 = 0.0, F3 = 0.0, F4 = 0.0;
+    if (!skip_fvm) {
     if (a.o.recon == 0) {
       // the j-direction original-variable reconstruction uses the global
       // scheme option, not the mesh's own (sub_Residual.f90:367)
@@ -270,10 +281,11 @@ __global__ __launch_bounds__(TX* TY) void k_residual(ResArgs a) {
     QL[0] = UL[0]; QL[1] = UL[1] * nj1 + UL[2] * nj2; QL[2] = -UL[1] * nj2 + UL[2] * nj1; QL[3] = UL[3];
     QR[0] = UR[0]; QR[1] = UR[1] * nj1 + UR[2] * nj2; QR[2] = -UR[1] * nj2 + UR[2] * nj1; QR[3] = UR[3];
     face_flux(a.o.flux, QL, QR, F0, g);
-    double F1 = -F0[0] * sj;
-    double F2 = -(F0[1] * nj1 - F0[2] * nj2) * sj;
-    double F3 = -(F0[1] * nj2 + F0[2] * nj1) * sj;
-    double F4 = -F0[3] * sj;
+    F1 = -F0[0] * sj;
+    F2 = -(F0[1] * nj1 - F0[2] * nj2) * sj;
+    F3 = -(F0[1] * nj2 + F0[2] * nj1) * sj;
+    F4 = -F0[3] * sj;
+    }
     if constexpr (VISC) {
       double Amu1, Amk1;
       if (j == 1) {
@@ -306,9 +318,11 @@ __global__ __launch_bounds__(TX* TY) void k_residual(ResArgs a) {
       double v1 = (sv[lj][li] + sv[lj - 1][li]) * 0.5;
       double E1 = u1 * t11 + v1 * t12 + Amk1 * Tx;
       double E2 = u1 * t12 + v1 * t22 + Amk1 * Ty;
-      F2 = F2 + (t11 * nj1 + t12 * nj2) * sj;
-      F3 = F3 + (t12 * nj1 + t22 * nj2) * sj;
-      F4 = F4 + (E1 * nj1 + E2 * nj2) * sj;
+      if (!skip_fvm) {
+        F2 = F2 + (t11 * nj1 + t12 * nj2) * sj;
+        F3 = F3 + (t12 * nj1 + t22 * nj2) * sj;
+        F4 = F4 + (E1 * nj1 + E2 * nj2) * sj;
+      }
       if constexpr (SA) {
         const double Cb2 = 0.622, SA_sigma = 2.0 / 3.0;
         const double c_diff = 0.5 * (1.0 + Cb2) / SA_sigma;

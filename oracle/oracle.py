@@ -287,16 +287,26 @@ class OracleCase:
             lib().o_set_un(self.M(nmesh))
             for _ in range(3):
                 self.advance_euler1(nmesh)
-        else:
+        elif self.ctl.Time_Method == F.TIME_RK3:
             self.advance_rk3(nmesh)
+        else:
+            # LU-SGS smoother on a coarse level: one LU-SGS step on the forced
+            # residual Res + QF.  The reference rejects LU-SGS with multigrid
+            # (opencfd_ec2d_v1.5.4.f90:216-221); this composes its pieces in
+            # the structure of NS_3stge_multigrid (sub_NS_multigrid.f90:66-142).
+            lib().o_set_un(self.M(nmesh))
+            self.advance_lusgs(nmesh, with_forcing=True)
 
     def mg3(self):
         """NS_3stge_multigrid (sub_NS_multigrid.f90:66-142); NS_2stge (:14-58) when two levels."""
         L = lib()
         if self.ctl.Time_Method == F.TIME_EULER1:
             self.advance_euler1(1)
-        else:
+        elif self.ctl.Time_Method == F.TIME_RK3:
             self.advance_rk3(1)
+        else:
+            self.advance_lusgs(1)
+            self.clip(1)
         self.residual(1)
         L.o_interp2h(self.M(1), self.M(2), 2)
         L.o_interp2h(self.M(1), self.M(2), 1)

@@ -166,3 +166,22 @@ def test_batch_of_cases_equals_separate_runs():
             ref.main_step()
         assert first_diff(Us[k], ref.block().U) is None, k
         np.testing.assert_allclose(norms[k][:4], ref.norms()[1][:4], rtol=1e-12)
+
+
+def test_fdm_weno5_euler():
+    d = naca_case("g_fdm_e", cases.control_euler_naca(iters=10), fdm=(1,))
+    _compare_steps(d, 3, arrays=("U", "Res", "dt"))
+
+
+def test_fdm_weno5_sa():
+    c = cases.control_sa_naca(iters=10, Re=1e5, Ma=0.5)
+    d = naca_case("g_fdm_sa", c, ds1=1e-4, fdm=(1,))
+    _compare_steps(d, 3, arrays=("U", "Res", "dt"), exact=False, rtol=1e-11)
+
+
+@pytest.mark.parametrize("nm", [2, 3])
+def test_multigrid_lusgs_sa(nm):
+    """LU-SGS-smoothed V-cycle (extension), SA on the fine level, FD-WENO5 block."""
+    c = cases.control_sa_naca(iters=10, Re=1e5, Ma=0.5, num_mesh=nm, time_method=F.TIME_LU_SGS)
+    d = naca_case(f"g_mg_lusgs_{nm}", c, ds1=1e-4, fdm=(1,))
+    _compare_steps(d, 3, arrays=("U", "Res"), exact=False, rtol=1e-11)
