@@ -144,6 +144,11 @@ long long ec2d_num_cells(int case_id, int nMesh);
  * measured).  No reference counterpart: measurement support. */
 int ec2d_timing(double *run_ms, double *residual_ms, double *lusgs_ms, double *ghost_ms);
 
+/* Per mesh level: mean launch time (ms) of the residual, LU-SGS and ghost
+ * kernels and their launch counts (3 ints) during the last profiled ec2d_run.
+ * No reference counterpart: measurement support. */
+int ec2d_timing_level(int nMesh, double *residual_ms, double *lusgs_ms, double *ghost_ms, int *launches);
+
 /* 1: ec2d_run launches eagerly and times every residual / LU-SGS / ghost
  * launch with HIP events (see ec2d_timing); 0 (default): one HIP graph.
  * No reference counterpart: measurement support. */

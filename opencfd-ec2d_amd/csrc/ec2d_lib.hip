@@ -62,6 +62,9 @@ struct DLevel {
   int* dblocks_all = nullptr;
   std::vector<int> case_first_block;
   int lusgs_threads = 64;
+  double* sk = nullptr;       // LU-SGS records (skewed, see ec2d_lusgs.hip)
+  int64_t sk_max_n = 0;       // largest record field of a block (doubles)
+  int max_cells = 0;          // largest block (cells)
   std::vector<long long> case_cells;
   bool fdm = false;
   int2 *d_ilines = nullptr, *d_jlines = nullptr, *d_frows = nullptr;
@@ -86,6 +89,8 @@ struct Runtime {
   int profile = 0;
   std::vector<hipEvent_t> pev;
   std::vector<int> pev_kind;
+  double prof_mean[40] = {0};
+  int prof_count[40] = {0};
 };
 
 Runtime R;
@@ -219,18 +224,22 @@ int launch_lusgs(int level, const int* dlist, int nblk, double alfa1, int update
   a.block_list = dlist;
   a.cps = R.dcps;
   a.arena = L.arena;
+  a.sk = L.sk;
   a.pm = L.pm;
   a.alfa1 = alfa1;
   a.update = update;
   bool visc = R.cases[0].cp.If_viscous == 1;
-  dim3 grid(nblk), blk(L.lusgs_threads);
+  dim3 gprep((unsigned)((L.sk_max_n + 255) / 256), nblk), gsweep(nblk), blk(L.lusgs_threads);
+  dim3 gscat((unsigned)((L.max_cells + 255) / 256), nblk);
   int nv = L.o.nvar;
-  // up to 256 rows: 4-step prefetch ring (one wave per SIMD, registers to
-  // spare); larger blocks: up to 16 waves and no ring
+  // up to 256 rows: 2-step prefetch ring (one wave per SIMD, registers to
+  // spare); larger blocks: up to 16 waves and one record ahead
   const bool big = L.lusgs_threads > 256;
 #define LLAUNCH(NV, V)                                                                      \
-  if (big) hipLaunchKernelGGL((k_lusgs<NV, V, 1, 1024>), grid, blk, 0, R.s, a);             \
-  else hipLaunchKernelGGL((k_lusgs<NV, V, 4, 256>), grid, blk, 0, R.s, a);
+  hipLaunchKernelGGL((k_lusgs_prep<NV, V>), gprep, dim3(256), 0, R.s, a);                   \
+  if (big) hipLaunchKernelGGL((k_lusgs_sweep<NV, V, 1, 1024>), gsweep, blk, 0, R.s, a);     \
+  else hipLaunchKernelGGL((k_lusgs_sweep<NV, V, 2, 256>), gsweep, blk, 0, R.s, a);          \
+  hipLaunchKernelGGL((k_lusgs_scatter<NV, V>), gscat, dim3(256), 0, R.s, a);
   if (nv == 4 && !visc) { LLAUNCH(4, false) }
   else if (nv == 4) { LLAUNCH(4, true) }
   else if (nv == 5) { LLAUNCH(5, true) }
@@ -330,9 +339,9 @@ int flush_clip(int level) {
 int residual(int level) {
   int clip = R.clip_pending[level] ? 1 : 0;
   R.clip_pending[level] = false;
-  prof_mark(1);
+  prof_mark(1 + 10 * level);
   int rc = launch_residual(level, clip);
-  prof_mark(-1);
+  prof_mark(-(1 + 10 * level));
   return rc;
 }
 
@@ -350,12 +359,12 @@ int advance_lusgs(int level, bool add_force) {
   DLevel& L = R.lv[level];
   if (residual(level)) return 1;
   if (add_force && launch_pw(level, PW_ADD_FORCE)) return 1;
-  prof_mark(2);
+  prof_mark(2 + 10 * level);
   if (launch_lusgs(level, L.dblocks_all, (int)L.hb.size(), 0.0, 1)) return 1;
-  prof_mark(-2);
-  prof_mark(3);
+  prof_mark(-(2 + 10 * level));
+  prof_mark(3 + 10 * level);
   if (launch_ghost(level, 1, 1)) return 1;
-  prof_mark(-3);
+  prof_mark(-(3 + 10 * level));
   tick(level);
   return 0;
 }
@@ -587,7 +596,7 @@ int ec2d_finalize(void) {
   for (auto& L : R.lv) {
     hipFree(L.arena); hipFree(L.db); hipFree(L.ds); hipFree(L.dnorm); hipFree(L.dt); hipFree(L.dperm);
     hipFree(L.dcase_tile0); hipFree(L.dncell); hipFree(L.dpartials); hipFree(L.dnorms); hipFree(L.drows);
-    hipFree(L.dblocks_all); hipFree(L.d_ilines); hipFree(L.d_jlines); hipFree(L.d_frows);
+    hipFree(L.dblocks_all); hipFree(L.sk); hipFree(L.d_ilines); hipFree(L.d_jlines); hipFree(L.d_frows);
   }
   hipFree(R.dcps);
   hipFree(R.derr);
@@ -806,6 +815,23 @@ int ec2d_commit(void) {
     L.case_first_block.push_back((int)L.hb.size());
     L.arena_n = off;
     L.lusgs_threads = 64 * ((max_rows + 63) / 64);
+    {
+      const bool visc = R.cases[0].cp.If_viscous == 1;
+      const int64_t per = 2 * lusgs_nf(nv, visc) + nv;
+      int64_t sk_off = 0;
+      for (auto& d : L.hb) {
+        const int ncol = d.nx - 1, nrow = d.ny - 1;
+        d.sk_S = ncol + nrow - 1;
+        d.sk_R = 64 * ((nrow + 63) / 64);
+        d.sk_n = (int64_t)d.sk_S * d.sk_R;
+        d.sk_base = sk_off;
+        sk_off += (per * d.sk_n + 31) / 32 * 32;
+        L.sk_max_n = std::max(L.sk_max_n, d.sk_n);
+        L.max_cells = std::max(L.max_cells, ncol * nrow);
+      }
+      HIPCHK(hipMalloc(&L.sk, sizeof(double) * (size_t)std::max<int64_t>(sk_off, 1)));
+      HIPCHK(hipMemset(L.sk, 0, sizeof(double) * (size_t)std::max<int64_t>(sk_off, 1)));
+    }
     if (L.lusgs_threads > 1024) return fail("blocks with more than 1024 rows of cells are not supported by the LU-SGS kernel");
     // host image of the arena
     std::vector<double> img((size_t)off, 0.0);
@@ -1162,20 +1188,25 @@ int ec2d_run(int n_iter) {
   HIPCHK(hipEventElapsedTime(&ms, R.ev0, R.ev1));
   R.last_run_ms = ms;
   if (R.profile) {
-    double acc[4] = {0, 0, 0, 0};
-    int cnt[4] = {0, 0, 0, 0};
+    // mean launch time per (kernel, level); ec2d_timing reports the finest level
+    double acc[40] = {0};
+    int cnt[40] = {0};
     for (size_t k = 0; k + 1 < R.pev.size(); k++) {
       int kind = R.pev_kind[k];
-      if (kind > 0 && R.pev_kind[k + 1] == -kind) {
+      if (kind > 0 && kind < 40 && R.pev_kind[k + 1] == -kind) {
         float t = 0;
         hipEventElapsedTime(&t, R.pev[k], R.pev[k + 1]);
         acc[kind] += t;
         cnt[kind]++;
       }
     }
-    R.res_ms = cnt[1] ? acc[1] / cnt[1] : 0.0;
-    R.lusgs_ms = cnt[2] ? acc[2] / cnt[2] : 0.0;
-    R.ghost_ms = cnt[3] ? acc[3] / cnt[3] : 0.0;
+    for (int k = 0; k < 40; k++) {
+      R.prof_mean[k] = cnt[k] ? acc[k] / cnt[k] : 0.0;
+      R.prof_count[k] = cnt[k];
+    }
+    R.res_ms = R.prof_mean[1];
+    R.lusgs_ms = R.prof_mean[2];
+    R.ghost_ms = R.prof_mean[3];
   }
   return 0;
 }
@@ -1190,6 +1221,20 @@ int ec2d_timing(double* run_ms, double* residual_ms, double* lusgs_ms, double* g
   if (residual_ms) *residual_ms = R.res_ms;
   if (lusgs_ms) *lusgs_ms = R.lusgs_ms;
   if (ghost_ms) *ghost_ms = R.ghost_ms;
+  return 0;
+}
+
+int ec2d_timing_level(int nMesh, double* residual_ms, double* lusgs_ms, double* ghost_ms, int* launches) {
+  if (nMesh < 1 || nMesh > 3) return fail("bad mesh level");
+  int b = 10 * (nMesh - 1);
+  if (residual_ms) *residual_ms = R.prof_mean[b + 1];
+  if (lusgs_ms) *lusgs_ms = R.prof_mean[b + 2];
+  if (ghost_ms) *ghost_ms = R.prof_mean[b + 3];
+  if (launches) {
+    launches[0] = R.prof_count[b + 1];
+    launches[1] = R.prof_count[b + 2];
+    launches[2] = R.prof_count[b + 3];
+  }
   return 0;
 }
 

@@ -300,22 +300,34 @@ __device__ __forceinline__ void face_flux(int iflux, const double* QL, const dou
   }
 }
 
-// F(U+dU).n - F(U).n of the Euler normal flux: sub_LU_SGS.f90:117-138
+// comput_DFn (sub_LU_SGS.f90:111-138), F(U+dU).n - F(U).n of the Euler normal
+// flux, split in two: flux_n_old gives F(U).n, which depends on the old state
+// only, and dfn_from_old the difference for U2 = U + dU.  Both halves evaluate
+// the reference's expressions unchanged, so the difference is bit-identical
+// to the undivided routine.
 template <int NV>
-__device__ __forceinline__ void comput_dfn(double* DF, const double* U, const double* DU, double n1, double n2, double g) {
-  double U2[NV];
-#pragma unroll
-  for (int m = 0; m < NV; m++) U2[m] = U[m] + DU[m];
+__device__ __forceinline__ void flux_n_old(double* F, const double* U, double n1, double n2, double g) {
   double un1 = (U[1] * n1 + U[2] * n2) / U[0];
   double p1 = (g - 1.0) * (U[3] - 0.5 * (U[1] * U[1] + U[2] * U[2]) / U[0]);
+  F[0] = U[0] * un1;
+  F[1] = U[1] * un1 + p1 * n1;
+  F[2] = U[2] * un1 + p1 * n2;
+  F[3] = (U[3] + p1) * un1;
+#pragma unroll
+  for (int m = 4; m < NV; m++) F[m] = U[m] * un1;
+}
+
+template <int NV>
+__device__ __forceinline__ void dfn_from_old(double* DF, const double* U2, const double* F, double n1, double n2,
+                                             double g) {
   double un2 = (U2[1] * n1 + U2[2] * n2) / U2[0];
   double p2 = (g - 1.0) * (U2[3] - 0.5 * (U2[1] * U2[1] + U2[2] * U2[2]) / U2[0]);
-  DF[0] = U2[0] * un2 - U[0] * un1;
-  DF[1] = (U2[1] * un2 + p2 * n1) - (U[1] * un1 + p1 * n1);
-  DF[2] = (U2[2] * un2 + p2 * n2) - (U[2] * un1 + p1 * n2);
-  DF[3] = (U2[3] + p2) * un2 - (U[3] + p1) * un1;
+  DF[0] = U2[0] * un2 - F[0];
+  DF[1] = (U2[1] * un2 + p2 * n1) - F[1];
+  DF[2] = (U2[2] * un2 + p2 * n2) - F[2];
+  DF[3] = (U2[3] + p2) * un2 - F[3];
 #pragma unroll
-  for (int m = 4; m < NV; m++) DF[m] = U2[m] * un2 - U[m] * un1;
+  for (int m = 4; m < NV; m++) DF[m] = U2[m] * un2 - F[m];
 }
 
 }  // namespace ec2d

@@ -239,6 +239,13 @@ class Batch:
         capi.call("ec2d_timing", *[C.byref(x) for x in v])
         return {"run_ms": v[0].value, "residual_ms": v[1].value, "lusgs_ms": v[2].value, "ghost_ms": v[3].value}
 
+    def timing_level(self, nmesh: int = 1):
+        v = [C.c_double() for _ in range(3)]
+        n = (C.c_int * 3)()
+        capi.call("ec2d_timing_level", nmesh, *[C.byref(x) for x in v], n)
+        return {"residual_ms": v[0].value, "lusgs_ms": v[1].value, "ghost_ms": v[2].value,
+                "launches": {"residual": n[0], "lusgs": n[1], "ghost": n[2]}}
+
     def close(self):
         capi.call("ec2d_finalize")
 
