@@ -15,9 +15,18 @@ Workloads (see opencfd-ec2d_amd/ec2d/cases.py):
 The JSON line also carries:
   roofline      the dominant kernel's algorithmic bytes per launch divided by
                 its mean launch time (HIP events on the library stream),
-                against the 8 TB/s HBM peak
-  cpu_baseline  the reference Fortran (flang build in oracle/_ref) timed on
-                one host core over a bounded sample of the same workload
+                against the 8 TB/s HBM peak; "traffic" is the HBM bytes per
+                launch read from the rocprofv3 FETCH_SIZE / WRITE_SIZE
+                summary committed under profiles/ (null if absent)
+  cpu_baseline  the reference Fortran (flang build in oracle/_ref) or, for
+                workloads the reference refuses (LU-SGS with multigrid), the C
+                restatement in oracle/, timed on one host core over a bounded
+                sample of the same workload
+
+    python bench.py --calibrate-copy
+
+copies 512 MiB between two device buffers five times (the plain copy kernel
+the HBM counters are scaled by) and prints the byte count.
 """
 import argparse
 import json
@@ -56,10 +65,15 @@ def make_workload(name, d):
 
 
 # algorithmic bytes per cell of each kernel (FP64, 8 B), counting every datum
-# the kernel must move once: see DESIGN.md "Kernels and their rooflines"
+# the step must move once.
+#   lusgs:    U read + written, Res read, forward dU written and read back,
+#             final dU written (6 NV); vol, dt, Lci, Lcj, si, sj, ni1, ni2,
+#             nj1, nj2 read (10); Lvi, Lvj when viscous (2)
+#   residual: U read, 7 metrics read, Res written, dt, Lci, Lcj written;
+#             viscous: Lvi, Lvj written, x1, y1 read; SA: dw read
 def kernel_bytes_per_cell(kernel, nv, viscous, sa):
     if kernel == "lusgs":
-        return 8 * (5 * nv + 10 + (2 if viscous else 0))
+        return 8 * (6 * nv + 10 + (2 if viscous else 0))
     if kernel == "residual":
         return 8 * (nv + 7 + nv + 3 + (2 + 2 if viscous else 0) + (1 if sa else 0))
     raise ValueError(kernel)
@@ -116,6 +130,32 @@ def cpu_baseline(case_dir, cells, budget_s=20.0, kind="reference"):
             "sample": f"{n} main-loop passes of the same case (time of {2 * n} passes minus time of {n}), {what}"}
 
 
+def read_traffic(workload):
+    """HBM bytes per launch of the LU-SGS update from the committed rocprofv3
+    counter summary (profiles/lusgs_traffic_<workload>.txt), or None."""
+    p = os.path.join(ROOT, "profiles", f"lusgs_traffic_{workload}.txt")
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        for line in f:
+            if line.startswith("hbm_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    return None
+
+
+def calibrate_copy():
+    import torch
+
+    n = 64 * 1024 * 1024  # doubles: 512 MiB
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": 8 * n, "copy_bytes_written": 8 * n, "launches": 5}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -124,7 +164,11 @@ def main():
     ap.add_argument("--workload", default=os.environ.get("EC2D_WORKLOAD", "c2"))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile-rows", type=int, default=4)
+    ap.add_argument("--calibrate-copy", action="store_true")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print("bench.py measures one MI355X: this build has no multi-GPU path (blocks and ensemble cases would "
               "shard one GPU each; see DESIGN.md).  Run with --gpus 1.", file=sys.stderr)
@@ -154,7 +198,10 @@ def main():
             b.set_profile(True)
             b.run(min(20, args.steps))
             tm = b.timing_level(1)
-            levels = {f"level{n}": b.timing_level(n) for n in range(1, b.num_mesh + 1)}
+            levels = {}
+            for n in range(1, b.num_mesh + 1):
+                levels[f"level{n}"] = {k: dict(zip(("ms", "launches"), b.kernel_time(k, n)))
+                                       for k in b.KERNEL_KINDS}
             b.set_profile(False)
         kind = "port" if desc["multigrid_levels"] > 1 else "reference"
         cpu = None if args.no_cpu else cpu_baseline(d, cells, kind=kind)
@@ -185,7 +232,8 @@ def main():
                        residual_rms_last=[float(v) for v in rrms[:nv]],
                        kernel_ms_per_level=levels, tile_rows=args.tile_rows),
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": read_traffic(desc["workload"]) if dom == "lusgs" else None,
                      "bytes_per_cell": bpc, "cells_per_launch": cells,
                      "ms_per_launch": per_launch_ms},
         "cpu_baseline": cpu,

@@ -149,6 +149,13 @@ int ec2d_timing(double *run_ms, double *residual_ms, double *lusgs_ms, double *g
  * No reference counterpart: measurement support. */
 int ec2d_timing_level(int nMesh, double *residual_ms, double *lusgs_ms, double *ghost_ms, int *launches);
 
+/* Per mesh level and kernel kind: mean launch time (ms) and launch count
+ * during the last profiled ec2d_run.  Kinds: 1 residual (all its kernels),
+ * 2 LU-SGS update (the three below together), 3 ghost fill, 4 LU-SGS record
+ * preparation, 5 LU-SGS sweeps, 6 LU-SGS scatter.  No reference
+ * counterpart: measurement support. */
+int ec2d_kernel_time(int nMesh, int kind, double *ms, int *launches);
+
 /* 1: ec2d_run launches eagerly and times every residual / LU-SGS / ghost
  * launch with HIP events (see ec2d_timing); 0 (default): one HIP graph.
  * No reference counterpart: measurement support. */

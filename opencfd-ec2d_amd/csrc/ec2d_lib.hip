@@ -217,6 +217,8 @@ int launch_residual(int level, int clip) {
   return 0;
 }
 
+void prof_mark(int kind);
+
 int launch_lusgs(int level, const int* dlist, int nblk, double alfa1, int update) {
   DLevel& L = R.lv[level];
   LusgsArgs a;
@@ -235,11 +237,18 @@ int launch_lusgs(int level, const int* dlist, int nblk, double alfa1, int update
   // up to 256 rows: 2-step prefetch ring (one wave per SIMD, registers to
   // spare); larger blocks: up to 16 waves and one record ahead
   const bool big = L.lusgs_threads > 256;
+  const int pk = 10 * level;
 #define LLAUNCH(NV, V)                                                                      \
+  prof_mark(4 + pk);                                                                        \
   hipLaunchKernelGGL((k_lusgs_prep<NV, V>), gprep, dim3(256), 0, R.s, a);                   \
+  prof_mark(-(4 + pk));                                                                     \
+  prof_mark(5 + pk);                                                                        \
   if (big) hipLaunchKernelGGL((k_lusgs_sweep<NV, V, 1, 1024>), gsweep, blk, 0, R.s, a);     \
   else hipLaunchKernelGGL((k_lusgs_sweep<NV, V, 2, 256>), gsweep, blk, 0, R.s, a);          \
-  hipLaunchKernelGGL((k_lusgs_scatter<NV, V>), gscat, dim3(256), 0, R.s, a);
+  prof_mark(-(5 + pk));                                                                     \
+  prof_mark(6 + pk);                                                                        \
+  hipLaunchKernelGGL((k_lusgs_scatter<NV, V>), gscat, dim3(256), 0, R.s, a);                \
+  prof_mark(-(6 + pk));
   if (nv == 4 && !visc) { LLAUNCH(4, false) }
   else if (nv == 4) { LLAUNCH(4, true) }
   else if (nv == 5) { LLAUNCH(5, true) }
@@ -1191,13 +1200,19 @@ int ec2d_run(int n_iter) {
     // mean launch time per (kernel, level); ec2d_timing reports the finest level
     double acc[40] = {0};
     int cnt[40] = {0};
+    // marks nest (the LU-SGS bracket holds its three kernels): pair each
+    // opening mark with the next closing mark of the same kind
     for (size_t k = 0; k + 1 < R.pev.size(); k++) {
       int kind = R.pev_kind[k];
-      if (kind > 0 && kind < 40 && R.pev_kind[k + 1] == -kind) {
+      if (kind <= 0 || kind >= 40) continue;
+      for (size_t e = k + 1; e < R.pev.size(); e++) {
+        if (R.pev_kind[e] == kind) break;
+        if (R.pev_kind[e] != -kind) continue;
         float t = 0;
-        hipEventElapsedTime(&t, R.pev[k], R.pev[k + 1]);
+        hipEventElapsedTime(&t, R.pev[k], R.pev[e]);
         acc[kind] += t;
         cnt[kind]++;
+        break;
       }
     }
     for (int k = 0; k < 40; k++) {
@@ -1221,6 +1236,15 @@ int ec2d_timing(double* run_ms, double* residual_ms, double* lusgs_ms, double* g
   if (residual_ms) *residual_ms = R.res_ms;
   if (lusgs_ms) *lusgs_ms = R.lusgs_ms;
   if (ghost_ms) *ghost_ms = R.ghost_ms;
+  return 0;
+}
+
+int ec2d_kernel_time(int nMesh, int kind, double* ms, int* launches) {
+  if (nMesh < 1 || nMesh > 3) return fail("bad mesh level");
+  if (kind < 1 || kind > 6) return fail("bad kernel kind");
+  const int k = 10 * (nMesh - 1) + kind;
+  if (ms) *ms = R.prof_mean[k];
+  if (launches) *launches = R.prof_count[k];
   return 0;
 }
 

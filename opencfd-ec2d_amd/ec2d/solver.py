@@ -246,6 +246,15 @@ class Batch:
         return {"residual_ms": v[0].value, "lusgs_ms": v[1].value, "ghost_ms": v[2].value,
                 "launches": {"residual": n[0], "lusgs": n[1], "ghost": n[2]}}
 
+    KERNEL_KINDS = {"residual": 1, "lusgs": 2, "ghost": 3, "lusgs_prep": 4, "lusgs_sweep": 5, "lusgs_scatter": 6}
+
+    def kernel_time(self, kind: str, nmesh: int = 1):
+        """(mean ms per launch, launches) of one kernel kind on one level, last profiled run."""
+        ms = C.c_double()
+        n = C.c_int()
+        capi.call("ec2d_kernel_time", nmesh, self.KERNEL_KINDS[kind], C.byref(ms), C.byref(n))
+        return ms.value, n.value
+
     def close(self):
         capi.call("ec2d_finalize")
 
