@@ -1552,3 +1552,37 @@ void o_interp2h(OMesh *Mf, OMesh *Mc, int flag) {
       }
   }
 }
+
+/* ------------------------------------------------------------------------
+ * pointwise entry points for the closed-form tests of the oracle itself
+ * (flux consistency, scheme exactness, WENO5 reproduction, DF identity)
+ * ---------------------------------------------------------------------- */
+void o_pt_flux(int iflux, const double *QL, const double *QR, double *F, double g) { face_flux(iflux, QL, QR, F, g); }
+
+double o_pt_scheme(int side, const double *u, int flag) {
+  return side > 0 ? scheme_fP(u[0], u[1], u[2], u[3], flag) : scheme_fm(u[0], u[1], u[2], u[3], flag);
+}
+
+/* kind 0 original, 1 conservative, 2 characteristic; U0[k*4+m] */
+void o_pt_recon(int kind, const double *U0, double *UL, double *UR, double g, int flag) {
+  double A[4][4];
+  for (int k = 0; k < 4; k++)
+    for (int m = 0; m < 4; m++) A[k][m] = U0[4 * k + m];
+  if (kind == 0) recon_original(A, UL, UR, flag);
+  else if (kind == 1) recon_conservative(A, UL, UR, g, flag);
+  else recon_characteristic(A, UL, UR, g, flag);
+}
+
+/* side > 0: fp_weno5, else fm_weno5; v and h are 0-based arrays of n values */
+void o_pt_weno5(int side, int n, const double *v, double *h) {
+  if (side > 0) fp_weno5(n, v - 1, h - 1);
+  else fm_weno5(n, v - 1, h - 1);
+}
+
+void o_pt_split_sw(double d, double u, double v, double cc, double A1, double A2, double g, double *fp, double *fm) {
+  split_sw(d, u, v, cc, fp, fm, A1, A2, g);
+}
+
+void o_pt_dfn(int NV, double *DF, const double *U, const double *DU, double n1, double n2, double g) {
+  comput_dfn(NV, DF, U, DU, n1, n2, g);
+}

@@ -56,4 +56,12 @@ void o_delt_u(OMesh *M);
 void o_set_un(OMesh *M);
 void o_new_u(OMesh *M);
 
+/* pointwise pieces, for closed-form tests of the oracle */
+void o_pt_flux(int iflux, const double *QL, const double *QR, double *F, double g);
+double o_pt_scheme(int side, const double *u, int flag);
+void o_pt_recon(int kind, const double *U0, double *UL, double *UR, double g, int flag);
+void o_pt_weno5(int side, int n, const double *v, double *h);
+void o_pt_split_sw(double d, double u, double v, double cc, double A1, double A2, double g, double *fp, double *fm);
+void o_pt_dfn(int NV, double *DF, const double *U, const double *DU, double n1, double n2, double g);
+
 #endif

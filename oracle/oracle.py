@@ -405,3 +405,69 @@ class OracleCase:
             self.main_step()
             hist.append(self.norms(1))
         return hist
+
+
+# ---------------------------------------------------------------- pointwise
+# thin wrappers of the oracle's pointwise pieces (o_pt_* in ec2d_oracle.c),
+# for closed-form checks of the oracle itself
+
+def _arr(x, n=None):
+    a = np.ascontiguousarray(x, dtype=np.float64)
+    if n is not None:
+        assert a.size == n
+    return a
+
+
+def pt_flux(iflux: int, QL, QR, g=1.4):
+    """Interface flux of sub_flux_split.f90 for Q = (rho, u_n, u_t, p)."""
+    L = lib()
+    out = np.zeros(4)
+    a, b = _arr(QL, 4), _arr(QR, 4)
+    L.o_pt_flux(C.c_int(iflux), a.ctypes.data_as(_dp), b.ctypes.data_as(_dp), out.ctypes.data_as(_dp),
+                C.c_double(g))
+    return out
+
+
+def pt_scheme(side: int, u, flag: int) -> float:
+    """scheme_fP (side > 0) or scheme_fm of the 4-point stencil u."""
+    L = lib()
+    L.o_pt_scheme.restype = C.c_double
+    a = _arr(u, 4)
+    return L.o_pt_scheme(C.c_int(side), a.ctypes.data_as(_dp), C.c_int(flag))
+
+
+def pt_recon(kind: int, U0, g=1.4, flag=5):
+    """Reconstruction (0 original, 1 conservative, 2 characteristic) of the
+    4x4 stencil U0[point][variable]; returns (UL, UR)."""
+    L = lib()
+    a = _arr(U0, 16)
+    ul, ur = np.zeros(4), np.zeros(4)
+    L.o_pt_recon(C.c_int(kind), a.ctypes.data_as(_dp), ul.ctypes.data_as(_dp), ur.ctypes.data_as(_dp),
+                 C.c_double(g), C.c_int(flag))
+    return ul, ur
+
+
+def pt_weno5(side: int, v):
+    """fp_weno5 (side > 0) or fm_weno5 over the line v; unset entries are NaN."""
+    L = lib()
+    a = _arr(v)
+    h = np.full(a.size, np.nan)
+    L.o_pt_weno5(C.c_int(side), C.c_int(a.size), a.ctypes.data_as(_dp), h.ctypes.data_as(_dp))
+    return h
+
+
+def pt_split_sw(d, u, v, cc, A1, A2, g=1.4):
+    L = lib()
+    fp, fm = np.zeros(4), np.zeros(4)
+    L.o_pt_split_sw(*[C.c_double(x) for x in (d, u, v, cc, A1, A2, g)], fp.ctypes.data_as(_dp),
+                    fm.ctypes.data_as(_dp))
+    return fp, fm
+
+
+def pt_dfn(U, DU, n1, n2, g=1.4):
+    L = lib()
+    a, b = _arr(U), _arr(DU)
+    out = np.zeros(a.size)
+    L.o_pt_dfn(C.c_int(a.size), out.ctypes.data_as(_dp), a.ctypes.data_as(_dp), b.ctypes.data_as(_dp),
+               C.c_double(n1), C.c_double(n2), C.c_double(g))
+    return out

@@ -208,7 +208,7 @@ contains
     type(Block_TYPE), pointer :: BK
     select case (op)
     case (1); call Comput_Residual_one_mesh(a)
-    case (2); call du_LU_SGS_2D(a, b, 0.d0)
+    case (2); call du_LU_SGS_2D(a, b, real(c, 8) * 0.125d0)   ! alfa1 = c/8
     case (3)
       ! U(n+1) = U(n) + dU on block b of mesh a (sub_NS_singlegrid.f90:203-211)
       BK => Mesh(a)%Block(b)

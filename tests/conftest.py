@@ -1,4 +1,5 @@
 import os
+import subprocess
 import sys
 
 import pytest
@@ -11,3 +12,13 @@ for p in (os.path.join(ROOT, "opencfd-ec2d_amd"), os.path.join(ROOT, "oracle")):
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library)")
+
+
+@pytest.fixture(scope="session", autouse=True)
+def _oracle_built():
+    """The C oracle is the checker of every test: build it if it is missing
+    (plain gcc, seconds)."""
+    lib = os.path.join(ROOT, "oracle", "libec2d_oracle.so")
+    if not os.path.exists(lib):
+        subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "-s"], check=True)
+    yield

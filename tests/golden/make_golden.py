@@ -1,0 +1,95 @@
+"""Write the golden fixtures of tests/test_golden.py from the reference build.
+
+    python tests/golden/make_golden.py
+
+Runs the flang build of the unmodified reference (oracle/_ref, built by
+oracle/Makefile.ref from the reference sources) on small generated cases and
+stores, per case, the input files and the reference's state after a few
+main-loop passes in full FP64 precision.  The reference itself writes its
+outputs with 10 significant digits only, so the harness (oracle/ref_harness.f90)
+reads its arrays directly.
+"""
+import os
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, os.path.join(ROOT, "opencfd-ec2d_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+from ec2d import cases  # noqa: E402
+from ec2d import formats as F  # noqa: E402
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FILES = ("control.in", "Mesh2d.dat", "bc2d_1.in", "FDM.in", "flow2d.dat")
+
+
+def specs():
+    """(name, builder, steps, arrays); builder(dir) writes the case."""
+    def naca(ctl, nx=65, ny=17, ds1=1e-3, fdm=()):
+        def b(d):
+            x, y, bc, _ = cases.naca_cgrid(nx, ny, ds1=ds1)
+            cases.write_case(d, ctl, [x], [y], [bc], fdm_blocks=fdm)
+        return b
+
+    def vortex(ctl):
+        def b(d):
+            from ec2d import geometry as G
+            xs, ys, bcs, field, _ = cases.vortex_multiblock(seed=3, nbx=3, nby=2, wmin=9, wmax=21, hmin=9, hmax=17,
+                                                           h=0.25)
+            lev = G.build_levels(xs, ys, bcs, 1)
+            flow = ([g.x1[1:-1, 1:-1] for g in lev[0]], [g.y1[1:-1, 1:-1] for g in lev[0]],
+                    [field(g.x1[1:-1, 1:-1], g.y1[1:-1, 1:-1]) for g in lev[0]])
+            cases.write_case(d, ctl, xs, ys, bcs, flow=flow)
+        return b
+
+    out = []
+    out.append(("euler_roe_muscl3_lusgs", naca(cases.control_euler_naca(iters=10)), 3, ("U", "Res", "dt")))
+    c = cases.control_sa_naca(iters=10, Re=1e5, Ma=0.5)
+    out.append(("sa_lusgs", naca(c, ds1=1e-4), 3, ("U", "Res", "dt", "Amu_t")))
+    c = cases.control_sa_naca(iters=10, Re=1e5, Ma=0.5)
+    c.Iflag_turbulence_model, c.Twall = F.TURB_SST, 300.0
+    out.append(("sst_isothermal_lusgs", naca(c, ds1=1e-4), 2, ("U", "Res", "dt", "Amu_t")))
+    out.append(("vortex_multiblock_ausm", vortex(cases.control_vortex(iters=10)), 3, ("U", "Res", "dt")))
+    c = cases.control_euler_naca(iters=10)
+    c.Time_Method, c.Num_Mesh, c.CFL = F.TIME_RK3, 3, 0.3
+    out.append(("euler_rk3_mg3", naca(c), 2, ("U", "Res")))
+    c = cases.control_sa_naca(iters=10, Re=1e5, Ma=0.5)
+    out.append(("sa_fdm_weno5", naca(c, ds1=1e-4, fdm=(1,)), 2, ("U", "Res", "dt")))
+    return out
+
+
+def main():
+    import refpy
+
+    if not refpy.available():
+        raise SystemExit("reference build missing: make -C oracle -f Makefile.ref")
+    for name, build, steps, arrays in specs():
+        with tempfile.TemporaryDirectory() as d:
+            build(d)
+            files = {f: open(os.path.join(d, f)).read() for f in FILES if os.path.exists(os.path.join(d, f))}
+            R = refpy.RefCase(d)
+            nblock = R.mesh_info(1)["nblock"]
+            rms, rmax = [], []
+            for _ in range(steps):
+                R.op("main_step")
+                mx, rm = R.norms(1)
+                rms.append(rm)
+                rmax.append(mx)
+            data = {f"file:{k}": np.array(v) for k, v in files.items()}
+            data["steps"] = np.array(steps)
+            data["nblock"] = np.array(nblock)
+            data["res_rms"] = np.array(rms)
+            data["res_max"] = np.array(rmax)
+            for mb in range(1, nblock + 1):
+                for a in arrays:
+                    data[f"{a}:{mb}"] = R.get(a, 1, mb)
+            np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **data)
+            print(name, "steps", steps, "blocks", nblock, "rms", rms[-1][:4])
+
+
+if __name__ == "__main__":
+    main()

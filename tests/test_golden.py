@@ -1,0 +1,55 @@
+"""The oracle against golden vectors of the reference (CPU).
+
+tests/golden/*.npz hold small generated cases (their input files) and the
+state of the unmodified reference (flang build of its sources) after a few
+main-loop passes, in full FP64 precision; tests/golden/make_golden.py wrote
+them.  The oracle must reproduce every stored array and residual norm bit for
+bit.
+"""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import first_diff
+
+HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+FIXTURES = sorted(glob.glob(os.path.join(HERE, "*.npz")))
+
+
+def load_case(path, d):
+    z = np.load(path, allow_pickle=False)
+    for k in z.files:
+        if k.startswith("file:"):
+            with open(os.path.join(d, k[5:]), "w") as f:
+                f.write(str(z[k]))
+    return z
+
+
+@pytest.mark.parametrize("path", FIXTURES, ids=[os.path.basename(p)[:-4] for p in FIXTURES])
+def test_oracle_reproduces_reference(path, tmp_path):
+    z = load_case(path, str(tmp_path))
+    oc = O.OracleCase(str(tmp_path))
+    oc.init_flow()
+    steps = int(z["steps"])
+    for k in range(steps):
+        oc.main_step()
+        rmax, rrms = oc.norms(1)
+        assert np.array_equal(rrms, z["res_rms"][k]), f"step {k} rms {rrms} vs {z['res_rms'][k]}"
+        assert np.array_equal(rmax, z["res_max"][k]), f"step {k} max"
+    for key in z.files:
+        if ":" not in key or key.startswith("file:"):
+            continue
+        name, mb = key.split(":")
+        got = getattr(oc.block(1, int(mb)), name)
+        want = z[key]
+        if name in ("Lvi", "Lvj"):
+            got, want = got[:-1, :-1], want[:-1, :-1]
+        diff = first_diff(got, want)
+        assert diff is None, f"{key}: {diff}"
+
+
+def test_fixtures_present():
+    assert len(FIXTURES) >= 6

@@ -1,0 +1,92 @@
+"""Host-side logic on the CPU: case files, case generators, batch set-up
+checks that need no device."""
+import dataclasses
+import os
+
+import numpy as np
+import pytest
+
+from ec2d import cases
+from ec2d import formats as F
+from ec2d import geometry as G
+
+
+def test_control_round_trip(tmp_path):
+    c = cases.control_sa_naca(iters=37, Ma=0.73, aoa=2.0, Re=6.5e6, cfl=5.0, num_mesh=3,
+                              time_method=F.TIME_LU_SGS, scheme=F.SCHEME_MUSCL3, flux=F.FLUX_ROE)
+    p = str(tmp_path / "control.in")
+    F.write_control(p, c)
+    r = F.read_control(p)
+    for f in dataclasses.fields(F.Control):
+        a, b = getattr(c, f.name), getattr(r, f.name)
+        assert (list(a) == list(b)) if isinstance(a, (list, tuple)) else a == b, f.name
+    assert r.iterations() == c.iterations() == 37
+
+
+def test_mesh_and_bc_round_trip(tmp_path):
+    x, y, bc, _ = cases.naca_cgrid(65, 17, ds1=1e-3)
+    F.write_mesh2d(str(tmp_path / "Mesh2d.dat"), [x], [y])
+    F.write_bcin(str(tmp_path / "bc2d_1.in"), [bc])
+    xs, ys = F.read_mesh2d(str(tmp_path / "Mesh2d.dat"))
+    assert np.array_equal(xs[0], x) and np.array_equal(ys[0], y)
+    assert F.read_bcin(str(tmp_path / "bc2d_1.in")) == [bc]
+
+
+@pytest.mark.parametrize("nx,ny,ds1", [(65, 17, 1e-3), (257, 65, 1e-3), (513, 129, 1e-6)])
+def test_cgrid_is_valid_for_three_levels(nx, ny, ds1):
+    x, y, bc, info = cases.naca_cgrid(nx, ny, ds1=ds1)
+    assert x.shape == (ny, nx)
+    # every subface limit admits two coarsenings
+    for s in bc:
+        for idx in s[2:6]:
+            assert (idx - 1) % 4 == 0, s
+    # wake cut: the two wake subfaces match point for point, reversed
+    lw = [s for s in bc if s[6] == 1]
+    assert len(lw) == 2 and lw[0][3] - lw[0][2] == lw[1][3] - lw[1][2]
+    iw1, iw2 = lw[0][3], lw[1][2]
+    assert np.array_equal(x[0, :iw1], x[0, iw2 - 1:][::-1]) and np.array_equal(y[0, :iw1], y[0, iw2 - 1:][::-1])
+    lev = G.build_levels([x], [y], [bc], 3)
+    for blocks in lev:
+        B = blocks[0]
+        assert np.all(B.vol[:-1, :-1] > 0)
+    assert info["first_spacing"] == pytest.approx(ds1, rel=1e-12)
+
+
+def test_vortex_multiblock_interfaces_match():
+    xs, ys, bcs, field, _ = cases.vortex_multiblock(seed=3, nbx=3, nby=2, wmin=9, wmax=21, hmin=9, hmax=17, h=0.25)
+    assert len(xs) == 6
+    for b, subs in enumerate(bcs):
+        for s in subs:
+            if s[6] <= 0:
+                continue
+            nb = bcs[s[6] - 1][s[7] - 1]
+            assert nb[6] == b + 1  # the neighbour points back
+            n1 = s[3] - s[2] + s[5] - s[4]
+            assert n1 == nb[3] - nb[2] + nb[5] - nb[4]
+
+            def pts(k, sub):
+                f, i0, i1, j0, j1 = sub[1:6]
+                return np.stack([xs[k][j0 - 1:j1, i0 - 1:i1].ravel(), ys[k][j0 - 1:j1, i0 - 1:i1].ravel()])
+
+            assert np.array_equal(pts(b, s), pts(s[6] - 1, nb))
+
+
+def test_case_generators_are_deterministic(tmp_path):
+    a = cases.ensemble_params(seed=5, n=8)
+    b = cases.ensemble_params(seed=5, n=8)
+    assert a == b
+    x1, y1, _, _ = cases.naca_cgrid(65, 17)
+    x2, y2, _, _ = cases.naca_cgrid(65, 17)
+    assert np.array_equal(x1, x2) and np.array_equal(y1, y2)
+
+
+def test_write_case_files(tmp_path):
+    x, y, bc, _ = cases.naca_cgrid(65, 17)
+    d = cases.write_case(str(tmp_path), cases.control_euler_naca(iters=3), [x], [y], [bc], fdm_blocks=(1,))
+    assert set(os.listdir(d)) >= {"control.in", "Mesh2d.dat", "bc2d_1.in", "FDM.in"}
+    assert F.read_fdm_in(os.path.join(d, "FDM.in")) == [1]
+
+
+def test_fortran_pow_is_left_to_right_chain():
+    x = 1.0000000000000002
+    assert F.fortran_pow_int(x, 6) == ((((x * x) * x) * x) * x) * x

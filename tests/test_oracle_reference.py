@@ -1,0 +1,159 @@
+"""The oracle against the reference itself (CPU, where oracle/_ref is built).
+
+oracle/_ref/libec2d_ref.so is the flang build of the unmodified reference
+sources plus a bind(C) harness (oracle/ref_harness.f90); __graft_entry__.build()
+makes it wherever the reference sources are present.  Every test runs the same
+generated case through both and requires bit-identical FP64 state.  Without
+the reference build the tests are skipped; tests/test_golden.py then pins the
+oracle against stored reference outputs.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+import oracle as O
+import refpy
+from ec2d import cases
+from ec2d import formats as F
+from helpers import first_diff, naca_case, vortex_case
+
+pytestmark = pytest.mark.skipif(not refpy.available(), reason="reference build (oracle/_ref) not present")
+
+
+def run_both(d, nsteps, arrays=("U", "Res", "dt")):
+    R = refpy.RefCase(d)
+    oc = O.OracleCase(d)
+    oc.init_flow()
+    nblock = R.mesh_info(1)["nblock"]
+    for mb in range(1, nblock + 1):
+        assert first_diff(oc.block(1, mb).U, R.get("U", 1, mb)) is None, f"initial U block {mb}"
+    for k in range(nsteps):
+        R.op("main_step")
+        oc.main_step()
+        for mb in range(1, nblock + 1):
+            for a in arrays:
+                got, want = getattr(oc.block(1, mb), a), R.get(a, 1, mb)
+                if a in ("Lvi", "Lvj"):
+                    got, want = got[:-1, :-1], want[:-1, :-1]
+                diff = first_diff(got, want)
+                assert diff is None, f"step {k} block {mb} {a}: {diff}"
+        assert np.array_equal(oc.norms(1)[1], R.norms(1)[1]), f"step {k} rms"
+    return R, oc
+
+
+@pytest.mark.parametrize("scheme,flux,recon", [(s, 4, 0) for s in range(7)] + [(5, f, 0) for f in range(1, 7)] +
+                         [(s, f, r) for s, f, r in itertools.product((1, 5), (2, 6), (1, 2))])
+def test_schemes_fluxes_reconstructions(scheme, flux, recon):
+    c = cases.control_euler_naca(iters=10)
+    c.Iflag_Scheme, c.Iflag_Flux, c.IFlag_Reconstruction = scheme, flux, recon
+    run_both(naca_case(f"r_sfr_{scheme}_{flux}_{recon}", c, nx=33, ny=9), 2)
+
+
+def test_laminar_isothermal_wall():
+    c = cases.control_sa_naca(iters=10, Re=1e5, Ma=0.5)
+    c.Iflag_turbulence_model, c.Twall = F.TURB_NONE, 350.0
+    run_both(naca_case("r_lam", c, ds1=1e-4), 3, ("U", "Res", "dt", "Lvi", "Lvj", "Amu"))
+
+
+def test_sa_adiabatic():
+    c = cases.control_sa_naca(iters=10, Re=1e5, Ma=0.5)
+    run_both(naca_case("r_sa", c, ds1=1e-4), 3, ("U", "Res", "dt", "Amu_t"))
+
+
+def test_sst_isothermal():
+    c = cases.control_sa_naca(iters=10, Re=1e5, Ma=0.5)
+    c.Iflag_turbulence_model, c.Twall = F.TURB_SST, 300.0
+    run_both(naca_case("r_sst", c, ds1=1e-4), 3, ("U", "Res", "dt", "Amu_t"))
+
+
+def _symm(bcs):
+    for subs in bcs:
+        for s in subs:
+            if s[6] == F.BC_FARFIELD and s[1] in (2, 4):
+                s[6] = F.BC_SYMMETRY
+
+
+def _inout(bcs):
+    for subs in bcs:
+        for s in subs:
+            if s[6] == F.BC_FARFIELD and s[1] == 1:
+                s[6] = F.BC_INLET
+            if s[6] == F.BC_FARFIELD and s[1] == 3:
+                s[6] = F.BC_OUTLET
+
+
+@pytest.mark.parametrize("variant", ["farfield", "symmetry", "inlet_outlet"])
+def test_multiblock(variant):
+    c = cases.control_vortex(iters=10)
+    edit = {"farfield": None, "symmetry": _symm, "inlet_outlet": _inout}[variant]
+    if variant == "inlet_outlet":
+        c.P_outlet = 1.0 / (1.4 * 0.5 * 0.5) * 0.98
+    run_both(vortex_case(f"r_mb_{variant}", c, bc_edit=edit), 3)
+
+
+@pytest.mark.parametrize("tm,nm,sa", [(F.TIME_RK3, 2, False), (F.TIME_RK3, 3, False), (F.TIME_EULER1, 2, False),
+                                      (F.TIME_EULER1, 3, False), (F.TIME_RK3, 3, True)])
+def test_multigrid(tm, nm, sa):
+    c = cases.control_sa_naca(iters=10, Re=1e5, Ma=0.5) if sa else cases.control_euler_naca(iters=10)
+    c.Time_Method, c.Num_Mesh, c.CFL = tm, nm, 0.3
+    run_both(naca_case(f"r_mg_{tm}_{nm}_{int(sa)}", c, ds1=1e-4 if sa else 1e-3), 2, ("U", "Res"))
+
+
+def test_multigrid_pre_steps():
+    c = cases.control_euler_naca(iters=10)
+    c.Time_Method, c.Num_Mesh, c.CFL, c.Pre_Step_Mesh = F.TIME_RK3, 3, 0.3, [0, 2, 3]
+    run_both(naca_case("r_mg_pre", c), 2, ("U", "Res"))
+
+
+@pytest.mark.parametrize("sa", [False, True])
+def test_fdm_weno5(sa):
+    c = cases.control_sa_naca(iters=10, Re=1e5, Ma=0.5) if sa else cases.control_euler_naca(iters=10)
+    run_both(naca_case(f"r_fdm_{int(sa)}", c, ds1=1e-4 if sa else 1e-3, fdm=(1,)), 2)
+
+
+def test_lusgs_with_dual_time_diagonal():
+    """du_LU_SGS_2D alone with alfa1 != 0 (the dual-time diagonal term)."""
+    c = cases.control_sa_naca(iters=10, Re=1e5, Ma=0.5)
+    d = naca_case("r_lusgs_alfa", c, ds1=1e-4)
+    R = refpy.RefCase(d)
+    oc = O.OracleCase(d)
+    oc.init_flow()
+    R.op("main_step")
+    oc.main_step()
+    R.op("residual", 1)
+    oc.residual(1)
+    for c8 in (0, 3, 40):  # the harness passes alfa1 = c8 / 8
+        R.op("lusgs", 1, 1, c8)
+        oc.lusgs(1, 1, c8 / 8.0)
+        assert first_diff(oc.block().dU, R.get("dU")) is None, c8
+
+
+@pytest.mark.parametrize("fdm", [False, True])
+def test_geometry_matches_reference(fdm):
+    """Cell geometry, ghost coordinates, wall distance and finite-difference
+    metrics built in Python (ec2d.geometry) are the reference's
+    (sub_init.f90:180-204, sub_boundary.f90:275-280, sub_turbulence_SST.f90:274-365,
+    sub_Finite_Difference.f90 metrics) bit for bit, on every multigrid level."""
+    c = cases.control_sa_naca(iters=10, Re=1e5, Ma=0.5)
+    c.Time_Method, c.Num_Mesh = F.TIME_RK3, 3
+    d = naca_case(f"r_geom_{int(fdm)}", c, ds1=1e-4, fdm=(1,) if fdm else ())
+    R = refpy.RefCase(d)
+    oc = O.OracleCase(d)
+    names = ["vol", "si", "sj", "ni1", "ni2", "nj1", "nj2", "x", "y", "x1", "y1"]
+    for lev in range(1, 4):
+        g = oc.block(lev, 1).g
+        for n in names + (["dw"] if lev == 1 else []) + (["Akx", "Aky", "Aix", "Aiy", "Ajac"] if fdm and lev == 1
+                                                           else []):
+            want = R.get(n, lev, 1)
+            got = np.asarray(g.fd[n] if n in g.fd else getattr(g, n))
+            # only the entries the reference defines (sub_init.f90:184-204):
+            # the rest of these arrays is never written there
+            if n in ("si", "ni1", "ni2"):
+                got, want = got[:-1, :], want[:-1, :]
+            elif n in ("sj", "nj1", "nj2"):
+                got, want = got[:, :-1], want[:, :-1]
+            elif n == "vol":
+                got, want = got[:-1, :-1], want[:-1, :-1]
+            diff = first_diff(got, want)
+            assert diff is None, f"level {lev} {n}: {diff}"
