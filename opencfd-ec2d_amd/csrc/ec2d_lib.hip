@@ -62,6 +62,7 @@ struct DLevel {
   int* dblocks_all = nullptr;
   std::vector<int> case_first_block;
   int lusgs_threads = 64;
+  int lusgs_lpc = 1;          // LU-SGS lanes per cell: 2 when every block has at most 32 rows
   double* sk = nullptr;       // LU-SGS records (skewed, see ec2d_lusgs.hip)
   int64_t sk_max_n = 0;       // largest record field of a block (doubles)
   int max_cells = 0;          // largest block (cells)
@@ -234,17 +235,20 @@ int launch_lusgs(int level, const int* dlist, int nblk, double alfa1, int update
   dim3 gprep((unsigned)((L.sk_max_n + 255) / 256), nblk), gsweep(nblk), blk(L.lusgs_threads);
   dim3 gscat((unsigned)((L.max_cells + 255) / 256), nblk);
   int nv = L.o.nvar;
-  // up to 256 rows: 2-step prefetch ring (one wave per SIMD, registers to
-  // spare); larger blocks: up to 16 waves and one record ahead
-  const bool big = L.lusgs_threads > 256;
+  // up to 4 waves: one wave per SIMD and two steps per register set; more
+  // waves (up to 16): one step per set
+  const int nth = L.lusgs_threads;
+  const bool pair = L.lusgs_lpc == 2;
   const int pk = 10 * level;
 #define LLAUNCH(NV, V)                                                                      \
   prof_mark(4 + pk);                                                                        \
   hipLaunchKernelGGL((k_lusgs_prep<NV, V>), gprep, dim3(256), 0, R.s, a);                   \
   prof_mark(-(4 + pk));                                                                     \
   prof_mark(5 + pk);                                                                        \
-  if (big) hipLaunchKernelGGL((k_lusgs_sweep<NV, V, 1, 1024>), gsweep, blk, 0, R.s, a);     \
-  else hipLaunchKernelGGL((k_lusgs_sweep<NV, V, 2, 256>), gsweep, blk, 0, R.s, a);          \
+  if (pair) hipLaunchKernelGGL((k_lusgs_sweep<NV, V, 2, 64, 2>), gsweep, blk, 0, R.s, a);    \
+  else if (nth > 512) hipLaunchKernelGGL((k_lusgs_sweep<NV, V, 1, 1024, 1>), gsweep, blk, 0, R.s, a); \
+  else if (nth > 256) hipLaunchKernelGGL((k_lusgs_sweep<NV, V, 1, 512, 1>), gsweep, blk, 0, R.s, a); \
+  else hipLaunchKernelGGL((k_lusgs_sweep<NV, V, 2, 256, 1>), gsweep, blk, 0, R.s, a);       \
   prof_mark(-(5 + pk));                                                                     \
   prof_mark(6 + pk);                                                                        \
   hipLaunchKernelGGL((k_lusgs_scatter<NV, V>), gscat, dim3(256), 0, R.s, a);                \
@@ -823,17 +827,22 @@ int ec2d_commit(void) {
     }
     L.case_first_block.push_back((int)L.hb.size());
     L.arena_n = off;
-    L.lusgs_threads = 64 * ((max_rows + 63) / 64);
+    // two lanes per cell when one wave holds every block's rows that way;
+    // otherwise one lane per row (fewer waves, fewer barriers per step)
+    L.lusgs_lpc = max_rows <= 32 ? 2 : 1;
+    L.lusgs_threads = 64 * ((max_rows * L.lusgs_lpc + 63) / 64);
     {
       const bool visc = R.cases[0].cp.If_viscous == 1;
-      const int64_t per = 2 * lusgs_nf(nv, visc) + nv;
+      const int64_t per = 2 * lusgs_rec_doubles(nv, visc) + nv;
       int64_t sk_off = 0;
       for (auto& d : L.hb) {
         const int ncol = d.nx - 1, nrow = d.ny - 1;
         d.sk_S = ncol + nrow - 1;
-        d.sk_R = 64 * ((nrow + 63) / 64);
-        d.sk_n = (int64_t)d.sk_S * d.sk_R;
+        const int rq = 64 / L.lusgs_lpc;  // rows per wave
+        d.sk_R = rq * ((nrow + rq - 1) / rq);
+        d.sk_n = (int64_t)d.sk_S * d.sk_R + 64;
         d.sk_base = sk_off;
+        if (per * d.sk_n * 8 >= (int64_t)1 << 31) return fail("block too large for the LU-SGS record buffer (2 GiB)");
         sk_off += (per * d.sk_n + 31) / 32 * 32;
         L.sk_max_n = std::max(L.sk_max_n, d.sk_n);
         L.max_cells = std::max(L.max_cells, ncol * nrow);

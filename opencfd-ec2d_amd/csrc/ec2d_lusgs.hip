@@ -13,26 +13,33 @@
 //
 // Sweep order.  The forward sweep needs dU(i-1,j) and dU(i,j-1), the backward
 // sweep dU(i+1,j) and dU(i,j+1).  Any order that respects these dependencies
-// gives the reference's result bit for bit.  One lane owns one row of cells
-// and step s handles hyperplane s (sub_LU_SGS.f90:25-29, 70-75): lane r takes
-// the cell of its row at column offset c = s - r.  The i-neighbour's result
-// is the lane's own from the step before (registers); the j-neighbour's is
-// that of lane r-1 from the step before, moved one lane up with a DPP wave
-// shift, or through LDS between 64-row waves.
+// gives the reference's result bit for bit.  Two lanes own one row of cells
+// and step s handles hyperplane s (sub_LU_SGS.f90:25-29, 70-75): row t takes
+// its cell at column offset c = s - t.  Of the pair, one lane evaluates the
+// j-neighbour's flux difference and the other the i-neighbour's; they swap
+// the two halves with a DPP quad permute and both finish the cell.  The
+// i-neighbour's result is the row's own from the step before (registers); the
+// j-neighbour's is that of row t-1 from the step before, one lane down (DPP
+// wave shift), or through LDS between waves.
 //
 // Record layout ("skewed").  A block's records are stored hyperplane by
-// hyperplane: field f of the cell that lane r handles at step s sits at
-// f*n + s*R + r, with R the row count rounded up to 64 and n = S*R for
-// S = ncol + nrow - 1 steps.  Every load of a step is then one contiguous run
-// per wave instead of 64 scattered lines, and the sweep is bound by its
-// dependent arithmetic, not by address divergence.  The forward record of
-// (s, r) belongs to cell (c+1, r+1), the backward record to cell
-// (ncol-c, nrow-r): both sweeps use the same neighbour tests (c != 0 for the
-// i-neighbour, r != 0 for the j-neighbour) and the same index arithmetic.
+// hyperplane: shared field f of the cell that row t handles at step s sits at
+// f*n + s*R + t, with R the row count rounded up to 32 (the rows of one wave)
+// and n = S*R for S = ncol + nrow - 1 steps; per-side fields are interleaved
+// by side.  Every load of a step is one contiguous run per wave (buffer loads
+// with one lane offset and a scalar offset per field), so the sweep is bound
+// by its dependent arithmetic, not by address divergence.  The forward record
+// of (s, t) belongs to cell (c+1, t+1), the backward record to cell
+// (ncol-c, nrow-t): both sweeps use the same neighbour tests (c != 0 for the
+// i-neighbour, t != 0 for the j-neighbour) and the same index arithmetic.
 //
-// Per block the records occupy (2*NF + NV) * n doubles: the forward set, the
-// backward set (whose X fields receive the forward dU from the forward
-// sweep), and NV output fields for the backward dU.
+// Per block the records occupy (2*NREC + NV) * n doubles: the forward set,
+// the backward set (whose X fields receive the forward dU from the forward
+// sweep), and NV output fields for the backward dU.  The field stride n is
+// S*R + 64: the last 64 slots of every field are a trash pad that lanes
+// without a cell to store write to, so the sweep's stores are unconditional
+// (a store under a branch makes the compiler drain every outstanding load,
+// the prefetch ring included, before the next use of a loaded register).
 #include "ec2d_device.h"
 #include "ec2d_physics.h"
 
@@ -49,34 +56,43 @@ struct LusgsArgs {
   int update;             // scatter: 1 also adds dU to U
 };
 
-// field offsets of one record (in units of n)
+// Field layout of one record set (one sweep direction) of a block.  A
+// record has a part shared by the two lanes of a cell and a part per side
+// (k = 0: the j-neighbour, k = 1: the i-neighbour), interleaved by side so
+// that the two lanes of a cell read adjacent doubles:
+//   shared field f of position pos:  f*n + pos
+//   side field g of position pos:    NSH*n + g*2n + 2*pos + k
 template <int NV, bool VISC>
 struct RecF {
-  static constexpr int X = 0, U = NV, FI = 2 * NV, FJ = 3 * NV;
-  static constexpr int N1I = 4 * NV, N2I = N1I + 1, N1J = N1I + 2, N2J = N1I + 3;
-  static constexpr int AI = N1I + 4, AJ = AI + 1, LCI = AI + 2, LCJ = AI + 3;
-  static constexpr int LVI = AI + 4, LVJ = AI + 5;  // VISC only
-  static constexpr int AL0 = VISC ? AI + 6 : AI + 4;
-  static constexpr int AL4 = AL0 + 1;               // NV == 5 only
-  static constexpr int NF = AL0 + (NV == 5 ? 2 : 1);
+  // shared: Res (forward) or the forward dU (backward), U, the diagonal
+  static constexpr int X = 0, U = NV, AL0 = 2 * NV, AL4 = 2 * NV + 1;
+  static constexpr int NSH = 2 * NV + (NV == 5 ? 2 : 1);
+  // per side: F(U_nb).n, the normal, the face area, the neighbour's inviscid
+  // and viscous spectral radius
+  static constexpr int F = 0, N1 = NV, N2 = NV + 1, A = NV + 2, LC = NV + 3, LV = NV + 4;
+  static constexpr int NSD = NV + 4 + (VISC ? 1 : 0);
+  static constexpr int NREC = NSH + 2 * NSD;  // doubles per position
 };
 
-// host-side record size, equal to RecF<nv, visc>::NF
-inline int lusgs_nf(int nv, bool visc) { return 4 * nv + 8 + (visc ? 2 : 0) + (nv == 5 ? 2 : 1); }
+// host-side record size in doubles per position, equal to RecF<nv, visc>::NREC
+inline int lusgs_rec_doubles(int nv, bool visc) { return 2 * nv + (nv == 5 ? 2 : 1) + 2 * (nv + 4 + (visc ? 1 : 0)); }
 
-template <int NV>
+// operands of one lane for one cell: the shared part and NS sides (NS = 1:
+// the lane's own side k, NS = 2: side 0 (j) and side 1 (i))
+template <int NV, int NS>
 struct Rec {
-  double X[NV], U[NV], Fi[NV], Fj[NV];
-  double n1i, n2i, n1j, n2j, ai, aj, Lci, Lcj, Lvi, Lvj, al0, al4;
+  double X[NV], U[NV];
+  double F[NS][NV], n1[NS], n2[NS], a[NS], Lc[NS], Lv[NS];
+  double al0, al4;
 };
 
-// result of one cell as its neighbours need it: dU and U + dU
+// one cell as its neighbours need it: dU and U + dU
 template <int NV>
 struct Out {
   double du[NV], W[NV];
 };
 
-// lane r-1 -> lane r (lane 0 receives 0)
+// lane l-1 -> lane l (lane 0 receives 0)
 __device__ __forceinline__ double shr1(double v) {
   int lo = __double2loint(v), hi = __double2hiint(v);
   lo = __builtin_amdgcn_update_dpp(0, lo, 0x138, 0xF, 0xF, false);
@@ -84,160 +100,157 @@ __device__ __forceinline__ double shr1(double v) {
   return __hiloint2double(hi, lo);
 }
 
-template <int NV>
-__device__ __forceinline__ void shift_out(Out<NV>& dst, const Out<NV>& src) {
-#pragma unroll
-  for (int m = 0; m < NV; m++) {
-    dst.du[m] = shr1(src.du[m]);
-    dst.W[m] = shr1(src.W[m]);
-  }
+// lane l <-> lane l^1 (quad_perm [1,0,3,2])
+__device__ __forceinline__ double swap_pair(double v) {
+  int lo = __double2loint(v), hi = __double2hiint(v);
+  lo = __builtin_amdgcn_update_dpp(0, lo, 0xB1, 0xF, 0xF, false);
+  hi = __builtin_amdgcn_update_dpp(0, hi, 0xB1, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
 }
 
-template <int NV, bool VISC>
-__device__ __forceinline__ void load_rec(Rec<NV>& p, const double* __restrict__ rb, int64_t n, int64_t pos) {
+__device__ __forceinline__ double bload(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+
+__device__ __forceinline__ void bstore(double v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, v), r, voff,
+                                        soff, 0);
+}
+
+// base: byte offset of the record set in the block's record region; NS = 1
+// loads side k, NS = 2 both sides
+template <int NV, bool VISC, int NS>
+__device__ __forceinline__ void load_rec(Rec<NV, NS>& p, __amdgpu_buffer_rsrc_t r, uint32_t base, uint32_t n8,
+                                         uint32_t pos, int k) {
   using F = RecF<NV, VISC>;
-  const double* b = rb + pos;
+  const uint32_t vs = pos * 8u;
+  const uint32_t sb = base + F::NSH * n8;  // side part
 #pragma unroll
   for (int m = 0; m < NV; m++) {
-    p.X[m] = b[(F::X + m) * n];
-    p.U[m] = b[(F::U + m) * n];
-    p.Fi[m] = b[(F::FI + m) * n];
-    p.Fj[m] = b[(F::FJ + m) * n];
+    p.X[m] = bload(r, vs, base + (F::X + m) * n8);
+    p.U[m] = bload(r, vs, base + (F::U + m) * n8);
   }
-  p.n1i = b[F::N1I * n]; p.n2i = b[F::N2I * n]; p.n1j = b[F::N1J * n]; p.n2j = b[F::N2J * n];
-  p.ai = b[F::AI * n]; p.aj = b[F::AJ * n]; p.Lci = b[F::LCI * n]; p.Lcj = b[F::LCJ * n];
-  if constexpr (VISC) {
-    p.Lvi = b[F::LVI * n];
-    p.Lvj = b[F::LVJ * n];
-  } else {
-    p.Lvi = p.Lvj = 0.0;
+#pragma unroll
+  for (int h = 0; h < NS; h++) {
+    const uint32_t vd = (2u * pos + (uint32_t)(NS == 1 ? k : h)) * 8u;
+#pragma unroll
+    for (int m = 0; m < NV; m++) p.F[h][m] = bload(r, vd, sb + (F::F + m) * 2u * n8);
+    p.n1[h] = bload(r, vd, sb + F::N1 * 2u * n8);
+    p.n2[h] = bload(r, vd, sb + F::N2 * 2u * n8);
+    p.a[h] = bload(r, vd, sb + F::A * 2u * n8);
+    p.Lc[h] = bload(r, vd, sb + F::LC * 2u * n8);
+    if constexpr (VISC) p.Lv[h] = bload(r, vd, sb + F::LV * 2u * n8);
+    else p.Lv[h] = 0.0;
   }
-  p.al0 = b[F::AL0 * n];
-  if constexpr (NV == 5) p.al4 = b[F::AL4 * n];
+  p.al0 = bload(r, vs, base + F::AL0 * n8);
+  if constexpr (NV == 5) p.al4 = bload(r, vs, base + F::AL4 * n8);
   else p.al4 = p.al0;
 }
 
 // ----------------------------------------------------------------- prep
-// One record.  FWD: cell (c+1, r+1), neighbours (i-1,j) and (i,j-1); the
-// face areas are the cell's own (sub_LU_SGS.f90:43-56).  Backward: cell
-// (ncol-c, nrow-r), neighbours (i+1,j) and (i,j+1), areas of the neighbour,
-// normals of the cell itself (sub_LU_SGS.f90:86-99).
+// One record set at position (c, r).  Forward: cell (c+1, r+1), neighbours
+// (i-1,j) and (i,j-1); the face areas are the cell's own (sub_LU_SGS.f90:43-56).
+// Backward: cell (ncol-c, nrow-r), neighbours (i+1,j) and (i,j+1), areas of the
+// neighbour, normals of the cell itself (sub_LU_SGS.f90:86-99).
 template <int NV, bool VISC, bool FWD>
-__device__ __forceinline__ void write_rec(double* __restrict__ rb, int64_t n, const BlockDesc& B, const double* base,
-                                          const PlaneMap& pm, int c, int r, double g, double alfa1) {
+__device__ __forceinline__ void write_rec(double* __restrict__ rb, int64_t n, int64_t pos, const BlockDesc& B,
+                                          const double* base, const PlaneMap& pm, int c, int r, double g,
+                                          double alfa1) {
   using F = RecF<NV, VISC>;
   const int ncol = B.nx - 1, nrow = B.ny - 1;
+  double* sh = rb + pos;                        // shared field f: sh[f*n]
+  double* sd = rb + F::NSH * n + 2 * pos;       // side field g, side k: sd[g*2n + k]
   auto PL = [&](int slot) { return base + (int64_t)slot * B.plane; };
   if (r >= nrow || c < 0 || c >= ncol) {
     // padding position: never active in a sweep
-#pragma unroll 4
-    for (int f = FWD ? 0 : NV; f < F::NF; f++) rb[f * n] = 0.0;
-    if (!FWD) {
 #pragma unroll
-      for (int m = 0; m < NV; m++) rb[(F::X + m) * n] = 0.0;
-    }
+    for (int f = 0; f < F::NSH; f++) sh[f * n] = 0.0;
+#pragma unroll
+    for (int f = 0; f < F::NSD; f++) sd[f * 2 * n] = sd[f * 2 * n + 1] = 0.0;
     return;
   }
   const int i = FWD ? c + 1 : ncol - c;
   const int j = FWD ? r + 1 : nrow - r;
   const int d = FWD ? -1 : 1;
-  const bool hs = c != 0, hl = r != 0;
   const int64_t q = pidx(B, i, j);
-  const int64_t qs = pidx(B, hs ? i + d : i, j);
-  const int64_t ql = pidx(B, i, hl ? j + d : j);
-  double U[NV];
 #pragma unroll
   for (int m = 0; m < NV; m++) {
-    U[m] = PL(pm.U + m)[q];
-    rb[(F::U + m) * n] = U[m];
-    if (FWD) rb[(F::X + m) * n] = PL(pm.Res + m)[q];
+    sh[(F::U + m) * n] = PL(pm.U + m)[q];
+    if (FWD) sh[(F::X + m) * n] = PL(pm.Res + m)[q];
   }
-  const double n1i = PL(pm.ni1)[q], n2i = PL(pm.ni2)[q], n1j = PL(pm.nj1)[q], n2j = PL(pm.nj2)[q];
-  rb[F::N1I * n] = n1i; rb[F::N2I * n] = n2i; rb[F::N1J * n] = n1j; rb[F::N2J * n] = n2j;
-  double Us[NV], Ul[NV], Fi[NV], Fj[NV];
-#pragma unroll
-  for (int m = 0; m < NV; m++) {
-    Us[m] = PL(pm.U + m)[qs];
-    Ul[m] = PL(pm.U + m)[ql];
-  }
-  flux_n_old<NV>(Fi, Us, n1i, n2i, g);
-  flux_n_old<NV>(Fj, Ul, n1j, n2j, g);
-#pragma unroll
-  for (int m = 0; m < NV; m++) {
-    rb[(F::FI + m) * n] = hs ? Fi[m] : 0.0;
-    rb[(F::FJ + m) * n] = hl ? Fj[m] : 0.0;
-  }
-  const double* si = PL(pm.si);
-  const double* sj = PL(pm.sj);
-  rb[F::AI * n] = hs ? (FWD ? si[q] : si[qs]) : 0.0;
-  rb[F::AJ * n] = hl ? (FWD ? sj[q] : sj[ql]) : 0.0;
   const double* Lci = PL(pm.Lci);
   const double* Lcj = PL(pm.Lcj);
-  rb[F::LCI * n] = hs ? Lci[qs] : 0.0;
-  rb[F::LCJ * n] = hl ? Lcj[ql] : 0.0;
   const double* Lvi = PL(pm.Lvi);
   const double* Lvj = PL(pm.Lvj);
-  if constexpr (VISC) {
-    rb[F::LVI * n] = hs ? Lvi[qs] : 0.0;
-    rb[F::LVJ * n] = hl ? Lvj[ql] : 0.0;
-  }
   // diagonal (sub_LU_SGS.f90:31-39, 76-83)
   const double vol = PL(pm.vol)[q], dt = PL(pm.dt)[q];
   double alfa = vol / dt + 1.0 * (Lci[q] + Lcj[q]) + vol * alfa1;
   if constexpr (VISC) alfa = alfa + 2.0 * (Lvi[q] + Lvj[q]);
-  rb[F::AL0 * n] = alfa;
-  if constexpr (NV == 5) rb[F::AL4 * n] = alfa + Lvi[q] + Lvj[q];
+  sh[F::AL0 * n] = alfa;
+  if constexpr (NV == 5) sh[F::AL4 * n] = alfa + Lvi[q] + Lvj[q];
+  // sides: k = 0 the j-neighbour, k = 1 the i-neighbour
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    const bool has = k == 0 ? r != 0 : c != 0;
+    double* o = sd + k;
+    if (!has) {
+#pragma unroll
+      for (int f = 0; f < F::NSD; f++) o[f * 2 * n] = 0.0;
+      continue;
+    }
+    const int64_t qn = k == 0 ? pidx(B, i, j + d) : pidx(B, i + d, j);
+    const double n1 = PL(k == 0 ? pm.nj1 : pm.ni1)[q], n2 = PL(k == 0 ? pm.nj2 : pm.ni2)[q];
+    double Un[NV], Fo[NV];
+#pragma unroll
+    for (int m = 0; m < NV; m++) Un[m] = PL(pm.U + m)[qn];
+    flux_n_old<NV>(Fo, Un, n1, n2, g);
+#pragma unroll
+    for (int m = 0; m < NV; m++) o[(F::F + m) * 2 * n] = Fo[m];
+    o[F::N1 * 2 * n] = n1;
+    o[F::N2 * 2 * n] = n2;
+    const double* sa = PL(k == 0 ? pm.sj : pm.si);
+    o[F::A * 2 * n] = FWD ? sa[q] : sa[qn];
+    o[F::LC * 2 * n] = (k == 0 ? Lcj : Lci)[qn];
+    if constexpr (VISC) o[F::LV * 2 * n] = (k == 0 ? Lvj : Lvi)[qn];
+  }
 }
 
 template <int NV, bool VISC>
 __global__ __launch_bounds__(256) void k_lusgs_prep(LusgsArgs a) {
+  using F = RecF<NV, VISC>;
   const BlockDesc B = a.blocks[a.block_list[blockIdx.y]];
   const int64_t n = B.sk_n;
   const int64_t pos = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (pos >= n) return;
+  if (pos >= (int64_t)B.sk_S * B.sk_R) return;
   const int s = (int)(pos / B.sk_R), r = (int)(pos - (int64_t)s * B.sk_R);
   const double g = a.cps[B.case_id].gamma;
   const double* base = a.arena + B.base;
-  double* fw = a.sk + B.sk_base + pos;
-  write_rec<NV, VISC, true>(fw, n, B, base, a.pm, s - r, r, g, a.alfa1);
-  write_rec<NV, VISC, false>(fw + RecF<NV, VISC>::NF * n, n, B, base, a.pm, s - r, r, g, a.alfa1);
+  double* fw = a.sk + B.sk_base;
+  write_rec<NV, VISC, true>(fw, n, pos, B, base, a.pm, s - r, r, g, a.alfa1);
+  write_rec<NV, VISC, false>(fw + F::NREC * n, n, pos, B, base, a.pm, s - r, r, g, a.alfa1);
+  if (r >= B.ny - 1 || s - r < 0 || s - r >= B.nx - 1) {
+    // the backward X fields of padding positions (valid ones come from the forward sweep)
+#pragma unroll
+    for (int m = 0; m < NV; m++) fw[F::NREC * n + (F::X + m) * n + pos] = 0.0;
+  }
 }
 
 // ----------------------------------------------------------------- sweeps
-// one cell; FWD: dU = (Res + dui + duj) / alfa, backward:
-// dU = dU + (dui + duj) / alfa (sub_LU_SGS.f90:41-62, 85-105)
-template <int NV, bool VISC, bool FWD>
-__device__ __forceinline__ void lusgs_cell(const Rec<NV>& p, const Out<NV>& side, const Out<NV>& low, bool has_side,
-                                           bool has_low, double g, Out<NV>& o) {
-  double DFi[NV], DFj[NV];
-  dfn_from_old<NV>(DFi, side.W, p.Fi, p.n1i, p.n2i, g);
-  dfn_from_old<NV>(DFj, low.W, p.Fj, p.n1j, p.n2j, g);
-  double dui[NV], duj[NV];
+// One side's share of a cell, dui (the i-neighbour) or duj (the
+// j-neighbour), of sub_LU_SGS.f90:42-58 (forward) or :85-100 (backward);
+// h selects the side held in the record.
+template <int NV, bool VISC, bool FWD, int NS>
+__device__ __forceinline__ void side_term(double* d, const Rec<NV, NS>& p, int h, const Out<NV>& nb, bool has,
+                                          double g) {
+  double DF[NV];
+  dfn_from_old<NV>(DF, nb.W, p.F[h], p.n1[h], p.n2[h], g);
 #pragma unroll
   for (int m = 0; m < NV; m++) {
-    double di, dj;
-    if (FWD) {
-      di = 0.5 * (DFi[m] * p.ai + 1.0 * p.Lci * side.du[m]);
-      dj = 0.5 * (DFj[m] * p.aj + 1.0 * p.Lcj * low.du[m]);
-    } else {
-      di = -0.5 * (DFi[m] * p.ai - 1.0 * p.Lci * side.du[m]);
-      dj = -0.5 * (DFj[m] * p.aj - 1.0 * p.Lcj * low.du[m]);
-    }
-    if constexpr (VISC) {
-      di = di + p.Lvi * side.du[m];
-      dj = dj + p.Lvj * low.du[m];
-    }
-    dui[m] = has_side ? di : 0.0;
-    duj[m] = has_low ? dj : 0.0;
-  }
-#pragma unroll
-  for (int m = 0; m < NV; m++) {
-    const double al = (NV == 5 && m == 4) ? p.al4 : p.al0;
-    double du;
-    if (FWD) du = (p.X[m] + dui[m] + duj[m]) / al;
-    else du = p.X[m] + (dui[m] + duj[m]) / al;
-    o.du[m] = du;
-    o.W[m] = p.U[m] + du;
+    double v;
+    if (FWD) v = 0.5 * (DF[m] * p.a[h] + 1.0 * p.Lc[h] * nb.du[m]);
+    else v = -0.5 * (DF[m] * p.a[h] - 1.0 * p.Lc[h] * nb.du[m]);
+    if constexpr (VISC) v = v + p.Lv[h] * nb.du[m];
+    d[m] = has ? v : 0.0;
   }
 }
 
@@ -253,54 +266,136 @@ __device__ __forceinline__ void lds_barrier() {
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
-// One sweep over the records rec (forward or backward set).  Each lane
-// loads the record it will need D steps later into a ring of registers while
-// it computes the current one; loads past the last step are clamped to it.
-// The cell's dU goes to dst: for the forward sweep the X field of the
-// backward record of the same cell, for the backward sweep the output fields.
-template <int NV, bool VISC, bool FWD, int D>
-__device__ __forceinline__ void sweep(const double* __restrict__ rec, double* __restrict__ dst, int64_t n, int S,
-                                      int R, int ncol, int nrow, int r, int lane, int wave, bool multi, double g,
-                                      double (*xfer)[16][2 * NV]) {
-  const int npad = (S + D - 1) / D * D;
-  const int rl = min(r, R - 1);  // loads of lanes beyond the block stay in its records
-  Out<NV> side = {}, low = {}, o;
-  Rec<NV> ring[D];
+// One step of a sweep: the cell of row t at column offset c = s - t.
+// LPC = 2: lanes 2t (k = 0, the j-neighbour's term) and 2t+1 (k = 1, the
+// i-neighbour's term) share the cell and swap their halves; LPC = 1: lane t
+// computes both.
+template <int NV, bool VISC, bool FWD, int LPC>
+__device__ __forceinline__ void sweep_step(const Rec<NV, 3 - LPC>& p, Out<NV>& own, __amdgpu_buffer_rsrc_t rs,
+                                           uint32_t dst, uint32_t n8, int s, int S, int R, int ncol, int nrow, int t,
+                                           int k, int lane, int wave, bool multi, double g,
+                                           double (*xfer)[16][2 * NV]) {
+  // the j-neighbour's result: row t-1's, one lane down (the last lane of the
+  // previous wave through LDS); the i-neighbour's is the own row's
+  Out<NV> up;
 #pragma unroll
-  for (int d = 0; d < D; d++) load_rec<NV, VISC>(ring[d], rec, n, (int64_t)min(d, S - 1) * R + rl);
-  for (int s0 = 0; s0 < npad; s0 += D) {
+  for (int m = 0; m < NV; m++) {
+    up.du[m] = shr1(own.du[m]);
+    up.W[m] = shr1(own.W[m]);
+  }
+  if (multi && wave > 0) {
+    // every lane reads the previous wave's last row (an LDS broadcast),
+    // lane 0 keeps it
+    const double* x = xfer[(s + 1) & 1][wave - 1];
 #pragma unroll
-    for (int d = 0; d < D; d++) {
-      const int s = s0 + d;
-      shift_out<NV>(low, side);
-      if (multi && lane == 0 && wave > 0) {
-        const double* x = xfer[(s + 1) & 1][wave - 1];
-#pragma unroll
-        for (int m = 0; m < NV; m++) { low.du[m] = x[m]; low.W[m] = x[NV + m]; }
-      }
-      const int c = s - r;
-      const bool act = r < nrow && c >= 0 && c < ncol;
-      lusgs_cell<NV, VISC, FWD>(ring[d], side, low, c != 0, r != 0, g, o);
-      if (act) {
-        const int64_t w = FWD ? (int64_t)(S - 1 - s) * R + (nrow - 1 - r) : (int64_t)s * R + r;
-#pragma unroll
-        for (int m = 0; m < NV; m++) dst[m * n + w] = o.du[m];
-        side = o;
-      }
-      load_rec<NV, VISC>(ring[d], rec, n, (int64_t)min(s + D, S - 1) * R + rl);
-      if (multi) {
-        if (lane == 63) {
-          double* x = xfer[s & 1][wave];
-#pragma unroll
-          for (int m = 0; m < NV; m++) { x[m] = side.du[m]; x[NV + m] = side.W[m]; }
-        }
-        lds_barrier();
-      }
+    for (int m = 0; m < NV; m++) {
+      const double xd = x[m], xw = x[NV + m];
+      up.du[m] = lane == 0 ? xd : up.du[m];
+      up.W[m] = lane == 0 ? xw : up.W[m];
     }
+  }
+  const int c = s - t;
+  const bool act = t < nrow && c >= 0 && c < ncol;
+  double dui[NV], duj[NV];
+  if constexpr (LPC == 2) {
+    Out<NV> nb;
+#pragma unroll
+    for (int m = 0; m < NV; m++) {
+      nb.du[m] = k == 0 ? up.du[m] : own.du[m];
+      nb.W[m] = k == 0 ? up.W[m] : own.W[m];
+    }
+    double mine[NV];
+    side_term<NV, VISC, FWD, 1>(mine, p, 0, nb, k == 0 ? t != 0 : c != 0, g);
+#pragma unroll
+    for (int m = 0; m < NV; m++) {
+      const double other = swap_pair(mine[m]);
+      dui[m] = k == 1 ? mine[m] : other;
+      duj[m] = k == 0 ? mine[m] : other;
+    }
+  } else {
+    side_term<NV, VISC, FWD, 2>(duj, p, 0, up, t != 0, g);
+    side_term<NV, VISC, FWD, 2>(dui, p, 1, own, c != 0, g);
+  }
+  Out<NV> o;
+#pragma unroll
+  for (int m = 0; m < NV; m++) {
+    const double al = (NV == 5 && m == 4) ? p.al4 : p.al0;
+    double du;
+    if (FWD) du = (p.X[m] + dui[m] + duj[m]) / al;
+    else du = p.X[m] + (dui[m] + duj[m]) / al;
+    o.du[m] = du;
+    o.W[m] = p.U[m] + du;
+  }
+  // Every lane keeps its result, active or not: an inactive lane's value
+  // never reaches an active cell.  Row t's first cell (c = 0) ignores its
+  // i-neighbour; row t+1 at step s+1 sits at the same column offset as row t
+  // at step s, so it is inactive whenever row t was.
+  own = o;
+  {
+    // one lane of an active cell stores; every other lane writes its own
+    // slot of the trash pad (positions S*R .. S*R+63)
+    const uint32_t w = (act && k == 0) ? (FWD ? (uint32_t)((S - 1 - s) * R + (nrow - 1 - t)) : (uint32_t)(s * R + t))
+                                       : (uint32_t)(S * R + lane);
+#pragma unroll
+    for (int m = 0; m < NV; m++) bstore(o.du[m], rs, w * 8u, dst + (uint32_t)m * n8);
+  }
+  if (multi) {
+    if (lane == 63) {
+      double* x = xfer[s & 1][wave];
+#pragma unroll
+      for (int m = 0; m < NV; m++) { x[m] = own.du[m]; x[NV + m] = own.W[m]; }
+    }
+    lds_barrier();
   }
 }
 
-template <int NV, bool VISC, int D, int MAXT>
+// One sweep.  With LPC = 2 lanes 2t and 2t+1 own row t: lane 2t computes
+// the j-neighbour's term, lane 2t+1 the i-neighbour's, the pair swaps them
+// (DPP) and both finish the cell, so both hold the cell's result.  With
+// LPC = 1 lane t owns row t alone.  The i-neighbour of a cell is the own
+// row's previous result (registers); the j-neighbour is the previous result
+// of row t-1, which the next lane down holds: one DPP shift up.
+// The records come in two register sets of U steps each, loaded one phase
+// ahead: [load B] [steps of A] [load A] [steps of B] ...  Whenever the code
+// waits for a set, the only loads still in flight are the other set's, issued
+// after it, so the waits the compiler places never drain a prefetch that is
+// not yet needed.  Loads past the last step are clamped to it.  The cell's dU goes to dst: for the forward sweep the X field of the
+// backward record of the same cell, for the backward sweep the output fields.
+template <int NV, bool VISC, bool FWD, int U, int LPC>
+__device__ __forceinline__ void sweep(__amdgpu_buffer_rsrc_t rs, uint32_t rec, uint32_t dst, uint32_t n8, int S, int R,
+                                      int ncol, int nrow, int t, int k, int lane, int wave, bool multi, double g,
+                                      double (*xfer)[16][2 * NV]) {
+  const int npad = (S + 2 * U - 1) / (2 * U) * (2 * U);
+  const int tl = min(t, R - 1);  // loads of lanes beyond the block stay in its records
+  constexpr int NS = 3 - LPC;
+  Out<NV> own = {};
+  Rec<NV, NS> A[U], B[U];
+  auto load_set = [&](Rec<NV, NS>* set, int s_first) {
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      load_rec<NV, VISC, NS>(set[u], rs, rec, n8, (uint32_t)(min(s_first + u, S - 1) * R + tl), k);
+  };
+  auto run_set = [&](const Rec<NV, NS>* set, int s_first) {
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      sweep_step<NV, VISC, FWD, LPC>(set[u], own, rs, dst, n8, s_first + u, S, R, ncol, nrow, t, k, lane, wave,
+                                     multi, g, xfer);
+  };
+  load_set(A, 0);
+  for (int s0 = 0; s0 < npad; s0 += 2 * U) {
+    load_set(B, s0 + U);
+    __builtin_amdgcn_sched_barrier(0);
+    run_set(A, s0);
+    __builtin_amdgcn_sched_barrier(0);
+    load_set(A, s0 + 2 * U);
+    __builtin_amdgcn_sched_barrier(0);
+    run_set(B, s0 + U);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+// LPC lanes per cell (1 or 2), U steps per register set, MAXT threads at most
+template <int NV, bool VISC, int U, int MAXT, int LPC>
 __global__ __launch_bounds__(MAXT) void k_lusgs_sweep(LusgsArgs a) {
   using F = RecF<NV, VISC>;
   __shared__ double xfer[2][16][2 * NV];
@@ -308,16 +403,18 @@ __global__ __launch_bounds__(MAXT) void k_lusgs_sweep(LusgsArgs a) {
   const double g = a.cps[B.case_id].gamma;
   const int ncol = B.nx - 1, nrow = B.ny - 1;
   const int S = B.sk_S, R = B.sk_R;
-  const int64_t n = B.sk_n;
+  const uint32_t n8 = (uint32_t)B.sk_n * 8u;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const bool multi = R > 64;
-  double* fw = a.sk + B.sk_base;
-  double* bw = fw + F::NF * n;
-  double* out = bw + F::NF * n;
-  sweep<NV, VISC, true, D>(fw, bw + F::X * n, n, S, R, ncol, nrow, tid, lane, wave, multi, g, xfer);
+  const int t = LPC == 2 ? tid >> 1 : tid, k = LPC == 2 ? tid & 1 : 0;
+  const bool multi = R > 64 / LPC;
+  // the block's record region: forward set, backward set, output fields
+  const uint32_t bytes = (uint32_t)((2 * F::NREC + NV) * B.sk_n * 8);
+  const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(a.sk + B.sk_base, 0, bytes, 0x00020000);
+  const uint32_t fw = 0, bw = F::NREC * n8, out = 2 * F::NREC * n8;
+  sweep<NV, VISC, true, U, LPC>(rs, fw, bw + F::X * n8, n8, S, R, ncol, nrow, t, k, lane, wave, multi, g, xfer);
   __threadfence();
   __syncthreads();
-  sweep<NV, VISC, false, D>(bw, out, n, S, R, ncol, nrow, tid, lane, wave, multi, g, xfer);
+  sweep<NV, VISC, false, U, LPC>(rs, bw, out, n8, S, R, ncol, nrow, t, k, lane, wave, multi, g, xfer);
 }
 
 // ----------------------------------------------------------------- scatter
@@ -330,7 +427,7 @@ __global__ __launch_bounds__(256) void k_lusgs_scatter(LusgsArgs a) {
   if (cidx >= ncol * nrow) return;
   const int j = cidx / ncol + 1, i = cidx - (j - 1) * ncol + 1;
   const int64_t n = B.sk_n;
-  const double* out = a.sk + B.sk_base + 2 * F::NF * n + (int64_t)(ncol - i + nrow - j) * B.sk_R + (nrow - j);
+  const double* out = a.sk + B.sk_base + 2 * F::NREC * n + (int64_t)(ncol - i + nrow - j) * B.sk_R + (nrow - j);
   double* base = a.arena + B.base;
   const int64_t q = pidx(B, i, j);
 #pragma unroll
