@@ -80,6 +80,21 @@ def kernel_bytes_per_cell(kernel, nv, viscous, sa):
 
 
 def cpu_baseline(case_dir, cells, budget_s=20.0, kind="reference"):
+    """The CPU baseline, measured in a child process (the reference's Fortran
+    runtime writes to the process's stdout at exit, which must not follow this
+    script's JSON line); None if the checker is not built."""
+    import subprocess
+
+    out = os.path.join(case_dir, "cpu_baseline.json")
+    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-baseline", case_dir, kind, str(cells), str(budget_s), out]
+    r = subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=600)
+    if r.returncode != 0 or not os.path.exists(out):
+        return None
+    with open(out) as f:
+        return json.load(f)
+
+
+def cpu_baseline_child(case_dir, kind, cells, budget_s, out):
     """CPU on one core: time n and 2n main-loop passes, take the difference.
 
     kind "reference": the unmodified reference (flang build in oracle/_ref).
@@ -94,7 +109,7 @@ def cpu_baseline(case_dir, cells, budget_s=20.0, kind="reference"):
         import refpy
 
         if not refpy.available():
-            return None
+            return
         with contextlib.redirect_stdout(io.StringIO()):
             R = refpy.RefCase(case_dir)
 
@@ -106,7 +121,7 @@ def cpu_baseline(case_dir, cells, budget_s=20.0, kind="reference"):
         import oracle as O
 
         if not O.available():
-            return None
+            return
         oc = O.OracleCase(case_dir)
         oc.init_flow()
 
@@ -126,8 +141,10 @@ def cpu_baseline(case_dir, cells, budget_s=20.0, kind="reference"):
     dt = (t2 - t1) - (t1 - t0)
     if dt <= 0:
         dt = (t2 - t1) / 2.0
-    return {"value": cells * n / dt, "unit": "cell-iters/s", "cores": 1, "kind": kind,
-            "sample": f"{n} main-loop passes of the same case (time of {2 * n} passes minus time of {n}), {what}"}
+    res = {"value": cells * n / dt, "unit": "cell-iters/s", "cores": 1, "kind": kind,
+           "sample": f"{n} main-loop passes of the same case (time of {2 * n} passes minus time of {n}), {what}"}
+    with open(out, "w") as f:
+        json.dump(res, f)
 
 
 def read_traffic(workload):
@@ -157,6 +174,10 @@ def calibrate_copy():
 
 
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "--cpu-baseline":
+        case_dir, kind, cells, budget, out = sys.argv[2:7]
+        cpu_baseline_child(case_dir, kind, int(cells), float(budget), out)
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
