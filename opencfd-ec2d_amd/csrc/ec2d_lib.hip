@@ -15,6 +15,7 @@
 #include "ec2d_fdm.hip"
 #include "ec2d_ghost.hip"
 #include "ec2d_lusgs.hip"
+#include "ec2d_sst.hip"
 #include "ec2d_pointwise.hip"
 #include "ec2d_residual.hip"
 
@@ -128,6 +129,12 @@ PlaneMap make_planes(int nv, int nv_un, bool fdm_any) {
     p.Akx = p.Aky = p.Aix = p.Aiy = p.Ajac = -1;
     p.fx = p.fy = p.ev = -1;
   }
+  if (nv == 6) {
+    p.sst = k;  // scratch of the SST model
+    k += SST_NPLANES;
+  } else {
+    p.sst = -1;
+  }
   p.nplanes = k;
   return p;
 }
@@ -147,6 +154,8 @@ int check_ready() {
 }
 
 // ---------------------------------------------------------------- launches
+int launch_pw(int level, int op, int krk = 0);
+
 int launch_residual(int level, int clip) {
   DLevel& L = R.lv[level];
   ResArgs a;
@@ -165,6 +174,22 @@ int launch_residual(int level, int clip) {
   int nv = L.o.nvar;
   int turb = L.o.turb;
   dim3 grid(L.nlaunch);
+  if (nv == 6 && turb == 3) {
+    // SST_kw runs before the mean-flow residual (sub_Residual.f90:158-191);
+    // the nu_t / k / omega clip of the previous step comes first
+    if (clip) {
+      if (launch_pw(level, PW_CLIP)) return 1;
+      a.clip = 0;
+    }
+    SstArgs sa;
+    sa.blocks = L.db;
+    sa.rows = L.drows;
+    sa.cps = R.dcps;
+    sa.arena = L.arena;
+    sa.pm = L.pm;
+    hipLaunchKernelGGL(k_sst_cell, dim3(L.nrows), dim3(256), 0, R.s, sa);
+    hipLaunchKernelGGL(k_sst_res, dim3(L.nrows), dim3(256), 0, R.s, sa);
+  }
 #define RLAUNCH(TY, NV, V, S) hipLaunchKernelGGL((k_residual<64, TY, NV, V, S>), grid, dim3(64 * TY), 0, R.s, a)
 #define RBY_TY(NV, V, S)                      \
   switch (R.tile_rows) {                      \
@@ -179,6 +204,8 @@ int launch_residual(int level, int clip) {
     RBY_TY(4, true, false)
   } else if (nv == 5 && turb == 2) {
     RBY_TY(5, true, true)
+  } else if (nv == 6 && turb == 3) {
+    RBY_TY(6, true, false)
   } else {
     return fail("turbulence model / equation count not supported by the residual kernel");
   }
@@ -286,7 +313,7 @@ int launch_ghost(int level, int do_bc, int do_exchange) {
   return 0;
 }
 
-int launch_pw(int level, int op, int krk = 0) {
+int launch_pw(int level, int op, int krk) {
   DLevel& L = R.lv[level];
   PwArgs a;
   a.blocks = L.db;

@@ -87,6 +87,10 @@ __global__ __launch_bounds__(TX* TY) void k_residual(ResArgs a) {
       bool ring = (i >= 0 && i <= nx && j >= 0 && j <= ny);
       double am = 0.0;
       if (ring) am = 1.0 / cp.Re * (1.0 + cp.Tsb) * sqrt(T * T * T) / (cp.Tsb + T);
+      // SST: the model scaled interior Amu by Re and back (sub_turbulence_SST.f90:85, 259)
+      if constexpr (NV == 6) {
+        if (interior) am = (am * cp.Re) / cp.Re;
+      }
       sAm[lj][li] = am;
       sx[lj][li] = PL(pm.x1)[q];
       sy[lj][li] = PL(pm.y1)[q];
@@ -106,6 +110,9 @@ __global__ __launch_bounds__(TX* TY) void k_residual(ResArgs a) {
           at = fv1 * d * vt;
         }
       }
+      // SST: mu_t as the model left it (interior and wall ghosts set by
+      // k_sst_res, other ghosts untouched)
+      if constexpr (NV == 6) at = ring ? PL(pm.Amu_t)[q] : 0.0;
       sAt[lj][li] = at;
     }
   }
@@ -402,6 +409,11 @@ __global__ __launch_bounds__(TX* TY) void k_residual(ResArgs a) {
         for (int m = 0; m < 4; m++) PL(pm.Res + m)[q] = R[m];
       }
       if constexpr (SA) PL(pm.Res + 4)[q] = R[4];
+      if constexpr (NV == 6) {
+        // Res(5:6) came from k_sst_res; read for the norms
+        R[4] = PL(pm.Res + 4)[q];
+        R[5] = PL(pm.Res + 5)[q];
+      }
       // comput_Lvc
       int64_t qi1 = pidx(B, i + 1, j), qj1 = pidx(B, i, j + 1);
       double s0 = PL(pm.vol)[q];
@@ -442,9 +454,12 @@ __global__ __launch_bounds__(TX* TY) void k_residual(ResArgs a) {
           rmax[m] = fabs(R[m]);
           rsum[m] = R[m] * R[m];
         }
-      } else if constexpr (SA) {
-        rmax[4] = fabs(R[4]);
-        rsum[4] = R[4] * R[4];
+      } else {
+#pragma unroll
+        for (int m = 4; m < NV; m++) {
+          rmax[m] = fabs(R[m]);
+          rsum[m] = R[m] * R[m];
+        }
       }
       if constexpr (VISC) {
         if (a.write_visc) {

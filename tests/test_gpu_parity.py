@@ -185,3 +185,39 @@ def test_multigrid_lusgs_sa(nm):
     c = cases.control_sa_naca(iters=10, Re=1e5, Ma=0.5, num_mesh=nm, time_method=F.TIME_LU_SGS)
     d = naca_case(f"g_mg_lusgs_{nm}", c, ds1=1e-4, fdm=(1,))
     _compare_steps(d, 3, arrays=("U", "Res"), exact=False, rtol=1e-11)
+
+
+def _sst(**kw):
+    c = cases.control_sa_naca(iters=10, Re=1e5, Ma=0.5, **kw)
+    c.Iflag_turbulence_model = F.TURB_SST
+    return c
+
+
+def test_sst_isothermal_wall():
+    """SST k-omega (device tanh in F1 / F2, hence the tolerance)."""
+    c = _sst()
+    c.Twall = 300.0
+    d = naca_case("g_sst_iso", c, ds1=1e-4)
+    _compare_steps(d, 4, arrays=("U", "Res", "dt", "Lci", "Lcj", "Lvi", "Lvj", "Amu_t"), exact=False, rtol=1e-11)
+
+
+def test_sst_adiabatic_wall():
+    """Adiabatic wall: the reference's omega ghost reads an unset density on
+    this branch; device and oracle both take 0 (parity with the reference
+    unpinned here, pinned on the isothermal wall)."""
+    d = naca_case("g_sst_adi", _sst(), ds1=1e-4)
+    _compare_steps(d, 3, arrays=("U", "Res", "dt", "Amu_t"), exact=False, rtol=1e-11)
+
+
+def test_sst_fdm_weno5():
+    c = _sst()
+    c.Twall = 300.0
+    d = naca_case("g_sst_fdm", c, ds1=1e-4, fdm=(1,))
+    _compare_steps(d, 3, arrays=("U", "Res", "dt"), exact=False, rtol=1e-11)
+
+
+def test_sst_multigrid_lusgs():
+    c = _sst(num_mesh=2, time_method=F.TIME_LU_SGS)
+    c.Twall = 300.0
+    d = naca_case("g_sst_mg", c, ds1=1e-4)
+    _compare_steps(d, 3, arrays=("U", "Res"), exact=False, rtol=1e-11)
