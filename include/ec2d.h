@@ -138,6 +138,18 @@ int ec2d_synchronize(void);
 /* Cells of the batch, and of one case, on a mesh level (Num_Cell, sub_init.f90:134). */
 long long ec2d_num_cells(int case_id, int nMesh);
 
+/* Force coefficients of a case on the finest mesh, from its current state.
+ * The reference has no force integration (only wall Cp, in the separate
+ * post-processing tool read_flow-1.2.f90:36-51).  Definition: every wall face
+ * contributes -(p - p_inf) n s (p of the adjacent interior cell) and the
+ * viscous traction (tau . n) s with tau from the face-centred gradient
+ * stencil of Residual_FVM (sub_Residual.f90:301-343, 408-450) and
+ * mu = Amu + Amu_t of that cell; n points into the fluid.  The total is
+ * rotated by AoA: Cl = (-Fx sin a + Fy cos a) / q, Cd = (Fx cos a + Fy sin a) / q,
+ * q = 1/2 (unit chord).  forces (4 doubles, may be NULL): pressure Fx, Fy,
+ * viscous Fx, Fy.  Synchronises the stream. */
+int ec2d_force_coefficients(int case_id, double *cl, double *cd, double *forces);
+
 /* Device time of the last ec2d_run call in milliseconds (HIP events on the
  * library stream), and the mean time per launch of the residual and LU-SGS
  * kernels measured with events around them during that call (0 if not

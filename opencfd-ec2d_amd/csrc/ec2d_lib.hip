@@ -16,6 +16,7 @@
 #include "ec2d_ghost.hip"
 #include "ec2d_lusgs.hip"
 #include "ec2d_sst.hip"
+#include "ec2d_forces.hip"
 #include "ec2d_pointwise.hip"
 #include "ec2d_residual.hip"
 
@@ -64,6 +65,9 @@ struct DLevel {
   std::vector<int> case_first_block;
   int lusgs_threads = 64;
   int lusgs_lpc = 1;          // LU-SGS lanes per cell: 2 when every block has at most 32 rows
+  int4* dwall = nullptr;      // wall faces (block, face, i, j), cases in order
+  int* dcase_wall0 = nullptr;
+  double* dforces = nullptr;  // 4 per case
   double* sk = nullptr;       // LU-SGS records (skewed, see ec2d_lusgs.hip)
   int64_t sk_max_n = 0;       // largest record field of a block (doubles)
   int max_cells = 0;          // largest block (cells)
@@ -636,7 +640,8 @@ int ec2d_finalize(void) {
   for (auto& L : R.lv) {
     hipFree(L.arena); hipFree(L.db); hipFree(L.ds); hipFree(L.dnorm); hipFree(L.dt); hipFree(L.dperm);
     hipFree(L.dcase_tile0); hipFree(L.dncell); hipFree(L.dpartials); hipFree(L.dnorms); hipFree(L.drows);
-    hipFree(L.dblocks_all); hipFree(L.sk); hipFree(L.d_ilines); hipFree(L.d_jlines); hipFree(L.d_frows);
+    hipFree(L.dblocks_all); hipFree(L.sk); hipFree(L.dwall); hipFree(L.dcase_wall0); hipFree(L.dforces);
+    hipFree(L.d_ilines); hipFree(L.d_jlines); hipFree(L.d_frows);
   }
   hipFree(R.dcps);
   hipFree(R.derr);
@@ -1001,6 +1006,32 @@ int ec2d_commit(void) {
     HIPCHK(hipMemcpy(L.drows, rows.data(), sizeof(int2) * rows.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMalloc(&L.dblocks_all, sizeof(int) * all.size()));
     HIPCHK(hipMemcpy(L.dblocks_all, all.data(), sizeof(int) * all.size(), hipMemcpyHostToDevice));
+    // wall faces for the force integration (finest level only)
+    if (l == 0) {
+      std::vector<int4> wf;
+      std::vector<int> w0;
+      for (size_t ci = 0; ci < R.cases.size(); ci++) {
+        w0.push_back((int)wf.size());
+        for (int b = L.case_first_block[ci]; b < L.case_first_block[ci + 1]; b++) {
+          const HBlock& hb = R.cases[ci].levels[0][L.blk_local[b]];
+          for (int k = 0; k < hb.nsub; k++) {
+            const int* sb = &hb.bc[9 * k];
+            if (sb[6] != -10) continue;
+            const int face = sb[1];
+            if (face == 1 || face == 3)
+              for (int j = sb[4]; j < sb[5]; j++) wf.push_back(make_int4(b, face, face == 1 ? 1 : hb.nx, j));
+            else
+              for (int i = sb[2]; i < sb[3]; i++) wf.push_back(make_int4(b, face, i, face == 2 ? 1 : hb.ny));
+          }
+        }
+      }
+      w0.push_back((int)wf.size());
+      HIPCHK(hipMalloc(&L.dwall, sizeof(int4) * std::max<size_t>(wf.size(), 1)));
+      if (!wf.empty()) HIPCHK(hipMemcpy(L.dwall, wf.data(), sizeof(int4) * wf.size(), hipMemcpyHostToDevice));
+      HIPCHK(hipMalloc(&L.dcase_wall0, sizeof(int) * w0.size()));
+      HIPCHK(hipMemcpy(L.dcase_wall0, w0.data(), sizeof(int) * w0.size(), hipMemcpyHostToDevice));
+      HIPCHK(hipMalloc(&L.dforces, sizeof(double) * 4 * R.cases.size()));
+    }
   }
   R.committed = true;
   return 0;
@@ -1272,6 +1303,33 @@ int ec2d_timing(double* run_ms, double* residual_ms, double* lusgs_ms, double* g
   if (residual_ms) *residual_ms = R.res_ms;
   if (lusgs_ms) *lusgs_ms = R.lusgs_ms;
   if (ghost_ms) *ghost_ms = R.ghost_ms;
+  return 0;
+}
+
+int ec2d_force_coefficients(int case_id, double* cl, double* cd, double* forces) {
+  if (check_ready()) return 1;
+  if (case_id < 0 || case_id >= (int)R.cases.size()) return fail("bad case id");
+  DLevel& L = R.lv[0];
+  ForceArgs a;
+  a.blocks = L.db;
+  a.faces = L.dwall;
+  a.case_face0 = L.dcase_wall0;
+  a.cps = R.dcps;
+  a.arena = L.arena;
+  a.pm = L.pm;
+  a.viscous = R.cases[0].cp.If_viscous == 1;
+  a.out = L.dforces;
+  hipLaunchKernelGGL(k_forces, dim3((unsigned)R.cases.size()), dim3(256), 0, R.s, a);
+  HIPCHK(hipGetLastError());
+  double F[4];
+  HIPCHK(hipMemcpyAsync(F, L.dforces + 4 * case_id, sizeof(F), hipMemcpyDeviceToHost, R.s));
+  HIPCHK(hipStreamSynchronize(R.s));
+  const double al = R.cases[case_id].ctl.AoA;
+  const double fx = F[0] + F[2], fy = F[1] + F[3];
+  if (cl) *cl = (-fx * std::sin(al) + fy * std::cos(al)) / 0.5;
+  if (cd) *cd = (fx * std::cos(al) + fy * std::sin(al)) / 0.5;
+  if (forces)
+    for (int k = 0; k < 4; k++) forces[k] = F[k];
   return 0;
 }
 

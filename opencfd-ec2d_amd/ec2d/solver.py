@@ -246,6 +246,13 @@ class Batch:
         return {"residual_ms": v[0].value, "lusgs_ms": v[1].value, "ghost_ms": v[2].value,
                 "launches": {"residual": n[0], "lusgs": n[1], "ghost": n[2]}}
 
+    def force_coefficients(self, case_id: int = 0):
+        """(Cl, Cd, forces[4]) of a case from its current state (see ec2d_force_coefficients)."""
+        cl, cd = C.c_double(), C.c_double()
+        f = np.zeros(4)
+        capi.call("ec2d_force_coefficients", case_id, C.byref(cl), C.byref(cd), f.ctypes.data_as(C.POINTER(C.c_double)))
+        return cl.value, cd.value, f
+
     KERNEL_KINDS = {"residual": 1, "lusgs": 2, "ghost": 3, "lusgs_prep": 4, "lusgs_sweep": 5, "lusgs_scatter": 6}
 
     def kernel_time(self, kind: str, nmesh: int = 1):

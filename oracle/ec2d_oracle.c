@@ -1586,3 +1586,90 @@ void o_pt_split_sw(double d, double u, double v, double cc, double A1, double A2
 void o_pt_dfn(int NV, double *DF, const double *U, const double *DU, double n1, double n2, double g) {
   comput_dfn(NV, DF, U, DU, n1, n2, g);
 }
+
+/* ------------------------------------------------------------------------
+ * Wall forces.  The reference has no force integration (only wall Cp in a
+ * separate post-processing tool, read_flow-1.2.f90:36-51); this is the
+ * definition the device path implements, restated serially:
+ *   every wall face (subfaces with neighb = -10, in block / file / index
+ *   order) contributes, with n its unit normal pointing into the fluid and s
+ *   its length,
+ *     pressure  -(p - p_inf) n s, p of the adjacent interior cell,
+ *     viscous   (tau . n) s, tau = mu (grad u + grad u^T - 2/3 div u I)
+ *               from the face-centred gradient stencil of Residual_FVM
+ *               (sub_Residual.f90:301-343, 408-450), mu = Amu + Amu_t of the
+ *               adjacent interior cell (the wall-face rule of that routine).
+ *   F[0..3] = (Fx, Fy) pressure, (Fx, Fy) viscous, sums in face order.
+ * ---------------------------------------------------------------------- */
+void o_forces(const OMesh *M, const OParams *P, double *F) {
+  double g = P->gamma, p_inf = 1.0 / (g * P->Ma * P->Ma);
+  int NV = M->nvar;
+  F[0] = F[1] = F[2] = F[3] = 0.0;
+  for (int mb = 0; mb < M->nblock; mb++) {
+    const OBlock *B = &M->blocks[mb];
+    int nx = B->nx, ny = B->ny;
+    const double *U = B->U, *x1 = B->x1, *y1 = B->y1;
+#define UU(i, j) (U[UI(B, NV, 2, i, j)] / U[UI(B, NV, 1, i, j)])
+#define VV(i, j) (U[UI(B, NV, 3, i, j)] / U[UI(B, NV, 1, i, j)])
+    for (int k = 0; k < B->nsub; k++) {
+      const int *s = B->bc + 9 * k;
+      if (s[6] != -10) continue;
+      int face = s[1];
+      int n = (face == 1 || face == 3) ? s[5] - s[4] : s[3] - s[2];
+      for (int t = 0; t < n; t++) {
+        int i, j, ic, jc;  /* face index (i, j) and adjacent interior cell (ic, jc) */
+        double sgn;
+        if (face == 1) { i = 1; j = s[4] + t; ic = 1; jc = j; sgn = 1.0; }
+        else if (face == 3) { i = nx; j = s[4] + t; ic = nx - 1; jc = j; sgn = -1.0; }
+        else if (face == 2) { i = s[2] + t; j = 1; ic = i; jc = 1; sgn = 1.0; }
+        else { i = s[2] + t; j = ny; ic = i; jc = ny - 1; sgn = -1.0; }
+        int iface = (face == 1 || face == 3);
+        size_t pp = PI_(B, i, j);
+        double sl = iface ? B->si[pp] : B->sj[pp];
+        double n1 = sgn * (iface ? B->ni1[pp] : B->nj1[pp]);
+        double n2 = sgn * (iface ? B->ni2[pp] : B->nj2[pp]);
+        size_t uc = GI(B, ic, jc);
+        double d = U[UI(B, NV, 1, ic, jc)];
+        double u = U[UI(B, NV, 2, ic, jc)] / d, v = U[UI(B, NV, 3, ic, jc)] / d;
+        double p = (g - 1.0) * (U[UI(B, NV, 4, ic, jc)] - 0.5 * d * (u * u + v * v));
+        double dp = p - p_inf;
+        F[0] = F[0] - dp * n1 * sl;
+        F[1] = F[1] - dp * n2 * sl;
+        if (P->If_viscous == 1) {
+          double Dix, Diy, Djx, Djy, Diu, Div, Dju, Djv;
+          if (iface) {
+            Dix = x1[GI(B, i, j)] - x1[GI(B, i - 1, j)];
+            Diy = y1[GI(B, i, j)] - y1[GI(B, i - 1, j)];
+            Djx = (x1[GI(B, i - 1, j + 1)] + x1[GI(B, i, j + 1)] - x1[GI(B, i - 1, j - 1)] - x1[GI(B, i, j - 1)]) * 0.25;
+            Djy = (y1[GI(B, i - 1, j + 1)] + y1[GI(B, i, j + 1)] - y1[GI(B, i - 1, j - 1)] - y1[GI(B, i, j - 1)]) * 0.25;
+            Diu = UU(i, j) - UU(i - 1, j);
+            Div = VV(i, j) - VV(i - 1, j);
+            Dju = (UU(i - 1, j + 1) + UU(i, j + 1) - UU(i - 1, j - 1) - UU(i, j - 1)) * 0.25;
+            Djv = (VV(i - 1, j + 1) + VV(i, j + 1) - VV(i - 1, j - 1) - VV(i, j - 1)) * 0.25;
+          } else {
+            Dix = (x1[GI(B, i + 1, j - 1)] + x1[GI(B, i + 1, j)] - x1[GI(B, i - 1, j - 1)] - x1[GI(B, i - 1, j)]) * 0.25;
+            Diy = (y1[GI(B, i + 1, j - 1)] + y1[GI(B, i + 1, j)] - y1[GI(B, i - 1, j - 1)] - y1[GI(B, i - 1, j)]) * 0.25;
+            Djx = x1[GI(B, i, j)] - x1[GI(B, i, j - 1)];
+            Djy = y1[GI(B, i, j)] - y1[GI(B, i, j - 1)];
+            Diu = (UU(i + 1, j - 1) + UU(i + 1, j) - UU(i - 1, j - 1) - UU(i - 1, j)) * 0.25;
+            Div = (VV(i + 1, j - 1) + VV(i + 1, j) - VV(i - 1, j - 1) - VV(i - 1, j)) * 0.25;
+            Dju = UU(i, j) - UU(i, j - 1);
+            Djv = VV(i, j) - VV(i, j - 1);
+          }
+          double Ds = 1.0 / (Dix * Djy - Djx * Diy);
+          double ux = (Diu * Djy - Dju * Diy) * Ds, vx = (Div * Djy - Djv * Diy) * Ds;
+          double uy = (-Diu * Djx + Dju * Dix) * Ds, vy = (-Div * Djx + Djv * Dix) * Ds;
+          double mu = B->Amu[uc] + B->Amu_t[uc];
+          double div = ux + vy;
+          double t11 = mu * (2.0 * ux - 2.0 / 3.0 * div);
+          double t12 = mu * (uy + vx);
+          double t22 = mu * (2.0 * vy - 2.0 / 3.0 * div);
+          F[2] = F[2] + (t11 * n1 + t12 * n2) * sl;
+          F[3] = F[3] + (t12 * n1 + t22 * n2) * sl;
+        }
+      }
+    }
+#undef UU
+#undef VV
+  }
+}

@@ -56,6 +56,9 @@ void o_delt_u(OMesh *M);
 void o_set_un(OMesh *M);
 void o_new_u(OMesh *M);
 
+/* wall forces of one mesh: F[0..3] = pressure (Fx, Fy), viscous (Fx, Fy) */
+void o_forces(const OMesh *M, const OParams *P, double *F);
+
 /* pointwise pieces, for closed-form tests of the oracle */
 void o_pt_flux(int iflux, const double *QL, const double *QR, double *F, double g);
 double o_pt_scheme(int side, const double *u, int flag);

@@ -342,6 +342,21 @@ class OracleCase:
             self.mg3()
 
     # ---- initial state ----------------------------------------------
+    def forces(self, nmesh=1):
+        """Wall forces (o_forces): pressure (Fx, Fy), viscous (Fx, Fy)."""
+        L = lib()
+        F = np.zeros(4)
+        L.o_forces.argtypes = [C.POINTER(OMesh), C.POINTER(OParams), _dp]
+        L.o_forces(self.M(nmesh), C.byref(self.P), F.ctypes.data_as(_dp))
+        return F
+
+    def force_coefficients(self, nmesh=1):
+        """(Cl, Cd) from the wall forces: rotated by AoA, over q_inf = 1/2 and unit chord."""
+        F = self.forces(nmesh)
+        a = self.ctl.aoa_rad
+        fx, fy = F[0] + F[2], F[1] + F[3]
+        return (-fx * math.sin(a) + fy * math.cos(a)) / 0.5, (fx * math.cos(a) + fy * math.sin(a)) / 0.5
+
     def init_flow(self):
         """Init_flow (sub_init.f90:383-414) for Iflag_init 0 and 1."""
         c = self.ctl

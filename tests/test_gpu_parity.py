@@ -221,3 +221,31 @@ def test_sst_multigrid_lusgs():
     c.Twall = 300.0
     d = naca_case("g_sst_mg", c, ds1=1e-4)
     _compare_steps(d, 3, arrays=("U", "Res"), exact=False, rtol=1e-11)
+
+
+@pytest.mark.parametrize("kind", ["euler", "laminar", "sa"])
+def test_force_coefficients(kind):
+    """ec2d_force_coefficients against the oracle's serial integral (the
+    device sums the faces in a fixed tree, hence a relative tolerance)."""
+    if kind == "euler":
+        c, ds1, rtol = cases.control_euler_naca(iters=10, aoa=2.0), 1e-3, 1e-13
+    else:
+        c, ds1 = cases.control_sa_naca(iters=10, Re=1e5, Ma=0.5), 1e-4
+        if kind == "laminar":
+            c.Iflag_turbulence_model, c.Twall, rtol = F.TURB_NONE, 350.0, 1e-12
+        else:
+            rtol = 1e-9  # the SA state itself agrees to 1e-11 per step
+    d = naca_case(f"g_forces_{kind}", c, ds1=ds1)
+    ref = O.OracleCase(d)
+    ref.init_flow()
+    with solver.Batch([solver.Case.from_dir(d)]) as b:
+        b.init_flow()
+        for _ in range(4):
+            ref.main_step()
+            b.main_step()
+        cl, cd, f = b.force_coefficients(0)
+    rf = ref.forces()
+    rcl, rcd = ref.force_coefficients()
+    np.testing.assert_allclose(f, rf, rtol=rtol, atol=1e-15 * np.abs(rf).max())
+    np.testing.assert_allclose([cl, cd], [rcl, rcd], rtol=10 * rtol)
+    assert abs(cl) > 1e-6  # a lifting state, not a trivial zero
