@@ -186,6 +186,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile-rows", type=int, default=4)
     ap.add_argument("--calibrate-copy", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy()
@@ -206,6 +207,9 @@ def main():
         with solver.Batch([case], tile_rows=args.tile_rows) as b:
             cells = b.num_cells(0, 1)
             b.init_flow()
+            # launch shapes per level (residual tile rows, LU-SGS lanes per
+            # cell), timed on this case before the measured run
+            tuning = b.autotune(reps=5) if not args.no_tune else "off"
             b.run(max(args.warmup, 1))
             b.synchronize()
             t0 = time.perf_counter()
@@ -251,7 +255,8 @@ def main():
         "data": "synthetic (generated NACA 0012 C-grid, freestream start)",
         "config": dict(desc, device_ms_per_step=dev_ms / args.steps,
                        residual_rms_last=[float(v) for v in rrms[:nv]],
-                       kernel_ms_per_level=levels, tile_rows=args.tile_rows),
+                       kernel_ms_per_level=levels, tile_rows=args.tile_rows,
+                       autotune=[t for t in tuning.strip().split("\n") if t]),
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": read_traffic(desc["workload"]) if dom == "lusgs" else None,

@@ -173,6 +173,17 @@ int ec2d_kernel_time(int nMesh, int kind, double *ms, int *launches);
  * No reference counterpart: measurement support. */
 int ec2d_set_profile(int on);
 
+/* Autotune the launch shapes of every mesh level on the committed batch:
+ * residual tile rows (1, 2, 4, 8 x 64 cells) and LU-SGS lanes per cell (1 or
+ * 2), each timed with HIP events over `reps` launches on the current state;
+ * the fastest is kept.  Results are cached per level configuration (equations,
+ * scheme, flux, reconstruction, turbulence model, viscosity, block shapes) in
+ * the text file cache_path (NULL or "": no cache).  report receives one line
+ * per level.  Changes no result bit: the state is recomputed by the next step
+ * (the timed launches write only Res, dt, the spectral radii, Amu, Amu_t and
+ * dU).  No reference counterpart. */
+int ec2d_autotune(const char *cache_path, int reps, char *report, int report_len);
+
 /* Tile shape of the fused residual kernel (rows x 64 cells); no reference
  * counterpart.  Call before ec2d_commit.  Valid rows: 1, 2, 4, 8. */
 int ec2d_set_tile_rows(int rows);

@@ -54,6 +54,7 @@ struct DLevel {
   TileDesc* dt = nullptr;
   int* dperm = nullptr;
   int ntiles = 0, nlaunch = 0;
+  int tile_rows = 4;          // residual tile: 64 x tile_rows cells
   std::vector<int> case_tile0;
   int* dcase_tile0 = nullptr;
   double* dncell = nullptr;
@@ -111,6 +112,90 @@ int fail(const std::string& m) {
     hipError_t e_ = (x);                                                                      \
     if (e_ != hipSuccess) return fail(std::string(#x) + ": " + hipGetErrorString(e_));       \
   } while (0)
+
+// LU-SGS record layout of a level for lpc lanes per cell (see ec2d_lusgs.hip);
+// upload: also refresh the device block table (after ec2d_commit).
+int lusgs_layout(DLevel& L, int lpc, bool upload) {
+  int max_rows = 1;
+  for (auto& d : L.hb) max_rows = std::max(max_rows, d.ny - 1);
+  const int threads = 64 * ((max_rows * lpc + 63) / 64);
+  if (threads > 1024) return fail("blocks with more than 1024 rows of cells are not supported by the LU-SGS kernel");
+  const int nv = L.o.nvar;
+  const bool visc = R.cases[0].cp.If_viscous == 1;
+  const int64_t per = 2 * lusgs_rec_doubles(nv, visc) + nv;
+  int64_t sk_off = 0;
+  L.sk_max_n = 0;
+  L.max_cells = 0;
+  for (auto& d : L.hb) {
+    const int ncol = d.nx - 1, nrow = d.ny - 1;
+    d.sk_S = ncol + nrow - 1;
+    const int rq = 64 / lpc;  // rows per wave
+    d.sk_R = rq * ((nrow + rq - 1) / rq);
+    d.sk_n = (int64_t)d.sk_S * d.sk_R + 64;
+    d.sk_base = sk_off;
+    if (per * d.sk_n * 8 >= (int64_t)1 << 31) return fail("block too large for the LU-SGS record buffer (2 GiB)");
+    sk_off += (per * d.sk_n + 31) / 32 * 32;
+    L.sk_max_n = std::max(L.sk_max_n, d.sk_n);
+    L.max_cells = std::max(L.max_cells, ncol * nrow);
+  }
+  if (L.sk) HIPCHK(hipFree(L.sk));
+  L.sk = nullptr;
+  HIPCHK(hipMalloc(&L.sk, sizeof(double) * (size_t)std::max<int64_t>(sk_off, 1)));
+  HIPCHK(hipMemset(L.sk, 0, sizeof(double) * (size_t)std::max<int64_t>(sk_off, 1)));
+  L.lusgs_lpc = lpc;
+  L.lusgs_threads = threads;
+  if (upload) HIPCHK(hipMemcpy(L.db, L.hb.data(), sizeof(BlockDesc) * L.hb.size(), hipMemcpyHostToDevice));
+  return 0;
+}
+
+// Residual tiles of a level (64 x L.tile_rows cells), their XCD-aware launch
+// order and the per-tile norm partials.
+int build_tiles(DLevel& L) {
+  const int nv = L.o.nvar;
+  L.ht.clear();
+  L.case_tile0.clear();
+  // tiles (logical order: case, block, row of tiles, column of tiles)
+  int TX = 64, TY = L.tile_rows;
+  for (size_t ci = 0; ci < R.cases.size(); ci++) {
+    L.case_tile0.push_back((int)L.ht.size());
+    for (int b = L.case_first_block[ci]; b < L.case_first_block[ci + 1]; b++) {
+      const BlockDesc& bd = L.hb[b];
+      for (int j0 = 1; j0 <= bd.ny - 1; j0 += TY)
+        for (int i0 = 1; i0 <= bd.nx - 1; i0 += TX) L.ht.push_back({b, i0, j0, 0});
+    }
+  }
+  L.case_tile0.push_back((int)L.ht.size());
+  L.ntiles = (int)L.ht.size();
+  // XCD-aware launch order: workgroup k runs on XCD k % 8; give each XCD a
+  // contiguous run of logical tiles so that halo rows shared by neighbouring
+  // tiles are fetched into one L2.  Padding slots are compacted away.
+  const int NX = 8;
+  int chunk = (L.ntiles + NX - 1) / NX;
+  std::vector<int> perm(chunk * NX, -1);
+  for (int x = 0; x < NX; x++)
+    for (int k = 0; k < chunk; k++) {
+      int lt = x * chunk + k;
+      if (lt < L.ntiles) perm[k * NX + x] = lt;
+    }
+  std::vector<int> perm2;
+  for (int v : perm)
+    if (v >= 0) perm2.push_back(v);
+  perm = perm2;
+  L.nlaunch = (int)perm.size();
+  hipFree(L.dt);
+  hipFree(L.dperm);
+  hipFree(L.dcase_tile0);
+  hipFree(L.dpartials);
+  L.dt = nullptr; L.dperm = nullptr; L.dcase_tile0 = nullptr; L.dpartials = nullptr;
+  HIPCHK(hipMalloc(&L.dt, sizeof(TileDesc) * L.ht.size()));
+  HIPCHK(hipMemcpy(L.dt, L.ht.data(), sizeof(TileDesc) * L.ht.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMalloc(&L.dperm, sizeof(int) * perm.size()));
+  HIPCHK(hipMemcpy(L.dperm, perm.data(), sizeof(int) * perm.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMalloc(&L.dcase_tile0, sizeof(int) * L.case_tile0.size()));
+  HIPCHK(hipMemcpy(L.dcase_tile0, L.case_tile0.data(), sizeof(int) * L.case_tile0.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMalloc(&L.dpartials, sizeof(double) * (size_t)L.ntiles * 2 * nv));
+  return 0;
+}
 
 PlaneMap make_planes(int nv, int nv_un, bool fdm_any) {
   PlaneMap p{};
@@ -196,7 +281,7 @@ int launch_residual(int level, int clip) {
   }
 #define RLAUNCH(TY, NV, V, S) hipLaunchKernelGGL((k_residual<64, TY, NV, V, S>), grid, dim3(64 * TY), 0, R.s, a)
 #define RBY_TY(NV, V, S)                      \
-  switch (R.tile_rows) {                      \
+  switch (L.tile_rows) {                      \
     case 1: RLAUNCH(1, NV, V, S); break;      \
     case 2: RLAUNCH(2, NV, V, S); break;      \
     case 8: RLAUNCH(8, NV, V, S); break;      \
@@ -236,9 +321,9 @@ int launch_residual(int level, int clip) {
     f.lines = L.d_frows;
     if (visc) hipLaunchKernelGGL(k_fdm_visc, dim3(L.n_frows), dim3(256), 0, R.s, f);
     hipLaunchKernelGGL(k_fdm_res, dim3(L.n_frows), dim3(256), 0, R.s, f);
-    if (nv == 4) hipLaunchKernelGGL(k_tile_norms<4>, dim3(L.ntiles), dim3(256), 0, R.s, L.dt, L.db, L.arena, L.pm, L.dpartials, R.tile_rows);
-    else if (nv == 5) hipLaunchKernelGGL(k_tile_norms<5>, dim3(L.ntiles), dim3(256), 0, R.s, L.dt, L.db, L.arena, L.pm, L.dpartials, R.tile_rows);
-    else hipLaunchKernelGGL(k_tile_norms<6>, dim3(L.ntiles), dim3(256), 0, R.s, L.dt, L.db, L.arena, L.pm, L.dpartials, R.tile_rows);
+    if (nv == 4) hipLaunchKernelGGL(k_tile_norms<4>, dim3(L.ntiles), dim3(256), 0, R.s, L.dt, L.db, L.arena, L.pm, L.dpartials, L.tile_rows);
+    else if (nv == 5) hipLaunchKernelGGL(k_tile_norms<5>, dim3(L.ntiles), dim3(256), 0, R.s, L.dt, L.db, L.arena, L.pm, L.dpartials, L.tile_rows);
+    else hipLaunchKernelGGL(k_tile_norms<6>, dim3(L.ntiles), dim3(256), 0, R.s, L.dt, L.db, L.arena, L.pm, L.dpartials, L.tile_rows);
     HIPCHK(hipGetLastError());
   }
   int ncase = (int)R.cases.size();
@@ -266,20 +351,29 @@ int launch_lusgs(int level, const int* dlist, int nblk, double alfa1, int update
   dim3 gprep((unsigned)((L.sk_max_n + 255) / 256), nblk), gsweep(nblk), blk(L.lusgs_threads);
   dim3 gscat((unsigned)((L.max_cells + 255) / 256), nblk);
   int nv = L.o.nvar;
-  // up to 4 waves: one wave per SIMD and two steps per register set; more
-  // waves (up to 16): one step per set
+  // the instantiation is chosen by thread count (its launch bound): up to 4
+  // waves one wave per SIMD and two steps per register set; more waves (up
+  // to 16) one step per set
   const int nth = L.lusgs_threads;
   const bool pair = L.lusgs_lpc == 2;
+  // every instantiation below is compiled for at most its MAXT threads
+  if (nth < 64 || nth > 1024 || nth % 64 != 0) return fail("LU-SGS launch shape out of range");
   const int pk = 10 * level;
 #define LLAUNCH(NV, V)                                                                      \
   prof_mark(4 + pk);                                                                        \
   hipLaunchKernelGGL((k_lusgs_prep<NV, V>), gprep, dim3(256), 0, R.s, a);                   \
   prof_mark(-(4 + pk));                                                                     \
   prof_mark(5 + pk);                                                                        \
-  if (pair) hipLaunchKernelGGL((k_lusgs_sweep<NV, V, 2, 64, 2>), gsweep, blk, 0, R.s, a);    \
-  else if (nth > 512) hipLaunchKernelGGL((k_lusgs_sweep<NV, V, 1, 1024, 1>), gsweep, blk, 0, R.s, a); \
-  else if (nth > 256) hipLaunchKernelGGL((k_lusgs_sweep<NV, V, 1, 512, 1>), gsweep, blk, 0, R.s, a); \
-  else hipLaunchKernelGGL((k_lusgs_sweep<NV, V, 2, 256, 1>), gsweep, blk, 0, R.s, a);       \
+  if (pair) {                                                                                \
+    if (nth <= 64) hipLaunchKernelGGL((k_lusgs_sweep<NV, V, 2, 64, 2>), gsweep, blk, 0, R.s, a); \
+    else if (nth <= 256) hipLaunchKernelGGL((k_lusgs_sweep<NV, V, 2, 256, 2>), gsweep, blk, 0, R.s, a); \
+    else if (nth <= 512) hipLaunchKernelGGL((k_lusgs_sweep<NV, V, 1, 512, 2>), gsweep, blk, 0, R.s, a); \
+    else hipLaunchKernelGGL((k_lusgs_sweep<NV, V, 1, 1024, 2>), gsweep, blk, 0, R.s, a);       \
+  } else {                                                                                  \
+    if (nth <= 256) hipLaunchKernelGGL((k_lusgs_sweep<NV, V, 2, 256, 1>), gsweep, blk, 0, R.s, a); \
+    else if (nth <= 512) hipLaunchKernelGGL((k_lusgs_sweep<NV, V, 1, 512, 1>), gsweep, blk, 0, R.s, a); \
+    else hipLaunchKernelGGL((k_lusgs_sweep<NV, V, 1, 1024, 1>), gsweep, blk, 0, R.s, a);       \
+  }                                                                                         \
   prof_mark(-(5 + pk));                                                                     \
   prof_mark(6 + pk);                                                                        \
   hipLaunchKernelGGL((k_lusgs_scatter<NV, V>), gscat, dim3(256), 0, R.s, a);                \
@@ -618,6 +712,21 @@ int init_default_state(DLevel& L, int level, std::vector<double>& img) {
 }  // namespace
 
 // ====================================================================== C-ABI
+// mean time (ms) of `reps` launches of fn after one untimed launch
+template <class Fn>
+int time_launches(int reps, Fn fn, double* ms) {
+  if (fn()) return 1;
+  HIPCHK(hipEventRecord(R.ev0, R.s));
+  for (int k = 0; k < reps; k++)
+    if (fn()) return 1;
+  HIPCHK(hipEventRecord(R.ev1, R.s));
+  HIPCHK(hipEventSynchronize(R.ev1));
+  float t = 0;
+  HIPCHK(hipEventElapsedTime(&t, R.ev0, R.ev1));
+  *ms = t / reps;
+  return 0;
+}
+
 extern "C" {
 
 int ec2d_init(void) {
@@ -861,28 +970,7 @@ int ec2d_commit(void) {
     L.arena_n = off;
     // two lanes per cell when one wave holds every block's rows that way;
     // otherwise one lane per row (fewer waves, fewer barriers per step)
-    L.lusgs_lpc = max_rows <= 32 ? 2 : 1;
-    L.lusgs_threads = 64 * ((max_rows * L.lusgs_lpc + 63) / 64);
-    {
-      const bool visc = R.cases[0].cp.If_viscous == 1;
-      const int64_t per = 2 * lusgs_rec_doubles(nv, visc) + nv;
-      int64_t sk_off = 0;
-      for (auto& d : L.hb) {
-        const int ncol = d.nx - 1, nrow = d.ny - 1;
-        d.sk_S = ncol + nrow - 1;
-        const int rq = 64 / L.lusgs_lpc;  // rows per wave
-        d.sk_R = rq * ((nrow + rq - 1) / rq);
-        d.sk_n = (int64_t)d.sk_S * d.sk_R + 64;
-        d.sk_base = sk_off;
-        if (per * d.sk_n * 8 >= (int64_t)1 << 31) return fail("block too large for the LU-SGS record buffer (2 GiB)");
-        sk_off += (per * d.sk_n + 31) / 32 * 32;
-        L.sk_max_n = std::max(L.sk_max_n, d.sk_n);
-        L.max_cells = std::max(L.max_cells, ncol * nrow);
-      }
-      HIPCHK(hipMalloc(&L.sk, sizeof(double) * (size_t)std::max<int64_t>(sk_off, 1)));
-      HIPCHK(hipMemset(L.sk, 0, sizeof(double) * (size_t)std::max<int64_t>(sk_off, 1)));
-    }
-    if (L.lusgs_threads > 1024) return fail("blocks with more than 1024 rows of cells are not supported by the LU-SGS kernel");
+    if (lusgs_layout(L, max_rows <= 32 ? 2 : 1, false)) return 1;
     // host image of the arena
     std::vector<double> img((size_t)off, 0.0);
     init_default_state(L, l, img);
@@ -926,50 +1014,12 @@ int ec2d_commit(void) {
     HIPCHK(hipMemcpy(L.ds, subs.data(), sizeof(SubfaceDesc) * subs.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMalloc(&L.dnorm, sizeof(double) * std::max<size_t>(norms.size(), 2)));
     if (!norms.empty()) HIPCHK(hipMemcpy(L.dnorm, norms.data(), sizeof(double) * norms.size(), hipMemcpyHostToDevice));
-    // tiles (logical order: case, block, row of tiles, column of tiles)
-    int TX = 64, TY = R.tile_rows;
-    for (size_t ci = 0; ci < R.cases.size(); ci++) {
-      L.case_tile0.push_back((int)L.ht.size());
-      for (int b = L.case_first_block[ci]; b < L.case_first_block[ci + 1]; b++) {
-        const BlockDesc& bd = L.hb[b];
-        for (int j0 = 1; j0 <= bd.ny - 1; j0 += TY)
-          for (int i0 = 1; i0 <= bd.nx - 1; i0 += TX) L.ht.push_back({b, i0, j0, 0});
-      }
-    }
-    L.case_tile0.push_back((int)L.ht.size());
-    L.ntiles = (int)L.ht.size();
-    // XCD-aware launch order: workgroup k runs on XCD k % 8; give each XCD a
-    // contiguous run of logical tiles so that halo rows shared by neighbouring
-    // tiles are fetched into one L2.  Padding slots map to -1 (exit at once).
-    const int NX = 8;
-    int chunk = (L.ntiles + NX - 1) / NX;
-    L.nlaunch = chunk * NX;
-    std::vector<int> perm(L.nlaunch, -1);
-    for (int x = 0; x < NX; x++)
-      for (int k = 0; k < chunk; k++) {
-        int lt = x * chunk + k;
-        if (lt < L.ntiles) perm[k * NX + x] = lt;
-      }
-    // compact the padding away if it would idle more than one wave per XCD
-    std::vector<int> perm2;
-    for (int v : perm)
-      if (v >= 0) perm2.push_back(v);
-    if ((int)perm2.size() != L.nlaunch) {
-      // keep full XCD interleave where every slot is used; fill the tail in order
-      perm = perm2;
-      L.nlaunch = (int)perm.size();
-    }
-    HIPCHK(hipMalloc(&L.dt, sizeof(TileDesc) * L.ht.size()));
-    HIPCHK(hipMemcpy(L.dt, L.ht.data(), sizeof(TileDesc) * L.ht.size(), hipMemcpyHostToDevice));
-    HIPCHK(hipMalloc(&L.dperm, sizeof(int) * perm.size()));
-    HIPCHK(hipMemcpy(L.dperm, perm.data(), sizeof(int) * perm.size(), hipMemcpyHostToDevice));
-    HIPCHK(hipMalloc(&L.dcase_tile0, sizeof(int) * L.case_tile0.size()));
-    HIPCHK(hipMemcpy(L.dcase_tile0, L.case_tile0.data(), sizeof(int) * L.case_tile0.size(), hipMemcpyHostToDevice));
+    L.tile_rows = R.tile_rows;
+    if (build_tiles(L)) return 1;
     std::vector<double> ncell;
     for (auto v : L.case_cells) ncell.push_back(1.0 * (double)v);
     HIPCHK(hipMalloc(&L.dncell, sizeof(double) * ncell.size()));
     HIPCHK(hipMemcpy(L.dncell, ncell.data(), sizeof(double) * ncell.size(), hipMemcpyHostToDevice));
-    HIPCHK(hipMalloc(&L.dpartials, sizeof(double) * (size_t)L.ntiles * 2 * nv));
     HIPCHK(hipMalloc(&L.dnorms, sizeof(double) * 12 * R.cases.size()));
     std::vector<double> z(12 * R.cases.size(), 0.0);
     HIPCHK(hipMemcpy(L.dnorms, z.data(), sizeof(double) * z.size(), hipMemcpyHostToDevice));
@@ -1303,6 +1353,83 @@ int ec2d_timing(double* run_ms, double* residual_ms, double* lusgs_ms, double* g
   if (residual_ms) *residual_ms = R.res_ms;
   if (lusgs_ms) *lusgs_ms = R.lusgs_ms;
   if (ghost_ms) *ghost_ms = R.ghost_ms;
+  return 0;
+}
+
+int ec2d_autotune(const char* cache_path, int reps, char* report, int report_len) {
+  if (check_ready()) return 1;
+  if (R.profile) return fail("autotune needs the profile mode off");
+  if (reps < 1) reps = 5;
+  // cache: one line per level configuration, "key tile_rows lpc"
+  std::vector<std::pair<std::string, std::pair<int, int>>> cache;
+  if (cache_path && cache_path[0]) {
+    if (FILE* f = std::fopen(cache_path, "r")) {
+      char key[512];
+      int tr, lp;
+      while (std::fscanf(f, "%511s %d %d", key, &tr, &lp) == 3) cache.push_back({key, {tr, lp}});
+      std::fclose(f);
+    }
+  }
+  std::string rep_txt;
+  std::vector<std::pair<std::string, std::pair<int, int>>> added;
+  for (int l = 0; l < R.num_mesh; l++) {
+    DLevel& L = R.lv[l];
+    // key: the level's block shapes, equations and discretisation
+    std::string key = "nv" + std::to_string(L.o.nvar) + "_s" + std::to_string(L.o.scheme) + "_f" +
+                      std::to_string(L.o.flux) + "_r" + std::to_string(L.o.recon) + "_t" + std::to_string(L.o.turb) +
+                      "_v" + std::to_string(R.cases[0].cp.If_viscous) + (L.fdm ? "_fdm" : "");
+    for (auto& b : L.hb) key += "_" + std::to_string(b.nx) + "x" + std::to_string(b.ny);
+    int best_tr = -1, best_lp = -1;
+    for (auto& e : cache)
+      if (e.first == key) { best_tr = e.second.first; best_lp = e.second.second; }
+    char line[256];
+    if (best_tr < 0) {
+      // residual tile rows
+      double best = 1e30;
+      for (int tr : {1, 2, 4, 8}) {
+        L.tile_rows = tr;
+        if (build_tiles(L)) return 1;
+        double ms;
+        if (time_launches(reps, [&] { return launch_residual(l, 0); }, &ms)) return 1;
+        if (ms < best) { best = ms; best_tr = tr; }
+      }
+      // LU-SGS lanes per cell (the state after a residual; dU only is written)
+      best = 1e30;
+      for (int lp : {1, 2}) {
+        if (lusgs_layout(L, lp, true)) {
+          R.err.clear();
+          continue;  // layout not possible for these blocks
+        }
+        double ms;
+        if (time_launches(reps, [&] { return launch_lusgs(l, L.dblocks_all, (int)L.hb.size(), 0.0, 0); }, &ms)) return 1;
+        if (ms < best) { best = ms; best_lp = lp; }
+      }
+      if (best_lp < 0) return fail("no LU-SGS layout fits this level");
+      added.push_back({key, {best_tr, best_lp}});
+      std::snprintf(line, sizeof(line), "level %d: measured tile_rows %d, lanes per cell %d\n", l + 1, best_tr, best_lp);
+    } else {
+      std::snprintf(line, sizeof(line), "level %d: cached tile_rows %d, lanes per cell %d\n", l + 1, best_tr, best_lp);
+    }
+    rep_txt += line;
+    L.tile_rows = best_tr;
+    if (build_tiles(L)) return 1;
+    if (lusgs_layout(L, best_lp, true)) return 1;
+  }
+  if (!added.empty() && cache_path && cache_path[0]) {
+    if (FILE* f = std::fopen(cache_path, "a")) {
+      for (auto& e : added) std::fprintf(f, "%s %d %d\n", e.first.c_str(), e.second.first, e.second.second);
+      std::fclose(f);
+    }
+  }
+  if (report && report_len > 0) {
+    std::strncpy(report, rep_txt.c_str(), report_len - 1);
+    report[report_len - 1] = 0;
+  }
+  // a captured graph holds the old launch shapes
+  if (R.graph) {
+    HIPCHK(hipGraphExecDestroy(R.graph));
+    R.graph = nullptr;
+  }
   return 0;
 }
 

@@ -66,6 +66,7 @@ SIGNATURES = {
     "ec2d_timing_level": [_i, _dp, _dp, _dp, _ip],
     "ec2d_kernel_time": [_i, _i, _dp, _ip],
     "ec2d_force_coefficients": [_i, _dp, _dp, _dp],
+    "ec2d_autotune": [C.c_char_p, _i, C.c_char_p, _i],
     "ec2d_set_tile_rows": [_i],
 }
 

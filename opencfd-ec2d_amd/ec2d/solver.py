@@ -253,6 +253,13 @@ class Batch:
         capi.call("ec2d_force_coefficients", case_id, C.byref(cl), C.byref(cd), f.ctypes.data_as(C.POINTER(C.c_double)))
         return cl.value, cd.value, f
 
+    def autotune(self, cache_path: str = "", reps: int = 5) -> str:
+        """Time and pick residual tile rows and LU-SGS lanes per cell per level
+        (see ec2d_autotune); returns the per-level report."""
+        buf = C.create_string_buffer(1024)
+        capi.call("ec2d_autotune", cache_path.encode(), reps, buf, 1024)
+        return buf.value.decode()
+
     KERNEL_KINDS = {"residual": 1, "lusgs": 2, "ghost": 3, "lusgs_prep": 4, "lusgs_sweep": 5, "lusgs_scatter": 6}
 
     def kernel_time(self, kind: str, nmesh: int = 1):

@@ -249,3 +249,42 @@ def test_force_coefficients(kind):
     np.testing.assert_allclose(f, rf, rtol=rtol, atol=1e-15 * np.abs(rf).max())
     np.testing.assert_allclose([cl, cd], [rcl, rcd], rtol=10 * rtol)
     assert abs(cl) > 1e-6  # a lifting state, not a trivial zero
+
+
+def test_euler_tall_block_multiwave_sweep():
+    """72 rows of cells: the LU-SGS sweep runs one lane per row over two waves
+    (exchanging the boundary row through LDS every step)."""
+    d = naca_case("g_tall", cases.control_euler_naca(iters=10), nx=129, ny=73)
+    _compare_steps(d, 3)
+
+
+def test_sa_tall_block_multiwave_sweep():
+    c = cases.control_sa_naca(iters=10, Re=1e5, Ma=0.5)
+    d = naca_case("g_tall_sa", c, nx=129, ny=73, ds1=1e-4)
+    _compare_steps(d, 2, arrays=("U", "Res", "dt"), exact=False, rtol=1e-11)
+
+
+def test_autotune_keeps_results_and_caches(tmp_path):
+    """ec2d_autotune picks launch shapes by timing; the results of the steps
+    that follow stay bit-identical to the oracle, and a second call reads the
+    cache instead of measuring."""
+    c = cases.control_euler_naca(iters=10)
+    c.Time_Method, c.Num_Mesh, c.CFL = F.TIME_LU_SGS, 2, 2.0
+    d = naca_case("g_tune", c, nx=129, ny=73)
+    cache = str(tmp_path / "tune.txt")
+    ref = O.OracleCase(d)
+    ref.init_flow()
+    with solver.Batch([solver.Case.from_dir(d)]) as b:
+        b.init_flow()
+        rep = b.autotune(cache, reps=2)
+        assert rep.count("measured") == 2, rep
+        for _ in range(3):
+            ref.main_step()
+            b.main_step()
+        assert first_diff(b.get("U"), ref.block().U) is None
+        b.run(2)
+        for _ in range(2):
+            ref.main_step()
+        assert first_diff(b.get("U"), ref.block().U) is None
+        rep2 = b.autotune(cache, reps=2)
+        assert rep2.count("cached") == 2, rep2
