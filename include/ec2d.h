@@ -42,7 +42,9 @@ enum {
   EC2D_AMU = 9, EC2D_AMU_T = 10,                          /* (-1:nx+1,-1:ny+1), interior filled */
   EC2D_UN = 23,    /* Un(NVun,0:nx,0:ny) */
   EC2D_QF = 24,    /* QF(4,1:nx-1,1:ny-1) */
-  EC2D_DELTU = 25  /* deltU(4,-1:nx+1,-1:ny+1) */
+  EC2D_DELTU = 25, /* deltU(4,-1:nx+1,-1:ny+1) */
+  EC2D_CLOSURE_MUT = 40,  /* learned closure: mu_t of the last inference (1:nx-1,1:ny-1), SA cases */
+  EC2D_CLOSURE_MASK = 41  /* learned closure: 1 where that mu_t applies (1:nx-1,1:ny-1) */
 };
 
 /* ---- life cycle --------------------------------------------------------- */
@@ -149,6 +151,31 @@ long long ec2d_num_cells(int case_id, int nMesh);
  * q = 1/2 (unit chord).  forces (4 doubles, may be NULL): pressure Fx, Fy,
  * viscous Fx, Fy.  Synchronises the stream. */
 int ec2d_force_coefficients(int case_id, double *cl, double *cd, double *forces);
+
+/* ---- learned turbulence closure (no reference counterpart) --------------
+ * A 6 -> 64 -> 64 -> 64 -> 1 MLP (tanh, softplus output) predicts nu_t / nu
+ * per cell of the finest level from six local dimensionless features; BF16
+ * weights and activations, FP32 accumulation, MFMA.  It runs inside SA cases:
+ * SA keeps evolving nu~, and where a cell's features lie in the training
+ * range the MLP's mu_t replaces SA's in the mean-flow viscous flux and the
+ * spectral radii; elsewhere SA's mu_t stays (per-cell fallback).
+ * Parameters: CL_NPARAM = 8833 floats (W1 6x64, b1, W2 64x64, b2, W3 64x64, b3,
+ * W4 64, b4; W row-major [in][out]) and 24 floats of feature normalisation
+ * (mean, 1/std, normalised lower and upper bounds of the valid range). */
+int ec2d_closure_collect(int *nsamples);  /* samples from the current SA state */
+/* Adam over the collected samples (first call: normalisation from the data,
+ * Xavier-uniform start from seed); mean mini-batch loss of the first and of
+ * the last epoch.  batch: a multiple of 64. */
+int ec2d_closure_train(int epochs, int batch, double lr, unsigned seed, double *first_epoch_loss,
+                       double *last_epoch_loss);
+int ec2d_closure_get_params(float *theta, float *norm);
+int ec2d_closure_set_params(const float *theta, const float *norm);
+int ec2d_closure_infer(void);             /* fills EC2D_CLOSURE_MUT / _MASK from the current state */
+int ec2d_closure_enable(int on);          /* use the closure in every following residual */
+/* Fallback on a stall: if the largest density Res_rms of the batch exceeds
+ * growth times the smallest seen since enabling, switch the closure off
+ * (SA everywhere).  active: 1 while the closure is on.  Synchronises. */
+int ec2d_closure_monitor(double growth, int *active);
 
 /* Device time of the last ec2d_run call in milliseconds (HIP events on the
  * library stream), and the mean time per launch of the residual and LU-SGS

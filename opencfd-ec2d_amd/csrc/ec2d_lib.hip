@@ -17,6 +17,7 @@
 #include "ec2d_lusgs.hip"
 #include "ec2d_sst.hip"
 #include "ec2d_forces.hip"
+#include "ec2d_closure.hip"
 #include "ec2d_pointwise.hip"
 #include "ec2d_residual.hip"
 
@@ -97,6 +98,17 @@ struct Runtime {
   std::vector<hipEvent_t> pev;
   std::vector<int> pev_kind;
   double prof_mean[40] = {0};
+  // learned closure (ec2d_closure.hip)
+  struct {
+    bool ready = false, enabled = false;
+    ClosureNet* dnet = nullptr;
+    float *theta = nullptr, *m1 = nullptr, *m2 = nullptr, *part = nullptr, *losses = nullptr, *data = nullptr;
+    long long* cells = nullptr;
+    int ncell = 0, step = 0, npart = 0, nlosses = 0;
+    int* idx = nullptr;
+    float norm[4 * CL_NF] = {0};
+    double best_rms = 0.0;
+  } clo;
   int prof_count[40] = {0};
 };
 
@@ -224,6 +236,12 @@ PlaneMap make_planes(int nv, int nv_un, bool fdm_any) {
   } else {
     p.sst = -1;
   }
+  if (nv == 5) {
+    p.clo = k;  // learned closure: mu_t and its in-range mask
+    k += 2;
+  } else {
+    p.clo = -1;
+  }
   p.nplanes = k;
   return p;
 }
@@ -245,6 +263,44 @@ int check_ready() {
 // ---------------------------------------------------------------- launches
 int launch_pw(int level, int op, int krk = 0);
 
+// interior cells of the finest level, all cases, for the closure kernels
+int closure_cells() {
+  if (R.clo.cells) return 0;
+  DLevel& L = R.lv[0];
+  std::vector<long long> c;
+  for (size_t b = 0; b < L.hb.size(); b++)
+    for (int j = 1; j <= L.hb[b].ny - 1; j++)
+      for (int i = 1; i <= L.hb[b].nx - 1; i++) c.push_back(((long long)b << 40) | ((long long)j << 20) | i);
+  R.clo.ncell = (int)c.size();
+  HIPCHK(hipMalloc(&R.clo.cells, sizeof(long long) * c.size()));
+  HIPCHK(hipMemcpyAsync(R.clo.cells, c.data(), sizeof(long long) * c.size(), hipMemcpyHostToDevice, R.s));
+  HIPCHK(hipStreamSynchronize(R.s));
+  return 0;
+}
+
+ClosureArgs closure_args() {
+  DLevel& L = R.lv[0];
+  ClosureArgs a;
+  a.blocks = L.db;
+  a.cells = R.clo.cells;
+  a.ncell = R.clo.ncell;
+  a.cps = R.dcps;
+  a.arena = L.arena;
+  a.pm = L.pm;
+  a.net = R.clo.dnet;
+  a.data = R.clo.data;
+  return a;
+}
+
+int closure_infer() {
+  if (!R.clo.ready) return fail("closure has no parameters (train or set them first)");
+  if (closure_cells()) return 1;
+  ClosureArgs a = closure_args();
+  hipLaunchKernelGGL(k_closure_infer, dim3((unsigned)((R.clo.ncell + 63) / 64)), dim3(256), 0, R.s, a);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
 int launch_residual(int level, int clip) {
   DLevel& L = R.lv[level];
   ResArgs a;
@@ -259,6 +315,8 @@ int launch_residual(int level, int clip) {
   a.pm = L.pm;
   a.clip = clip;
   a.write_visc = 1;
+  a.closure = (level == 0 && R.clo.enabled && L.pm.clo >= 0) ? 1 : 0;
+  if (a.closure && closure_infer()) return 1;
   bool visc = R.cases[0].cp.If_viscous == 1;
   int nv = L.o.nvar;
   int turb = L.o.turb;
@@ -679,6 +737,8 @@ int array_map(const DLevel& L, int which, int* slot, int* ncomp, int* kind) {
     case EC2D_UN: *slot = p.Un; *ncomp = L.o.nvar_un; *kind = 3; return 0;
     case EC2D_QF: *slot = p.QF; *ncomp = 4; *kind = 2; return 0;
     case EC2D_DELTU: *slot = p.deltU; *ncomp = 4; *kind = 0; return 0;
+    case EC2D_CLOSURE_MUT: if (p.clo < 0) return 1; *slot = p.clo; *ncomp = 1; *kind = 2; return 0;
+    case EC2D_CLOSURE_MASK: if (p.clo < 0) return 1; *slot = p.clo + 1; *ncomp = 1; *kind = 2; return 0;
   }
   return 1;
 }
@@ -727,6 +787,69 @@ int time_launches(int reps, Fn fn, double* ms) {
   return 0;
 }
 
+
+// ---- learned closure: host side ------------------------------------------
+unsigned short host_f2bf(float x) {
+  unsigned u;
+  std::memcpy(&u, &x, 4);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+
+// theta (CL_NPARAM floats) + norm (mean, inv_std, lo, hi) -> device network
+int closure_upload(const float* theta, const float* norm) {
+  ClosureNet N;
+  std::memset(&N, 0, sizeof(N));
+  for (int k = 0; k < CL_NF; k++)
+    for (int n = 0; n < 64; n++) N.w1t[n * 16 + k] = host_f2bf(theta[CL_OW1 + k * 64 + n]);
+  for (int k = 0; k < 64; k++)
+    for (int n = 0; n < 64; n++) {
+      N.w2t[n * 64 + k] = host_f2bf(theta[CL_OW2 + k * 64 + n]);
+      N.w3t[n * 64 + k] = host_f2bf(theta[CL_OW3 + k * 64 + n]);
+    }
+  for (int k = 0; k < 64; k++) N.w4t[k] = host_f2bf(theta[CL_OW4 + k]);
+  for (int n = 0; n < 64; n++) {
+    N.b1[n] = theta[CL_OB1 + n];
+    N.b2[n] = theta[CL_OB2 + n];
+    N.b3[n] = theta[CL_OB3 + n];
+  }
+  N.b4 = theta[CL_OB4];
+  for (int k = 0; k < CL_NF; k++) {
+    N.mean[k] = norm[k];
+    N.inv_std[k] = norm[CL_NF + k];
+    N.lo[k] = norm[2 * CL_NF + k];
+    N.hi[k] = norm[3 * CL_NF + k];
+  }
+  if (!R.clo.dnet) HIPCHK(hipMalloc(&R.clo.dnet, sizeof(ClosureNet)));
+  if (!R.clo.theta) {
+    HIPCHK(hipMalloc(&R.clo.theta, sizeof(float) * CL_NPARAM));
+    HIPCHK(hipMalloc(&R.clo.m1, sizeof(float) * CL_NPARAM));
+    HIPCHK(hipMalloc(&R.clo.m2, sizeof(float) * CL_NPARAM));
+  }
+  // R.s is a non-blocking stream: every copy of the closure's state goes
+  // through it, so that it orders with the kernels that read and write it
+  HIPCHK(hipMemcpyAsync(R.clo.dnet, &N, sizeof(N), hipMemcpyHostToDevice, R.s));
+  HIPCHK(hipMemcpyAsync(R.clo.theta, theta, sizeof(float) * CL_NPARAM, hipMemcpyHostToDevice, R.s));
+  HIPCHK(hipMemsetAsync(R.clo.m1, 0, sizeof(float) * CL_NPARAM, R.s));
+  HIPCHK(hipMemsetAsync(R.clo.m2, 0, sizeof(float) * CL_NPARAM, R.s));
+  HIPCHK(hipStreamSynchronize(R.s));
+  R.clo.step = 0;
+  std::memcpy(R.clo.norm, norm, sizeof(R.clo.norm));
+  R.clo.ready = true;
+  return 0;
+}
+
+// a small deterministic generator for the initial weights and the sample order
+struct Lcg {
+  unsigned long long s;
+  explicit Lcg(unsigned seed) : s(0x9E3779B97F4A7C15ull ^ seed) {}
+  unsigned next() {
+    s = s * 6364136223846793005ull + 1442695040888963407ull;
+    return (unsigned)(s >> 33);
+  }
+  double uniform() { return (next() + 0.5) / 2147483648.0; }  // (0, 1)
+};
+
 extern "C" {
 
 int ec2d_init(void) {
@@ -752,6 +875,8 @@ int ec2d_finalize(void) {
     hipFree(L.dblocks_all); hipFree(L.sk); hipFree(L.dwall); hipFree(L.dcase_wall0); hipFree(L.dforces);
     hipFree(L.d_ilines); hipFree(L.d_jlines); hipFree(L.d_frows);
   }
+  hipFree(R.clo.dnet); hipFree(R.clo.theta); hipFree(R.clo.m1); hipFree(R.clo.m2); hipFree(R.clo.part);
+  hipFree(R.clo.losses); hipFree(R.clo.data); hipFree(R.clo.cells); hipFree(R.clo.idx);
   hipFree(R.dcps);
   hipFree(R.derr);
   for (auto e : R.pev) hipEventDestroy(e);
@@ -1131,6 +1256,7 @@ int ec2d_get_array(int case_id, int nMesh, int mBlock, int which, double* dst) {
 
 int ec2d_init_flow(void) {
   if (check_ready()) return 1;
+  HIPCHK(hipStreamSynchronize(R.s));  // the plane fills below copy on the null stream
   const ec2d_control& c0 = R.cases[0].ctl;
   if (c0.Iflag_init != 0) {
     if (launch_ghost(0, 1, 1)) return 1;
@@ -1360,6 +1486,7 @@ int ec2d_autotune(const char* cache_path, int reps, char* report, int report_len
   if (check_ready()) return 1;
   if (R.profile) return fail("autotune needs the profile mode off");
   if (reps < 1) reps = 5;
+  HIPCHK(hipStreamSynchronize(R.s));  // the layouts below are rewritten with null-stream copies
   // cache: one line per level configuration, "key tile_rows lpc"
   std::vector<std::pair<std::string, std::pair<int, int>>> cache;
   if (cache_path && cache_path[0]) {
@@ -1430,6 +1557,185 @@ int ec2d_autotune(const char* cache_path, int reps, char* report, int report_len
     HIPCHK(hipGraphExecDestroy(R.graph));
     R.graph = nullptr;
   }
+  return 0;
+}
+
+
+int ec2d_closure_collect(int* nsamples) {
+  if (check_ready()) return 1;
+  if (R.lv[0].o.nvar != 5 || R.lv[0].o.turb != 2) return fail("the closure trains on SA cases");
+  if (closure_cells()) return 1;
+  if (!R.clo.data) HIPCHK(hipMalloc(&R.clo.data, sizeof(float) * 8 * (size_t)R.clo.ncell));
+  ClosureArgs a = closure_args();
+  hipLaunchKernelGGL(k_closure_collect, dim3((unsigned)((R.clo.ncell + 255) / 256)), dim3(256), 0, R.s, a);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(R.s));
+  if (nsamples) *nsamples = R.clo.ncell;
+  return 0;
+}
+
+int ec2d_closure_train(int epochs, int batch, double lr, unsigned seed, double* first_epoch_loss,
+                       double* last_epoch_loss) {
+  if (check_ready()) return 1;
+  if (!R.clo.data) return fail("no training data (ec2d_closure_collect)");
+  if (epochs < 1 || batch < 64 || batch % 64 != 0) return fail("epochs >= 1, batch a multiple of 64");
+  const int n = R.clo.ncell;
+  std::vector<float> d((size_t)8 * n);
+  HIPCHK(hipMemcpyAsync(d.data(), R.clo.data, sizeof(float) * d.size(), hipMemcpyDeviceToHost, R.s));
+  HIPCHK(hipStreamSynchronize(R.s));
+  std::vector<int> valid;
+  for (int c = 0; c < n; c++)
+    if (d[(size_t)8 * c + 7] != 0.f) valid.push_back(c);
+  if ((int)valid.size() < batch) return fail("fewer valid samples than one batch");
+  Lcg rng(seed);
+  if (!R.clo.ready) {
+    // normalisation and range of the training features
+    float norm[4 * CL_NF];
+    for (int k = 0; k < CL_NF; k++) {
+      double s1 = 0, s2 = 0, lo = 1e300, hi = -1e300;
+      for (int c : valid) {
+        const double x = d[(size_t)8 * c + k];
+        s1 += x;
+        s2 += x * x;
+      }
+      const double mean = s1 / valid.size();
+      const double var = std::max(s2 / valid.size() - mean * mean, 1e-12);
+      const double inv = 1.0 / std::sqrt(var);
+      for (int c : valid) {
+        const double x = (d[(size_t)8 * c + k] - mean) * inv;
+        lo = std::min(lo, x);
+        hi = std::max(hi, x);
+      }
+      const double pad = 0.05 * (hi - lo);
+      norm[k] = (float)mean;
+      norm[CL_NF + k] = (float)inv;
+      norm[2 * CL_NF + k] = (float)(lo - pad);
+      norm[3 * CL_NF + k] = (float)(hi + pad);
+    }
+    // Xavier-uniform weights, zero biases
+    std::vector<float> th(CL_NPARAM, 0.f);
+    auto fill = [&](int off, int fin, int fout) {
+      const double lim = std::sqrt(6.0 / (fin + fout));
+      for (int k = 0; k < fin * fout; k++) th[off + k] = (float)((2.0 * rng.uniform() - 1.0) * lim);
+    };
+    fill(CL_OW1, CL_NF, 64);
+    fill(CL_OW2, 64, 64);
+    fill(CL_OW3, 64, 64);
+    fill(CL_OW4, 64, 1);
+    if (closure_upload(th.data(), norm)) return 1;
+  }
+  const int steps_per_epoch = (int)valid.size() / batch;
+  const int nsteps = epochs * steps_per_epoch;
+  // sample order: one permutation per epoch
+  std::vector<int> order((size_t)nsteps * batch);
+  for (int e = 0; e < epochs; e++) {
+    std::vector<int> p = valid;
+    for (int k = (int)p.size() - 1; k > 0; k--) std::swap(p[k], p[rng.next() % (unsigned)(k + 1)]);
+    std::copy(p.begin(), p.begin() + (size_t)steps_per_epoch * batch, order.begin() + (size_t)e * steps_per_epoch * batch);
+  }
+  hipFree(R.clo.idx);
+  hipFree(R.clo.part);
+  hipFree(R.clo.losses);
+  HIPCHK(hipMalloc(&R.clo.idx, sizeof(int) * order.size()));
+  HIPCHK(hipMemcpyAsync(R.clo.idx, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice, R.s));
+  const int nwg = batch / 64;
+  HIPCHK(hipMalloc(&R.clo.part, sizeof(float) * (size_t)nwg * (CL_NPARAM + 1)));
+  HIPCHK(hipMalloc(&R.clo.losses, sizeof(float) * nsteps));
+  for (int st = 0; st < nsteps; st++) {
+    TrainArgs ta;
+    ta.net = R.clo.dnet;
+    ta.data = R.clo.data;
+    ta.idx = R.clo.idx + (size_t)st * batch;
+    ta.nbatch = batch;
+    ta.part = R.clo.part;
+    hipLaunchKernelGGL(k_closure_train, dim3(nwg), dim3(256), 0, R.s, ta);
+    AdamArgs aa;
+    aa.part = R.clo.part;
+    aa.nparts = nwg;
+    aa.theta = R.clo.theta;
+    aa.m1 = R.clo.m1;
+    aa.m2 = R.clo.m2;
+    aa.step = ++R.clo.step;
+    aa.lr = (float)lr;
+    aa.beta1 = 0.9f;
+    aa.beta2 = 0.999f;
+    aa.eps = 1e-8f;
+    aa.net = R.clo.dnet;
+    aa.loss = R.clo.losses + st;
+    hipLaunchKernelGGL(k_closure_adam, dim3((CL_NPARAM + 1 + 255) / 256), dim3(256), 0, R.s, aa);
+  }
+  HIPCHK(hipGetLastError());
+  std::vector<float> L(nsteps);
+  HIPCHK(hipMemcpyAsync(L.data(), R.clo.losses, sizeof(float) * nsteps, hipMemcpyDeviceToHost, R.s));
+  HIPCHK(hipStreamSynchronize(R.s));
+  double f = 0, l = 0;
+  for (int k = 0; k < steps_per_epoch; k++) {
+    f += L[k];
+    l += L[(size_t)(epochs - 1) * steps_per_epoch + k];
+  }
+  if (first_epoch_loss) *first_epoch_loss = f / steps_per_epoch;
+  if (last_epoch_loss) *last_epoch_loss = l / steps_per_epoch;
+  return 0;
+}
+
+int ec2d_closure_get_params(float* theta, float* norm) {
+  if (!R.clo.ready) return fail("closure has no parameters");
+  if (theta) {
+    HIPCHK(hipMemcpyAsync(theta, R.clo.theta, sizeof(float) * CL_NPARAM, hipMemcpyDeviceToHost, R.s));
+    HIPCHK(hipStreamSynchronize(R.s));
+  }
+  if (norm) std::memcpy(norm, R.clo.norm, sizeof(R.clo.norm));
+  return 0;
+}
+
+int ec2d_closure_set_params(const float* theta, const float* norm) {
+  if (check_ready()) return 1;
+  if (!theta || !norm) return fail("null parameters");
+  return closure_upload(theta, norm);
+}
+
+int ec2d_closure_infer(void) {
+  if (check_ready()) return 1;
+  if (R.lv[0].pm.clo < 0) return fail("the closure runs on SA cases");
+  return closure_infer();
+}
+
+int ec2d_closure_enable(int on) {
+  if (check_ready()) return 1;
+  if (on && R.lv[0].pm.clo < 0) return fail("the closure runs on SA cases");
+  if (on && !R.clo.ready) return fail("closure has no parameters (train or set them first)");
+  // the cell list exists before a run graph is captured (no allocation under capture)
+  if (on && closure_cells()) return 1;
+  R.clo.enabled = on != 0;
+  R.clo.best_rms = 0.0;
+  if (R.graph) {
+    HIPCHK(hipGraphExecDestroy(R.graph));
+    R.graph = nullptr;
+  }
+  return 0;
+}
+
+int ec2d_closure_monitor(double growth, int* active) {
+  if (check_ready()) return 1;
+  if (R.clo.enabled) {
+    double worst = 0.0;
+    for (int c = 0; c < (int)R.cases.size(); c++) {
+      double mx[6], rms[6];
+      int rc = ec2d_res_norms(c, 1, mx, rms);
+      if (rc == 1) return 1;
+      worst = std::max(worst, rms[0]);
+    }
+    if (R.clo.best_rms == 0.0 || worst < R.clo.best_rms) R.clo.best_rms = worst;
+    else if (worst > growth * R.clo.best_rms || !std::isfinite(worst)) {
+      // the residual stalls or grows: fall back to SA everywhere
+      R.clo.enabled = false;
+      if (R.graph) {
+        HIPCHK(hipGraphExecDestroy(R.graph));
+        R.graph = nullptr;
+      }
+    }
+  }
+  if (active) *active = R.clo.enabled ? 1 : 0;
   return 0;
 }
 

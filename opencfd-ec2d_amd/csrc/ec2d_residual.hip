@@ -33,6 +33,7 @@ struct ResArgs {
   PlaneMap pm;
   int clip;                   // apply force_vt_kw to interior nu_t on load
   int write_visc;             // store Amu / Amu_t planes
+  int closure;                // SA: take mu_t of interior cells from the closure planes where in range
 };
 
 template <int TX, int TY, int NV, bool VISC, bool SA>
@@ -109,6 +110,7 @@ __global__ __launch_bounds__(TX* TY) void k_residual(ResArgs a) {
           double fv1 = X3 / (X3 + Cv1_3);
           at = fv1 * d * vt;
         }
+        if (a.closure && interior && PL(pm.clo + 1)[q] != 0.0) at = PL(pm.clo)[q];
       }
       // SST: mu_t as the model left it (interior and wall ghosts set by
       // k_sst_res, other ghosts untouched)

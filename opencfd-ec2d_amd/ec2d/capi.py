@@ -16,6 +16,9 @@ EC2D_U, EC2D_RES, EC2D_DT, EC2D_DU = 1, 2, 3, 4
 EC2D_LCI, EC2D_LCJ, EC2D_LVI, EC2D_LVJ = 5, 6, 7, 8
 EC2D_AMU, EC2D_AMU_T = 9, 10
 EC2D_UN, EC2D_QF, EC2D_DELTU = 23, 24, 25
+EC2D_CLOSURE_MUT, EC2D_CLOSURE_MASK = 40, 41
+CLOSURE_NPARAM = 6 * 64 + 64 + 64 * 64 + 64 + 64 * 64 + 64 + 64 + 1  # W1 b1 W2 b2 W3 b3 W4 b4
+CLOSURE_NNORM = 24  # mean, inv_std, lo, hi of the 6 features
 
 
 class Control(C.Structure):
@@ -67,6 +70,13 @@ SIGNATURES = {
     "ec2d_kernel_time": [_i, _i, _dp, _ip],
     "ec2d_force_coefficients": [_i, _dp, _dp, _dp],
     "ec2d_autotune": [C.c_char_p, _i, C.c_char_p, _i],
+    "ec2d_closure_collect": [_ip],
+    "ec2d_closure_train": [_i, _i, C.c_double, C.c_uint, _dp, _dp],
+    "ec2d_closure_get_params": [C.POINTER(C.c_float), C.POINTER(C.c_float)],
+    "ec2d_closure_set_params": [C.POINTER(C.c_float), C.POINTER(C.c_float)],
+    "ec2d_closure_infer": [],
+    "ec2d_closure_enable": [_i],
+    "ec2d_closure_monitor": [C.c_double, _ip],
     "ec2d_set_tile_rows": [_i],
 }
 

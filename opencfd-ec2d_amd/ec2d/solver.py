@@ -146,10 +146,13 @@ class Batch:
         capi.EC2D_UN: lambda B, nv: (B.ny + 1, B.nx + 1, nv),
         capi.EC2D_QF: lambda B, nv: (B.ny - 1, B.nx - 1, 4),
         capi.EC2D_DELTU: lambda B, nv: (B.ny + 3, B.nx + 3, 4),
+        capi.EC2D_CLOSURE_MUT: lambda B, nv: (B.ny - 1, B.nx - 1),
+        capi.EC2D_CLOSURE_MASK: lambda B, nv: (B.ny - 1, B.nx - 1),
     }
     NAMES = {"U": capi.EC2D_U, "Res": capi.EC2D_RES, "dt": capi.EC2D_DT, "dU": capi.EC2D_DU, "Lci": capi.EC2D_LCI,
              "Lcj": capi.EC2D_LCJ, "Lvi": capi.EC2D_LVI, "Lvj": capi.EC2D_LVJ, "Amu": capi.EC2D_AMU,
-             "Amu_t": capi.EC2D_AMU_T, "Un": capi.EC2D_UN, "QF": capi.EC2D_QF, "deltU": capi.EC2D_DELTU}
+             "Amu_t": capi.EC2D_AMU_T, "Un": capi.EC2D_UN, "QF": capi.EC2D_QF, "deltU": capi.EC2D_DELTU,
+             "closure_mut": capi.EC2D_CLOSURE_MUT, "closure_mask": capi.EC2D_CLOSURE_MASK}
 
     def _nv(self, case: Case, nmesh: int) -> int:
         return case.ctl.nvar if nmesh == 1 else 4
@@ -259,6 +262,45 @@ class Batch:
         buf = C.create_string_buffer(1024)
         capi.call("ec2d_autotune", cache_path.encode(), reps, buf, 1024)
         return buf.value.decode()
+
+    # ---- learned closure (SA cases; see ec2d_closure.hip) -------------------
+    def closure_collect(self) -> int:
+        """Training samples (features, SA nu_t/nu) from the current state; returns the sample count."""
+        n = C.c_int()
+        capi.call("ec2d_closure_collect", C.byref(n))
+        return n.value
+
+    def closure_train(self, epochs=5, batch=1024, lr=1e-3, seed=0):
+        """Adam on the collected samples; returns (mean loss of the first epoch, of the last epoch)."""
+        f, l = C.c_double(), C.c_double()
+        capi.call("ec2d_closure_train", epochs, batch, lr, seed, C.byref(f), C.byref(l))
+        return f.value, l.value
+
+    def closure_params(self):
+        th = np.zeros(capi.CLOSURE_NPARAM, dtype=np.float32)
+        nm = np.zeros(capi.CLOSURE_NNORM, dtype=np.float32)
+        fp = C.POINTER(C.c_float)
+        capi.call("ec2d_closure_get_params", th.ctypes.data_as(fp), nm.ctypes.data_as(fp))
+        return th, nm
+
+    def closure_set_params(self, theta, norm):
+        th = np.ascontiguousarray(theta, dtype=np.float32)
+        nm = np.ascontiguousarray(norm, dtype=np.float32)
+        assert th.size == capi.CLOSURE_NPARAM and nm.size == capi.CLOSURE_NNORM
+        fp = C.POINTER(C.c_float)
+        capi.call("ec2d_closure_set_params", th.ctypes.data_as(fp), nm.ctypes.data_as(fp))
+
+    def closure_infer(self):
+        capi.call("ec2d_closure_infer")
+
+    def closure_enable(self, on=True):
+        capi.call("ec2d_closure_enable", 1 if on else 0)
+
+    def closure_monitor(self, growth=10.0) -> bool:
+        """Fall back to SA if the density residual grew past growth x its best; returns whether the closure is on."""
+        a = C.c_int()
+        capi.call("ec2d_closure_monitor", growth, C.byref(a))
+        return bool(a.value)
 
     KERNEL_KINDS = {"residual": 1, "lusgs": 2, "ghost": 3, "lusgs_prep": 4, "lusgs_sweep": 5, "lusgs_scatter": 6}
 

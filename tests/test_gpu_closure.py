@@ -1,0 +1,276 @@
+"""Learned closure on the GPU: BF16 MFMA inference and training against a
+numpy reference that rounds to bf16 at the same points (inputs, weights,
+hidden activations, deltas), and its use inside the solver.
+
+The reference has no learned closure; these tests pin the device kernels to
+the definition written in opencfd-ec2d_amd/csrc/ec2d_closure.hip.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from ec2d import cases
+from ec2d import solver
+from helpers import first_diff, naca_case
+
+pytestmark = pytest.mark.gpu
+
+NP = 6 * 64 + 64 + 64 * 64 + 64 + 64 * 64 + 64 + 64 + 1
+
+
+def bf16(x):
+    """Round float32 to bfloat16 (nearest even), returned as float32."""
+    u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32)
+
+
+def unpack(theta):
+    o = 0
+    out = []
+    for shape in ((6, 64), (64,), (64, 64), (64,), (64, 64), (64,), (64, 1), (1,)):
+        n = int(np.prod(shape))
+        out.append(theta[o:o + n].reshape(shape).astype(np.float32))
+        o += n
+    return out
+
+
+def features(b, case):
+    """The six raw features of every interior cell (float32), as the device
+    defines them, from the GPU state and the host geometry."""
+    U = b.get("U")  # (ny+3, nx+3, nv), Fortran index k at numpy k+1
+    g = case.levels[0][0]
+    ctl = case.ctl
+    gam = ctl.gamma
+    ny, nx = g.ny, g.nx
+    d, mu_, mv, E = (U[..., m] for m in range(4))
+    u, v = mu_ / d, mv / d
+    p = (gam - 1.0) * (E - 0.5 * d * (u * u + v * v))
+    x1, y1 = g.x1, g.y1
+    I = slice(2, nx + 1)  # Fortran 1..nx-1
+    J = slice(2, ny + 1)
+    Ie, Iw, Jn, Js = slice(3, nx + 2), slice(1, nx), slice(3, ny + 2), slice(1, ny)
+    Dix = (x1[J, Ie] - x1[J, Iw]) * 0.5
+    Diy = (y1[J, Ie] - y1[J, Iw]) * 0.5
+    Djx = (x1[Jn, I] - x1[Js, I]) * 0.5
+    Djy = (y1[Jn, I] - y1[Js, I]) * 0.5
+    Ds = 1.0 / (Dix * Djy - Djx * Diy)
+
+    def grad(a):
+        Di = (a[J, Ie] - a[J, Iw]) * 0.5
+        Dj = (a[Jn, I] - a[Js, I]) * 0.5
+        return (Di * Djy - Dj * Diy) * Ds, (-Di * Djx + Dj * Dix) * Ds
+
+    ux, uy = grad(u)
+    vx, vy = grad(v)
+    px, py = grad(p)
+    dc, uc, vc, pc = d[J, I], u[J, I], v[J, I], p[J, I]
+    T = (E[J, I] - 0.5 * dc * (uc * uc + vc * vc)) / (ctl.Cv * dc)
+    Tsb = 110.4 / ctl.T_inf
+    mu = 1.0 / ctl.Re * (1.0 + Tsb) * np.sqrt(T * T * T) / (Tsb + T)
+    nu = mu / dc
+    dw = g.dw[1:ny, 1:nx]
+    om = np.abs(vx - uy)
+    S = np.sqrt(2.0 * ux * ux + 2.0 * vy * vy + (uy + vx) ** 2)
+    um = np.sqrt(uc * uc + vc * vc)
+    c = np.sqrt(gam * pc / dc)
+    gp = np.sqrt(px * px + py * py)
+    f = np.stack([np.log1p(dw * dw * om / nu), np.log1p(dw * um / nu), S / (S + om + 1e-30), um / c,
+                  (uc * px + vc * py) / (um * gp + 1e-30), np.log1p(dw * gp / (dc * um * um + 1e-30))], axis=-1)
+    return f.astype(np.float32).reshape(-1, 6), mu.reshape(-1)
+
+
+def forward(theta, norm, f, ok=None):
+    """Reference forward pass with the kernels' rounding points; returns
+    (z4, ok, X, H1, H2, H3).  ok: rows fed to the network (default: finite
+    features inside [lo, hi], as inference decides)."""
+    W1, b1, W2, b2, W3, b3, W4, b4 = unpack(theta)
+    mean, inv, lo, hi = norm[:6], norm[6:12], norm[12:18], norm[18:24]
+    xn = (f - mean) * inv
+    if ok is None:
+        ok = np.all(np.isfinite(f), axis=1) & np.all((xn >= lo) & (xn <= hi), axis=1)
+    X = bf16(np.where(ok[:, None], xn, 0.0).astype(np.float32))
+    H1 = np.tanh(X @ bf16(W1) + b1)
+    H2 = np.tanh(bf16(H1) @ bf16(W2) + b2)
+    H3 = np.tanh(bf16(H2) @ bf16(W3) + b3)
+    z4 = (bf16(H3) @ bf16(W4))[:, 0] + b4[0]
+    return z4, ok, X, H1, H2, H3
+
+
+def softplus(z):
+    return np.maximum(z, 0) + np.log1p(np.exp(-np.abs(z)))
+
+
+def sa_state(name, steps=20):
+    c = cases.control_sa_naca(iters=40, Re=1e5, Ma=0.5)
+    d = naca_case(name, c, ds1=1e-4)  # 64 x 16 = 1024 interior cells
+    case = solver.Case.from_dir(d)
+    return d, case
+
+
+def test_closure_inference_matches_reference():
+    d, case = sa_state("g_clo_inf")
+    with solver.Batch([case]) as b:
+        b.init_flow()
+        b.run(20)
+        n = b.closure_collect()
+        assert n == 1024
+        b.closure_train(epochs=2, batch=256, lr=3e-3, seed=7)
+        theta, norm = b.closure_params()
+        b.closure_infer()
+        mut = b.get("closure_mut").reshape(-1)
+        mask = b.get("closure_mask").reshape(-1)
+        f, mu = features(b, case)
+    z4, ok, *_ = forward(theta, norm, f)
+    assert np.array_equal(mask != 0, ok)
+    assert ok.sum() > 900
+    ref = mu * softplus(z4.astype(np.float64))
+    rel = np.abs(mut[ok] - ref[ok]) / np.maximum(np.abs(ref[ok]), 1e-300)
+    assert np.median(rel) < 1e-4, np.median(rel)
+    np.testing.assert_allclose(mut[ok], ref[ok], rtol=2e-2)
+    assert np.all(mut[~ok] == 0.0)
+
+
+def grad_reference(theta, norm, f, t):
+    """Loss mean((log1p(yhat) - log1p(t))^2) and its gradient, with the
+    kernels' bf16 rounding of X, weights, activations and deltas (training
+    feeds every valid sample, without the range test)."""
+    W1, b1, W2, b2, W3, b3, W4, b4 = unpack(theta)
+    z4, ok, X, H1, H2, H3 = forward(theta, norm, f, ok=np.ones(f.shape[0], bool))
+    nb = f.shape[0]
+    y = softplus(z4)
+    e = np.log1p(y) - np.log1p(np.maximum(t, 0))
+    d4 = (2 * e / (1 + y) / (1 + np.exp(-z4)) / nb).astype(np.float32)
+    D3 = d4[:, None] * bf16(W4)[:, 0][None, :] * (1 - H3 * H3)
+    gW4 = (H3 * d4[:, None]).sum(0)
+    gW3 = bf16(H2).T @ bf16(D3)
+    D2 = (bf16(D3) @ bf16(W3).T) * (1 - H2 * H2)
+    gW2 = bf16(H1).T @ bf16(D2)
+    D1 = (bf16(D2) @ bf16(W2).T) * (1 - H1 * H1)
+    gW1 = X[:, :6].T @ bf16(D1)
+    g = np.concatenate([gW1.ravel(), D1.sum(0), gW2.ravel(), D2.sum(0), gW3.ravel(), D3.sum(0), gW4.ravel(),
+                        [d4.sum()]])
+    return (e * e).mean(), g
+
+
+def test_closure_training_step_gradient():
+    """One Adam step from known parameters over the whole sample set moves
+    every parameter with a clear gradient against that gradient's sign."""
+    d, case = sa_state("g_clo_grad")
+    with solver.Batch([case]) as b:
+        b.init_flow()
+        b.run(20)
+        b.closure_collect()
+        b.closure_train(epochs=1, batch=256, lr=1e-3, seed=3)  # sets the normalisation
+        theta0, norm = b.closure_params()
+        b.closure_set_params(theta0, norm)  # fresh Adam moments
+        lr = 1e-4
+        b.closure_train(epochs=1, batch=1024, lr=lr, seed=5)
+        theta1, _ = b.closure_params()
+        f, mu = features(b, case)
+        U = b.get("U")
+    # the SA target nu_t / nu of each cell (fv1 d nu~ / mu)
+    dd = U[2:-2, 2:-2, 0].reshape(-1)
+    vt = U[2:-2, 2:-2, 4].reshape(-1)
+    X = dd * vt / mu
+    t = (X * X * X / (X * X * X + 7.1 * 7.1 * 7.1) * dd * vt / mu).astype(np.float32)
+    valid = np.all(np.isfinite(f), axis=1) & np.isfinite(t) & (t >= 0)
+    assert valid.all(), "the one-batch gradient check needs every cell valid"
+    loss, g = grad_reference(theta0, norm, f, t)
+    step = (theta0 - theta1).astype(np.float64)
+    big = np.abs(g) > 1e-2 * np.abs(g).max()
+    agree = np.sign(step[big]) == np.sign(g[big])
+    assert agree.mean() > 0.99, agree.mean()
+    # Adam's first step has size lr where |g| >> eps
+    np.testing.assert_allclose(np.abs(step[big]), lr, rtol=0.05)
+
+
+def test_closure_loss_finite_differences():
+    """Central differences of the device's batch loss (training at lr = 0
+    leaves the parameters as they are) on the FP32 biases agree with the
+    reference gradient."""
+    d, case = sa_state("g_clo_fd")
+    with solver.Batch([case]) as b:
+        b.init_flow()
+        b.run(20)
+        b.closure_collect()
+        b.closure_train(epochs=3, batch=256, lr=3e-3, seed=4)
+        theta0, norm = b.closure_params()
+        f, mu = features(b, case)
+        U = b.get("U")
+        dd = U[2:-2, 2:-2, 0].reshape(-1)
+        vt = U[2:-2, 2:-2, 4].reshape(-1)
+        X = dd * vt / mu
+        t = (X * X * X / (X * X * X + 7.1 * 7.1 * 7.1) * dd * vt / mu).astype(np.float32)
+        loss_ref, g = grad_reference(theta0, norm, f, t)
+
+        def loss(th):
+            b.closure_set_params(th, norm)
+            return b.closure_train(epochs=1, batch=1024, lr=0.0, seed=0)[0]
+
+        L0 = loss(theta0)
+        np.testing.assert_allclose(L0, loss_ref, rtol=1e-3)
+        ob = {"b1": 384, "b2": 384 + 64 + 4096, "b3": 384 + 64 + 4096 + 64 + 4096, "b4": NP - 1}
+        picks = [ob["b4"]] + [ob[k] + int(np.argmax(np.abs(g[ob[k]:ob[k] + 64]))) for k in ("b1", "b2", "b3")]
+        for k in picks:
+            h = 1e-2 * max(abs(float(theta0[k])), 1.0)
+            tp, tm = theta0.copy(), theta0.copy()
+            tp[k] += h
+            tm[k] -= h
+            fd = (loss(tp) - loss(tm)) / (2 * h)
+            assert abs(fd - g[k]) <= 0.05 * abs(g[k]) + 2e-5 * abs(L0), (k, fd, g[k], L0)
+
+
+def test_closure_training_reduces_loss():
+    d, case = sa_state("g_clo_train")
+    with solver.Batch([case]) as b:
+        b.init_flow()
+        b.run(20)
+        b.closure_collect()
+        first, last = b.closure_train(epochs=12, batch=256, lr=3e-3, seed=1)
+    assert np.isfinite(first) and np.isfinite(last)
+    assert last < 0.5 * first, (first, last)
+
+
+def test_closure_out_of_range_is_sa():
+    """Every feature outside the valid range: the closure never applies, and
+    the run is the SA run (per-cell fallback)."""
+    d, case = sa_state("g_clo_fb")
+    theta = np.zeros(NP, dtype=np.float32)
+    norm = np.zeros(24, dtype=np.float32)
+    norm[6:12] = 1.0
+    norm[12:18] = 1e30   # lo > hi: nothing is in range
+    norm[18:24] = -1e30
+    ref = O.OracleCase(d)
+    ref.init_flow()
+    with solver.Batch([case]) as b:
+        b.init_flow()
+        b.closure_set_params(theta, norm)
+        b.closure_enable(True)
+        for _ in range(3):
+            ref.main_step()
+            b.main_step()
+        assert np.all(b.get("closure_mask") == 0)
+        np.testing.assert_allclose(b.get("U"), ref.block(1, 1).U, rtol=1e-11, atol=1e-300)
+
+
+def test_closure_changes_the_flow_and_monitor_falls_back():
+    d, case = sa_state("g_clo_on")
+    with solver.Batch([case]) as b:
+        b.init_flow()
+        b.run(10)
+        b.closure_collect()
+        b.closure_train(epochs=2, batch=256, lr=3e-3, seed=2)
+        b.closure_enable(True)
+        b.run(3)
+        assert b.get("closure_mask").sum() > 500
+        U_on = b.get("U")
+        assert b.closure_monitor(growth=1e30)      # records the best residual
+        assert not b.closure_monitor(growth=1e-30)  # no improvement, grown past 1e-30 x best: back to SA
+        assert b.get("closure_mask").sum() > 500    # the last inference stays readable
+    with solver.Batch([solver.Case.from_dir(d)]) as b2:
+        b2.init_flow()
+        b2.run(13)
+        U_sa = b2.get("U")
+    assert first_diff(U_on, U_sa) is not None
+    assert np.all(np.isfinite(U_on))
