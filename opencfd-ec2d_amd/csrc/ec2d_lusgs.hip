@@ -15,12 +15,25 @@
 // sweep dU(i+1,j) and dU(i,j+1).  Any order that respects these dependencies
 // gives the reference's result bit for bit.  Two lanes own one row of cells
 // and step s handles hyperplane s (sub_LU_SGS.f90:25-29, 70-75): row t takes
-// its cell at column offset c = s - t.  Of the pair, one lane evaluates the
-// j-neighbour's flux difference and the other the i-neighbour's; they swap
-// the two halves with a DPP quad permute and both finish the cell.  The
-// i-neighbour's result is the row's own from the step before (registers); the
-// j-neighbour's is that of row t-1 from the step before, one lane down (DPP
-// wave shift), or through LDS between waves.
+// its cell at column offset c = s - t.  Of the pair, lane 2t evaluates the
+// j-neighbour's flux difference and lane 2t+1 the i-neighbour's; they swap
+// the two halves with a DPP quad permute and both finish the cell with the
+// same bits.  Every lane then finds its neighbour one lane down (DPP wave
+// shift): lane 2t the result of row t-1 (the j-neighbour), lane 2t+1 the
+// result of its own row at the step before (the i-neighbour); the first lane
+// of a wave gets row t-1 through LDS.
+//
+// The pair never selects by lane.  The backward sum dui + duj is one addition
+// and commutes.  The forward sum (Res + dui) + duj becomes pre + swap(pre)
+// with pre = X_k + (own term): lane 2t+1 loads Res as X_k, lane 2t loads -0.0
+// (the unwritten trash pad of the forward set's X fields holds it), and
+// -0.0 + x == x for every x, so both lanes add (Res + dui) and duj.
+//
+// Division by the diagonal.  The prep stores y = 1/alfa (correctly rounded);
+// the sweep forms q = a*y and refines it twice with FMA residuals (Markstein):
+// the first refinement makes q faithful, the second, with y = RN(1/alfa),
+// returns RN(a/alfa) - the quotient the reference's division gives, in five
+// dependent operations instead of the eleven of a full division.
 //
 // Record layout ("skewed").  A block's records are stored hyperplane by
 // hyperplane: shared field f of the cell that row t handles at step s sits at
@@ -64,9 +77,10 @@ struct LusgsArgs {
 //   side field g of position pos:    NSH*n + g*2n + 2*pos + k
 template <int NV, bool VISC>
 struct RecF {
-  // shared: Res (forward) or the forward dU (backward), U, the diagonal
-  static constexpr int X = 0, U = NV, AL0 = 2 * NV, AL4 = 2 * NV + 1;
-  static constexpr int NSH = 2 * NV + (NV == 5 ? 2 : 1);
+  // shared: Res (forward) or the forward dU (backward), U, the diagonal and
+  // its correctly rounded reciprocal (a separate one for the SA equation)
+  static constexpr int X = 0, U = NV, AL0 = 2 * NV, AL4 = 2 * NV + 1, Y0 = 2 * NV + 2, Y4 = 2 * NV + 3;
+  static constexpr int NSH = 2 * NV + 4;
   // per side: F(U_nb).n, the normal, the face area, the neighbour's inviscid
   // and viscous spectral radius
   static constexpr int F = 0, N1 = NV, N2 = NV + 1, A = NV + 2, LC = NV + 3, LV = NV + 4;
@@ -75,7 +89,7 @@ struct RecF {
 };
 
 // host-side record size in doubles per position, equal to RecF<nv, visc>::NREC
-inline int lusgs_rec_doubles(int nv, bool visc) { return 2 * nv + (nv == 5 ? 2 : 1) + 2 * (nv + 4 + (visc ? 1 : 0)); }
+inline int lusgs_rec_doubles(int nv, bool visc) { return 2 * nv + 4 + 2 * (nv + 4 + (visc ? 1 : 0)); }
 
 // operands of one lane for one cell: the shared part and NS sides (NS = 1:
 // the lane's own side k, NS = 2: side 0 (j) and side 1 (i))
@@ -83,7 +97,7 @@ template <int NV, int NS>
 struct Rec {
   double X[NV], U[NV];
   double F[NS][NV], n1[NS], n2[NS], a[NS], Lc[NS], Lv[NS];
-  double al0, al4;
+  double al0, al4, y0, y4;
 };
 
 // one cell as its neighbours need it: dU and U + dU
@@ -92,20 +106,29 @@ struct Out {
   double du[NV], W[NV];
 };
 
-// lane l-1 -> lane l (lane 0 receives 0)
+// lane l-1 -> lane l (lane 0 receives 0: bound_ctrl)
 __device__ __forceinline__ double shr1(double v) {
-  int lo = __double2loint(v), hi = __double2hiint(v);
-  lo = __builtin_amdgcn_update_dpp(0, lo, 0x138, 0xF, 0xF, false);
-  hi = __builtin_amdgcn_update_dpp(0, hi, 0x138, 0xF, 0xF, false);
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x138, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x138, 0xF, 0xF, true);
   return __hiloint2double(hi, lo);
 }
 
 // lane l <-> lane l^1 (quad_perm [1,0,3,2])
 __device__ __forceinline__ double swap_pair(double v) {
-  int lo = __double2loint(v), hi = __double2hiint(v);
-  lo = __builtin_amdgcn_update_dpp(0, lo, 0xB1, 0xF, 0xF, false);
-  hi = __builtin_amdgcn_update_dpp(0, hi, 0xB1, 0xF, 0xF, false);
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0xB1, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0xB1, 0xF, 0xF, true);
   return __hiloint2double(hi, lo);
+}
+
+// a / b, correctly rounded, from y = RN(1/b) (Markstein): for b > 0 and
+// normal-range operands; q is faithful after the first refinement, and a
+// faithful q refined with RN(1/b) is RN(a/b)
+__device__ __forceinline__ double div_rcp(double a, double b, double y) {
+  double q = a * y;
+  double r = __builtin_fma(-b, q, a);
+  q = __builtin_fma(r, y, q);
+  r = __builtin_fma(-b, q, a);
+  return __builtin_fma(r, y, q);
 }
 
 __device__ __forceinline__ double bload(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
@@ -119,15 +142,16 @@ __device__ __forceinline__ void bstore(double v, __amdgpu_buffer_rsrc_t r, uint3
 
 // base: byte offset of the record set in the block's record region; NS = 1
 // loads side k, NS = 2 both sides
+// vx: byte offset of the X loads (the lane's position, or a trash-pad slot)
 template <int NV, bool VISC, int NS>
 __device__ __forceinline__ void load_rec(Rec<NV, NS>& p, __amdgpu_buffer_rsrc_t r, uint32_t base, uint32_t n8,
-                                         uint32_t pos, int k) {
+                                         uint32_t pos, uint32_t vx, int k) {
   using F = RecF<NV, VISC>;
   const uint32_t vs = pos * 8u;
   const uint32_t sb = base + F::NSH * n8;  // side part
 #pragma unroll
   for (int m = 0; m < NV; m++) {
-    p.X[m] = bload(r, vs, base + (F::X + m) * n8);
+    p.X[m] = bload(r, vx, base + (F::X + m) * n8);
     p.U[m] = bload(r, vs, base + (F::U + m) * n8);
   }
 #pragma unroll
@@ -143,8 +167,14 @@ __device__ __forceinline__ void load_rec(Rec<NV, NS>& p, __amdgpu_buffer_rsrc_t 
     else p.Lv[h] = 0.0;
   }
   p.al0 = bload(r, vs, base + F::AL0 * n8);
-  if constexpr (NV == 5) p.al4 = bload(r, vs, base + F::AL4 * n8);
-  else p.al4 = p.al0;
+  p.y0 = bload(r, vs, base + F::Y0 * n8);
+  if constexpr (NV == 5) {
+    p.al4 = bload(r, vs, base + F::AL4 * n8);
+    p.y4 = bload(r, vs, base + F::Y4 * n8);
+  } else {
+    p.al4 = p.al0;
+    p.y4 = p.y0;
+  }
 }
 
 // ----------------------------------------------------------------- prep
@@ -187,7 +217,12 @@ __device__ __forceinline__ void write_rec(double* __restrict__ rb, int64_t n, in
   double alfa = vol / dt + 1.0 * (Lci[q] + Lcj[q]) + vol * alfa1;
   if constexpr (VISC) alfa = alfa + 2.0 * (Lvi[q] + Lvj[q]);
   sh[F::AL0 * n] = alfa;
-  if constexpr (NV == 5) sh[F::AL4 * n] = alfa + Lvi[q] + Lvj[q];
+  sh[F::Y0 * n] = 1.0 / alfa;
+  if constexpr (NV == 5) {
+    const double alfa4 = alfa + Lvi[q] + Lvj[q];
+    sh[F::AL4 * n] = alfa4;
+    sh[F::Y4 * n] = 1.0 / alfa4;
+  }
   // sides: k = 0 the j-neighbour, k = 1 the i-neighbour
 #pragma unroll
   for (int k = 0; k < 2; k++) {
@@ -232,6 +267,12 @@ __global__ __launch_bounds__(256) void k_lusgs_prep(LusgsArgs a) {
     // the backward X fields of padding positions (valid ones come from the forward sweep)
 #pragma unroll
     for (int m = 0; m < NV; m++) fw[F::NREC * n + (F::X + m) * n + pos] = 0.0;
+  }
+  if (pos == 0) {
+    // -0.0 in the first trash-pad slot of the forward X fields: the X operand
+    // of the j-neighbour lane of a pair (no sweep stores there)
+#pragma unroll
+    for (int m = 0; m < NV; m++) fw[(F::X + m) * n + (int64_t)B.sk_S * B.sk_R] = -0.0;
   }
 }
 
@@ -296,33 +337,31 @@ __device__ __forceinline__ void sweep_step(const Rec<NV, 3 - LPC>& p, Out<NV>& o
   }
   const int c = s - t;
   const bool act = t < nrow && c >= 0 && c < ncol;
-  double dui[NV], duj[NV];
+  // sum: (Res + dui) + duj forward, dui + duj backward
+  double sum[NV];
   if constexpr (LPC == 2) {
-    Out<NV> nb;
-#pragma unroll
-    for (int m = 0; m < NV; m++) {
-      nb.du[m] = k == 0 ? up.du[m] : own.du[m];
-      nb.W[m] = k == 0 ? up.W[m] : own.W[m];
-    }
+    // the neighbour one lane down is the j-neighbour for lane 2t and the
+    // i-neighbour for lane 2t+1
     double mine[NV];
-    side_term<NV, VISC, FWD, 1>(mine, p, 0, nb, k == 0 ? t != 0 : c != 0, g);
+    side_term<NV, VISC, FWD, 1>(mine, p, 0, up, k == 0 ? t != 0 : c != 0, g);
 #pragma unroll
     for (int m = 0; m < NV; m++) {
-      const double other = swap_pair(mine[m]);
-      dui[m] = k == 1 ? mine[m] : other;
-      duj[m] = k == 0 ? mine[m] : other;
+      const double pre = FWD ? p.X[m] + mine[m] : mine[m];  // forward: X = Res (lane 2t+1), -0.0 (lane 2t)
+      sum[m] = pre + swap_pair(pre);
     }
   } else {
+    double dui[NV], duj[NV];
     side_term<NV, VISC, FWD, 2>(duj, p, 0, up, t != 0, g);
     side_term<NV, VISC, FWD, 2>(dui, p, 1, own, c != 0, g);
+#pragma unroll
+    for (int m = 0; m < NV; m++) sum[m] = FWD ? p.X[m] + dui[m] + duj[m] : dui[m] + duj[m];
   }
   Out<NV> o;
 #pragma unroll
   for (int m = 0; m < NV; m++) {
-    const double al = (NV == 5 && m == 4) ? p.al4 : p.al0;
-    double du;
-    if (FWD) du = (p.X[m] + dui[m] + duj[m]) / al;
-    else du = p.X[m] + (dui[m] + duj[m]) / al;
+    const bool sa = NV == 5 && m == 4;
+    const double q = div_rcp(sum[m], sa ? p.al4 : p.al0, sa ? p.y4 : p.y0);
+    const double du = FWD ? q : p.X[m] + q;
     o.du[m] = du;
     o.W[m] = p.U[m] + du;
   }
@@ -370,10 +409,14 @@ __device__ __forceinline__ void sweep(__amdgpu_buffer_rsrc_t rs, uint32_t rec, u
   constexpr int NS = 3 - LPC;
   Out<NV> own = {};
   Rec<NV, NS> A[U], B[U];
+  // forward pairs: lane 2t reads X (-0.0) from the first trash-pad slot
+  const bool xpad = FWD && LPC == 2 && k == 0;
   auto load_set = [&](Rec<NV, NS>* set, int s_first) {
 #pragma unroll
-    for (int u = 0; u < U; u++)
-      load_rec<NV, VISC, NS>(set[u], rs, rec, n8, (uint32_t)(min(s_first + u, S - 1) * R + tl), k);
+    for (int u = 0; u < U; u++) {
+      const uint32_t pos = (uint32_t)(min(s_first + u, S - 1) * R + tl);
+      load_rec<NV, VISC, NS>(set[u], rs, rec, n8, pos, xpad ? (uint32_t)(S * R) * 8u : pos * 8u, k);
+    }
   };
   auto run_set = [&](const Rec<NV, NS>* set, int s_first) {
 #pragma unroll
