@@ -295,15 +295,53 @@ __device__ __forceinline__ void side_term(double* d, const Rec<NV, NS>& p, int h
   }
 }
 
-// Barrier between the waves of one block inside a sweep.  Only LDS data
-// (the carries in xfer) crosses waves here, so wait for this wave's LDS
-// operations (lgkmcnt(0)) and not for its global loads: __syncthreads() would
-// also drain vmcnt and with it the prefetch ring.  s_waitcnt encoding
-// (gfx9): vmcnt = 63, expcnt = 7, lgkmcnt = 0.
-__device__ __forceinline__ void lds_barrier() {
-  __atomic_signal_fence(__ATOMIC_SEQ_CST);
-  __builtin_amdgcn_s_waitcnt(0xC07F);
-  __builtin_amdgcn_s_barrier();
+// The waves of a block as a pipeline.  Wave w needs, at step s, the result
+// of the last row of wave w-1 at step s-1 and nothing else from other waves,
+// so waves do not meet at a barrier per step: wave w-1 puts its last row's
+// result of step s into slot s % RING of an LDS ring and then publishes
+// prog = s + 1; wave w waits for prog >= s, reads slot (s-1) % RING and
+// publishes cons = s (the input of step s taken).  Before wave w-1 reuses a
+// slot it waits for cons >= s - RING + 1.  Neither wait can block forever:
+// wave w-1 waits only while wave w is RING steps behind, and then wave w does
+// not wait (prog is ahead of it), so wave w advances; wave 0 never waits for
+// input.  Every wave runs the same fixed number of steps, so all waits end.
+// LDS operations of one wave execute in issue order, so a published counter
+// is never seen before the ring data written ahead of it, and a slot is never
+// rewritten before the read that cons acknowledges; compiler fences keep
+// that order in the instruction stream.  The spins read LDS only and never
+// wait for the global-load prefetch.
+//
+// Both sides read ahead, so that no LDS latency sits on a step's dependent
+// path: during step s the consumer reads prog and then, behind it, the slot
+// holding its input of step s+1; at step s+1 the data is used if that prog
+// showed the producer past step s (the data read executed after the prog
+// read, so it saw the slot written), and read again after a wait otherwise.
+// The producer likewise reads cons one step ahead; a stale value only makes
+// it wait longer.
+constexpr int RING = 8;
+template <int NV>
+struct Chain {
+  double ring[15][RING][2 * NV];  // wave w -> w+1: (dU, U + dU) of its last row
+  int prog[16];                   // steps finished by wave w
+  int cons[16];                   // steps whose input wave w has read
+};
+
+// per-wave read-ahead state: the next input (lane 0's) and the counters
+// seen when it was read
+template <int NV>
+struct Ahead {
+  double du[NV], W[NV];
+  int prog, cons;
+};
+
+__device__ __forceinline__ int lds_get(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void lds_put(int* p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void wait_at_least(const int* p, int v) {
+  while (lds_get(p) < v) __builtin_amdgcn_s_sleep(1);
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
@@ -314,25 +352,43 @@ __device__ __forceinline__ void lds_barrier() {
 template <int NV, bool VISC, bool FWD, int LPC>
 __device__ __forceinline__ void sweep_step(const Rec<NV, 3 - LPC>& p, Out<NV>& own, __amdgpu_buffer_rsrc_t rs,
                                            uint32_t dst, uint32_t n8, int s, int S, int R, int ncol, int nrow, int t,
-                                           int k, int lane, int wave, bool multi, double g,
-                                           double (*xfer)[16][2 * NV]) {
+                                           int k, int lane, int wave, int nw, double g, Chain<NV>& ch,
+                                           Ahead<NV>& ah) {
   // the j-neighbour's result: row t-1's, one lane down (the last lane of the
-  // previous wave through LDS); the i-neighbour's is the own row's
+  // previous wave through the LDS ring); the i-neighbour's is the own row's
   Out<NV> up;
 #pragma unroll
   for (int m = 0; m < NV; m++) {
     up.du[m] = shr1(own.du[m]);
     up.W[m] = shr1(own.W[m]);
   }
-  if (multi && wave > 0) {
-    // every lane reads the previous wave's last row (an LDS broadcast),
-    // lane 0 keeps it
-    const double* x = xfer[(s + 1) & 1][wave - 1];
+  if (wave > 0) {
+    // the previous wave's last row at step s-1 (an LDS broadcast), lane 0
+    // keeps it; read one step ahead unless the producer was not there yet
+    if (ah.prog < s) {
+      wait_at_least(&ch.prog[wave - 1], s);
+      const double* x = ch.ring[wave - 1][(s - 1) & (RING - 1)];
+#pragma unroll
+      for (int m = 0; m < NV; m++) {
+        ah.du[m] = x[m];
+        ah.W[m] = x[NV + m];
+      }
+    }
 #pragma unroll
     for (int m = 0; m < NV; m++) {
-      const double xd = x[m], xw = x[NV + m];
-      up.du[m] = lane == 0 ? xd : up.du[m];
-      up.W[m] = lane == 0 ? xw : up.W[m];
+      up.du[m] = lane == 0 ? ah.du[m] : up.du[m];
+      up.W[m] = lane == 0 ? ah.W[m] : up.W[m];
+    }
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    lds_put(&ch.cons[wave], s);
+    // the input of step s+1: prog first, the slot behind it
+    ah.prog = lds_get(&ch.prog[wave - 1]);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    const double* x = ch.ring[wave - 1][s & (RING - 1)];
+#pragma unroll
+    for (int m = 0; m < NV; m++) {
+      ah.du[m] = x[m];
+      ah.W[m] = x[NV + m];
     }
   }
   const int c = s - t;
@@ -378,13 +434,17 @@ __device__ __forceinline__ void sweep_step(const Rec<NV, 3 - LPC>& p, Out<NV>& o
 #pragma unroll
     for (int m = 0; m < NV; m++) bstore(o.du[m], rs, w * 8u, dst + (uint32_t)m * n8);
   }
-  if (multi) {
+  if (wave < nw - 1) {
+    // slot s % RING is free once the next wave has read step s - RING's result
+    if (ah.cons < s - RING + 1) wait_at_least(&ch.cons[wave + 1], s - RING + 1);
     if (lane == 63) {
-      double* x = xfer[s & 1][wave];
+      double* x = ch.ring[wave][s & (RING - 1)];
 #pragma unroll
       for (int m = 0; m < NV; m++) { x[m] = own.du[m]; x[NV + m] = own.W[m]; }
     }
-    lds_barrier();
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    lds_put(&ch.prog[wave], s + 1);
+    ah.cons = lds_get(&ch.cons[wave + 1]);
   }
 }
 
@@ -402,8 +462,9 @@ __device__ __forceinline__ void sweep_step(const Rec<NV, 3 - LPC>& p, Out<NV>& o
 // backward record of the same cell, for the backward sweep the output fields.
 template <int NV, bool VISC, bool FWD, int U, int LPC>
 __device__ __forceinline__ void sweep(__amdgpu_buffer_rsrc_t rs, uint32_t rec, uint32_t dst, uint32_t n8, int S, int R,
-                                      int ncol, int nrow, int t, int k, int lane, int wave, bool multi, double g,
-                                      double (*xfer)[16][2 * NV]) {
+                                      int ncol, int nrow, int t, int k, int lane, int wave, int nw, double g,
+                                      Chain<NV>& ch) {
+  Ahead<NV> ah = {};  // prog = 0: step 0 needs nothing (its input is never used)
   const int npad = (S + 2 * U - 1) / (2 * U) * (2 * U);
   const int tl = min(t, R - 1);  // loads of lanes beyond the block stay in its records
   constexpr int NS = 3 - LPC;
@@ -422,7 +483,7 @@ __device__ __forceinline__ void sweep(__amdgpu_buffer_rsrc_t rs, uint32_t rec, u
 #pragma unroll
     for (int u = 0; u < U; u++)
       sweep_step<NV, VISC, FWD, LPC>(set[u], own, rs, dst, n8, s_first + u, S, R, ncol, nrow, t, k, lane, wave,
-                                     multi, g, xfer);
+                                     nw, g, ch, ah);
   };
   load_set(A, 0);
   for (int s0 = 0; s0 < npad; s0 += 2 * U) {
@@ -441,7 +502,7 @@ __device__ __forceinline__ void sweep(__amdgpu_buffer_rsrc_t rs, uint32_t rec, u
 template <int NV, bool VISC, int U, int MAXT, int LPC>
 __global__ __launch_bounds__(MAXT) void k_lusgs_sweep(LusgsArgs a) {
   using F = RecF<NV, VISC>;
-  __shared__ double xfer[2][16][2 * NV];
+  __shared__ Chain<NV> ch[2];  // one per sweep
   const BlockDesc B = a.blocks[a.block_list[blockIdx.x]];
   const double g = a.cps[B.case_id].gamma;
   const int ncol = B.nx - 1, nrow = B.ny - 1;
@@ -449,15 +510,23 @@ __global__ __launch_bounds__(MAXT) void k_lusgs_sweep(LusgsArgs a) {
   const uint32_t n8 = (uint32_t)B.sk_n * 8u;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int t = LPC == 2 ? tid >> 1 : tid, k = LPC == 2 ? tid & 1 : 0;
-  const bool multi = R > 64 / LPC;
+  // waves holding rows of this block (R is a multiple of 64 / LPC); the
+  // launch may carry more for the level's tallest block, they sit out
+  const int nw = R * LPC / 64;
+  if (tid < 16) {
+    ch[0].prog[tid] = ch[0].cons[tid] = ch[1].prog[tid] = ch[1].cons[tid] = 0;
+  }
+  __syncthreads();
   // the block's record region: forward set, backward set, output fields
   const uint32_t bytes = (uint32_t)((2 * F::NREC + NV) * B.sk_n * 8);
   const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(a.sk + B.sk_base, 0, bytes, 0x00020000);
   const uint32_t fw = 0, bw = F::NREC * n8, out = 2 * F::NREC * n8;
-  sweep<NV, VISC, true, U, LPC>(rs, fw, bw + F::X * n8, n8, S, R, ncol, nrow, t, k, lane, wave, multi, g, xfer);
+  if (wave < nw)
+    sweep<NV, VISC, true, U, LPC>(rs, fw, bw + F::X * n8, n8, S, R, ncol, nrow, t, k, lane, wave, nw, g, ch[0]);
   __threadfence();
   __syncthreads();
-  sweep<NV, VISC, false, U, LPC>(rs, bw, out, n8, S, R, ncol, nrow, t, k, lane, wave, multi, g, xfer);
+  if (wave < nw)
+    sweep<NV, VISC, false, U, LPC>(rs, bw, out, n8, S, R, ncol, nrow, t, k, lane, wave, nw, g, ch[1]);
 }
 
 // ----------------------------------------------------------------- scatter
