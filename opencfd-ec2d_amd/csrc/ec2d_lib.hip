@@ -134,7 +134,7 @@ int lusgs_layout(DLevel& L, int lpc, bool upload) {
   if (threads > 1024) return fail("blocks with more than 1024 rows of cells are not supported by the LU-SGS kernel");
   const int nv = L.o.nvar;
   const bool visc = R.cases[0].cp.If_viscous == 1;
-  const int64_t per = 2 * lusgs_rec_doubles(nv, visc) + nv;
+  const int64_t per = 2 * lusgs_rec_doubles(nv, visc) + lusgs_out_doubles(nv);
   int64_t sk_off = 0;
   L.sk_max_n = 0;
   L.max_cells = 0;

@@ -36,23 +36,25 @@
 // dependent operations instead of the eleven of a full division.
 //
 // Record layout ("skewed").  A block's records are stored hyperplane by
-// hyperplane: shared field f of the cell that row t handles at step s sits at
-// f*n + s*R + t, with R the row count rounded up to 32 (the rows of one wave)
-// and n = S*R for S = ncol + nrow - 1 steps; per-side fields are interleaved
-// by side.  Every load of a step is one contiguous run per wave (buffer loads
-// with one lane offset and a scalar offset per field), so the sweep is bound
-// by its dependent arithmetic, not by address divergence.  The forward record
-// of (s, t) belongs to cell (c+1, t+1), the backward record to cell
+// hyperplane: the record of the cell that row t handles at step s sits at
+// position s*R + t, with R the row count rounded up to the rows of one wave,
+// S = ncol + nrow - 1 steps and n = S*R + 64 positions per field.  Fields go
+// in pairs of doubles, so that every load and store moves 16 bytes per lane:
+// the vector memory address path costs the same per lane for 8 and for 16
+// bytes, and with 8-byte loads it, not the arithmetic, bounds a block of four
+// waves.  Every load of a step is one contiguous run per wave (buffer loads
+// with one lane offset and a scalar offset per pair).  The forward record of
+// (s, t) belongs to cell (c+1, t+1), the backward record to cell
 // (ncol-c, nrow-t): both sweeps use the same neighbour tests (c != 0 for the
 // i-neighbour, t != 0 for the j-neighbour) and the same index arithmetic.
 //
-// Per block the records occupy (2*NREC + NV) * n doubles: the forward set,
-// the backward set (whose X fields receive the forward dU from the forward
-// sweep), and NV output fields for the backward dU.  The field stride n is
-// S*R + 64: the last 64 slots of every field are a trash pad that lanes
-// without a cell to store write to, so the sweep's stores are unconditional
-// (a store under a branch makes the compiler drain every outstanding load,
-// the prefetch ring included, before the next use of a loaded register).
+// Per block the records occupy (2*NREC + 2*XP) * n doubles: the forward set,
+// the backward set (whose X pairs receive the forward dU from the forward
+// sweep), and XP output pairs for the backward dU.  The last 64 positions of
+// every field are a trash pad that lanes without a cell to store write to,
+// so the sweep's stores are unconditional (a store under a branch makes the
+// compiler drain every outstanding load, the prefetch ring included, before
+// the next use of a loaded register).
 #include "ec2d_device.h"
 #include "ec2d_physics.h"
 
@@ -71,25 +73,34 @@ struct LusgsArgs {
 
 // Field layout of one record set (one sweep direction) of a block.  A
 // record has a part shared by the two lanes of a cell and a part per side
-// (k = 0: the j-neighbour, k = 1: the i-neighbour), interleaved by side so
-// that the two lanes of a cell read adjacent doubles:
-//   shared field f of position pos:  f*n + pos
-//   side field g of position pos:    NSH*n + g*2n + 2*pos + k
+// (k = 0: the j-neighbour, k = 1: the i-neighbour).  Both are sequences of
+// doubles stored in pairs; in doubles from the set's start:
+//   shared double d of position pos:   (d/2)*2n + 2*pos + d%2
+//   side double e of (pos, k):         NSH*n + (e/2)*4n + 2*(2*pos + k) + e%2
+// so that a wave's pair loads are contiguous and the two lanes of a cell
+// read adjacent pairs of their sides.
 template <int NV, bool VISC>
 struct RecF {
-  // shared: Res (forward) or the forward dU (backward), U, the diagonal and
-  // its correctly rounded reciprocal (a separate one for the SA equation)
-  static constexpr int X = 0, U = NV, AL0 = 2 * NV, AL4 = 2 * NV + 1, Y0 = 2 * NV + 2, Y4 = 2 * NV + 3;
-  static constexpr int NSH = 2 * NV + 4;
+  // shared: X = Res (forward) or the forward dU (backward), in whole pairs
+  // of their own (the forward pair lanes read them from another position);
+  // then U, the diagonal and its correctly rounded reciprocal (a separate one
+  // for the SA equation)
+  static constexpr int XP = (NV + 1) / 2;
+  static constexpr int U = 2 * XP, AL0 = U + NV, Y0 = AL0 + 1, AL4 = AL0 + 2, Y4 = AL0 + 3;
+  static constexpr int NSH = 2 * XP + (NV + 2 + (NV == 5 ? 2 : 0) + 1) / 2 * 2;
   // per side: F(U_nb).n, the normal, the face area, the neighbour's inviscid
-  // and viscous spectral radius
+  // and viscous spectral radius (padded to whole pairs)
   static constexpr int F = 0, N1 = NV, N2 = NV + 1, A = NV + 2, LC = NV + 3, LV = NV + 4;
-  static constexpr int NSD = NV + 4 + (VISC ? 1 : 0);
+  static constexpr int NSD = (NV + 4 + (VISC ? 1 : 0) + 1) / 2 * 2;
   static constexpr int NREC = NSH + 2 * NSD;  // doubles per position
 };
 
-// host-side record size in doubles per position, equal to RecF<nv, visc>::NREC
-inline int lusgs_rec_doubles(int nv, bool visc) { return 2 * nv + 4 + 2 * (nv + 4 + (visc ? 1 : 0)); }
+// host side: doubles per position of one record set (RecF<nv, visc>::NREC)
+// and of the output pairs
+inline int lusgs_rec_doubles(int nv, bool visc) {
+  return 2 * ((nv + 1) / 2) + (nv + 2 + (nv == 5 ? 2 : 0) + 1) / 2 * 2 + 2 * ((nv + 4 + (visc ? 1 : 0) + 1) / 2 * 2);
+}
+inline int lusgs_out_doubles(int nv) { return 2 * ((nv + 1) / 2); }
 
 // operands of one lane for one cell: the shared part and NS sides (NS = 1:
 // the lane's own side k, NS = 2: side 0 (j) and side 1 (i))
@@ -131,49 +142,69 @@ __device__ __forceinline__ double div_rcp(double a, double b, double y) {
   return __builtin_fma(r, y, q);
 }
 
-__device__ __forceinline__ double bload(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
-  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+struct D2 {
+  double x, y;
+};
+
+// one pair of doubles (16 bytes) at voff + soff
+__device__ __forceinline__ D2 bload2(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  return __builtin_bit_cast(D2, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
 }
 
-__device__ __forceinline__ void bstore(double v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
-  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, v), r, voff,
-                                        soff, 0);
+__device__ __forceinline__ void bstore2(double x, double y, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  const D2 v = {x, y};
+  __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, v), r, voff, soff, 0);
 }
 
-// base: byte offset of the record set in the block's record region; NS = 1
-// loads side k, NS = 2 both sides
-// vx: byte offset of the X loads (the lane's position, or a trash-pad slot)
+// v16: byte offset 16*pos of the lane's position, vx16 that of its X pairs
+// (its own, or a trash-pad slot); the field offsets are scalar and fixed.
+// base: byte offset of the record set in the block's record region, n16 =
+// 16n (bytes of one pair field); NS = 1 loads side k, NS = 2 both sides.
 template <int NV, bool VISC, int NS>
-__device__ __forceinline__ void load_rec(Rec<NV, NS>& p, __amdgpu_buffer_rsrc_t r, uint32_t base, uint32_t n8,
-                                         uint32_t pos, uint32_t vx, int k) {
+__device__ __forceinline__ void load_rec(Rec<NV, NS>& p, __amdgpu_buffer_rsrc_t r, uint32_t base, uint32_t n16,
+                                         uint32_t v16, uint32_t vx16, int k) {
   using F = RecF<NV, VISC>;
-  const uint32_t vs = pos * 8u;
-  const uint32_t sb = base + F::NSH * n8;  // side part
+  double sh[F::NSH];
+#pragma unroll
+  for (int j = 0; j < F::NSH / 2; j++) {
+    const D2 v = bload2(r, j < F::XP ? vx16 : v16, base + (uint32_t)j * n16);
+    sh[2 * j] = v.x;
+    sh[2 * j + 1] = v.y;
+  }
 #pragma unroll
   for (int m = 0; m < NV; m++) {
-    p.X[m] = bload(r, vx, base + (F::X + m) * n8);
-    p.U[m] = bload(r, vs, base + (F::U + m) * n8);
+    p.X[m] = sh[m];
+    p.U[m] = sh[F::U + m];
   }
-#pragma unroll
-  for (int h = 0; h < NS; h++) {
-    const uint32_t vd = (2u * pos + (uint32_t)(NS == 1 ? k : h)) * 8u;
-#pragma unroll
-    for (int m = 0; m < NV; m++) p.F[h][m] = bload(r, vd, sb + (F::F + m) * 2u * n8);
-    p.n1[h] = bload(r, vd, sb + F::N1 * 2u * n8);
-    p.n2[h] = bload(r, vd, sb + F::N2 * 2u * n8);
-    p.a[h] = bload(r, vd, sb + F::A * 2u * n8);
-    p.Lc[h] = bload(r, vd, sb + F::LC * 2u * n8);
-    if constexpr (VISC) p.Lv[h] = bload(r, vd, sb + F::LV * 2u * n8);
-    else p.Lv[h] = 0.0;
-  }
-  p.al0 = bload(r, vs, base + F::AL0 * n8);
-  p.y0 = bload(r, vs, base + F::Y0 * n8);
+  p.al0 = sh[F::AL0];
+  p.y0 = sh[F::Y0];
   if constexpr (NV == 5) {
-    p.al4 = bload(r, vs, base + F::AL4 * n8);
-    p.y4 = bload(r, vs, base + F::Y4 * n8);
+    p.al4 = sh[F::AL4];
+    p.y4 = sh[F::Y4];
   } else {
     p.al4 = p.al0;
     p.y4 = p.y0;
+  }
+  const uint32_t sb = base + (uint32_t)(F::NSH / 2) * n16;  // side part, at 2*pos + k
+#pragma unroll
+  for (int h = 0; h < NS; h++) {
+    const uint32_t vd = 2u * v16 + (uint32_t)(NS == 1 ? k : h) * 16u;
+    double sd[F::NSD];
+#pragma unroll
+    for (int j = 0; j < F::NSD / 2; j++) {
+      const D2 v = bload2(r, vd, sb + (uint32_t)j * 2u * n16);
+      sd[2 * j] = v.x;
+      sd[2 * j + 1] = v.y;
+    }
+#pragma unroll
+    for (int m = 0; m < NV; m++) p.F[h][m] = sd[F::F + m];
+    p.n1[h] = sd[F::N1];
+    p.n2[h] = sd[F::N2];
+    p.a[h] = sd[F::A];
+    p.Lc[h] = sd[F::LC];
+    if constexpr (VISC) p.Lv[h] = sd[F::LV];
+    else p.Lv[h] = 0.0;
   }
 }
 
@@ -188,15 +219,18 @@ __device__ __forceinline__ void write_rec(double* __restrict__ rb, int64_t n, in
                                           double alfa1) {
   using F = RecF<NV, VISC>;
   const int ncol = B.nx - 1, nrow = B.ny - 1;
-  double* sh = rb + pos;                        // shared field f: sh[f*n]
-  double* sd = rb + F::NSH * n + 2 * pos;       // side field g, side k: sd[g*2n + k]
+  // shared double d, side double e of side k (layout of RecF)
+  auto SH = [&](int d) -> double& { return rb[(int64_t)(d / 2) * 2 * n + 2 * pos + (d & 1)]; };
+  auto SD = [&](int k, int e) -> double& {
+    return rb[(int64_t)F::NSH * n + (int64_t)(e / 2) * 4 * n + 2 * (2 * pos + k) + (e & 1)];
+  };
   auto PL = [&](int slot) { return base + (int64_t)slot * B.plane; };
   if (r >= nrow || c < 0 || c >= ncol) {
     // padding position: never active in a sweep
 #pragma unroll
-    for (int f = 0; f < F::NSH; f++) sh[f * n] = 0.0;
+    for (int d = 0; d < F::NSH; d++) SH(d) = 0.0;
 #pragma unroll
-    for (int f = 0; f < F::NSD; f++) sd[f * 2 * n] = sd[f * 2 * n + 1] = 0.0;
+    for (int e = 0; e < F::NSD; e++) SD(0, e) = SD(1, e) = 0.0;
     return;
   }
   const int i = FWD ? c + 1 : ncol - c;
@@ -205,8 +239,8 @@ __device__ __forceinline__ void write_rec(double* __restrict__ rb, int64_t n, in
   const int64_t q = pidx(B, i, j);
 #pragma unroll
   for (int m = 0; m < NV; m++) {
-    sh[(F::U + m) * n] = PL(pm.U + m)[q];
-    if (FWD) sh[(F::X + m) * n] = PL(pm.Res + m)[q];
+    SH(F::U + m) = PL(pm.U + m)[q];
+    if (FWD) SH(m) = PL(pm.Res + m)[q];
   }
   const double* Lci = PL(pm.Lci);
   const double* Lcj = PL(pm.Lcj);
@@ -216,21 +250,20 @@ __device__ __forceinline__ void write_rec(double* __restrict__ rb, int64_t n, in
   const double vol = PL(pm.vol)[q], dt = PL(pm.dt)[q];
   double alfa = vol / dt + 1.0 * (Lci[q] + Lcj[q]) + vol * alfa1;
   if constexpr (VISC) alfa = alfa + 2.0 * (Lvi[q] + Lvj[q]);
-  sh[F::AL0 * n] = alfa;
-  sh[F::Y0 * n] = 1.0 / alfa;
+  SH(F::AL0) = alfa;
+  SH(F::Y0) = 1.0 / alfa;
   if constexpr (NV == 5) {
     const double alfa4 = alfa + Lvi[q] + Lvj[q];
-    sh[F::AL4 * n] = alfa4;
-    sh[F::Y4 * n] = 1.0 / alfa4;
+    SH(F::AL4) = alfa4;
+    SH(F::Y4) = 1.0 / alfa4;
   }
   // sides: k = 0 the j-neighbour, k = 1 the i-neighbour
 #pragma unroll
   for (int k = 0; k < 2; k++) {
     const bool has = k == 0 ? r != 0 : c != 0;
-    double* o = sd + k;
     if (!has) {
 #pragma unroll
-      for (int f = 0; f < F::NSD; f++) o[f * 2 * n] = 0.0;
+      for (int e = 0; e < F::NSD; e++) SD(k, e) = 0.0;
       continue;
     }
     const int64_t qn = k == 0 ? pidx(B, i, j + d) : pidx(B, i + d, j);
@@ -240,13 +273,13 @@ __device__ __forceinline__ void write_rec(double* __restrict__ rb, int64_t n, in
     for (int m = 0; m < NV; m++) Un[m] = PL(pm.U + m)[qn];
     flux_n_old<NV>(Fo, Un, n1, n2, g);
 #pragma unroll
-    for (int m = 0; m < NV; m++) o[(F::F + m) * 2 * n] = Fo[m];
-    o[F::N1 * 2 * n] = n1;
-    o[F::N2 * 2 * n] = n2;
+    for (int m = 0; m < NV; m++) SD(k, F::F + m) = Fo[m];
+    SD(k, F::N1) = n1;
+    SD(k, F::N2) = n2;
     const double* sa = PL(k == 0 ? pm.sj : pm.si);
-    o[F::A * 2 * n] = FWD ? sa[q] : sa[qn];
-    o[F::LC * 2 * n] = (k == 0 ? Lcj : Lci)[qn];
-    if constexpr (VISC) o[F::LV * 2 * n] = (k == 0 ? Lvj : Lvi)[qn];
+    SD(k, F::A) = FWD ? sa[q] : sa[qn];
+    SD(k, F::LC) = (k == 0 ? Lcj : Lci)[qn];
+    if constexpr (VISC) SD(k, F::LV) = (k == 0 ? Lvj : Lvi)[qn];
   }
 }
 
@@ -264,15 +297,16 @@ __global__ __launch_bounds__(256) void k_lusgs_prep(LusgsArgs a) {
   write_rec<NV, VISC, true>(fw, n, pos, B, base, a.pm, s - r, r, g, a.alfa1);
   write_rec<NV, VISC, false>(fw + F::NREC * n, n, pos, B, base, a.pm, s - r, r, g, a.alfa1);
   if (r >= B.ny - 1 || s - r < 0 || s - r >= B.nx - 1) {
-    // the backward X fields of padding positions (valid ones come from the forward sweep)
+    // the backward X pairs of padding positions (valid ones come from the forward sweep)
 #pragma unroll
-    for (int m = 0; m < NV; m++) fw[F::NREC * n + (F::X + m) * n + pos] = 0.0;
+    for (int d = 0; d < 2 * F::XP; d++) fw[F::NREC * n + (int64_t)(d / 2) * 2 * n + 2 * pos + (d & 1)] = 0.0;
   }
   if (pos == 0) {
-    // -0.0 in the first trash-pad slot of the forward X fields: the X operand
+    // -0.0 in the first trash-pad slot of the forward X pairs: the X operand
     // of the j-neighbour lane of a pair (no sweep stores there)
+    const int64_t pad = (int64_t)B.sk_S * B.sk_R;
 #pragma unroll
-    for (int m = 0; m < NV; m++) fw[(F::X + m) * n + (int64_t)B.sk_S * B.sk_R] = -0.0;
+    for (int d = 0; d < 2 * F::XP; d++) fw[(int64_t)(d / 2) * 2 * n + 2 * pad + (d & 1)] = -0.0;
   }
 }
 
@@ -351,7 +385,7 @@ __device__ __forceinline__ void wait_at_least(const int* p, int v) {
 // computes both.
 template <int NV, bool VISC, bool FWD, int LPC>
 __device__ __forceinline__ void sweep_step(const Rec<NV, 3 - LPC>& p, Out<NV>& own, __amdgpu_buffer_rsrc_t rs,
-                                           uint32_t dst, uint32_t n8, int s, int S, int R, int ncol, int nrow, int t,
+                                           uint32_t dst, uint32_t n16, int s, int S, int R, int ncol, int nrow, int t,
                                            int k, int lane, int wave, int nw, double g, Chain<NV>& ch,
                                            Ahead<NV>& ah) {
   // the j-neighbour's result: row t-1's, one lane down (the last lane of the
@@ -432,7 +466,8 @@ __device__ __forceinline__ void sweep_step(const Rec<NV, 3 - LPC>& p, Out<NV>& o
     const uint32_t w = (act && k == 0) ? (FWD ? (uint32_t)((S - 1 - s) * R + (nrow - 1 - t)) : (uint32_t)(s * R + t))
                                        : (uint32_t)(S * R + lane);
 #pragma unroll
-    for (int m = 0; m < NV; m++) bstore(o.du[m], rs, w * 8u, dst + (uint32_t)m * n8);
+    for (int j = 0; j < (NV + 1) / 2; j++)
+      bstore2(o.du[2 * j], 2 * j + 1 < NV ? o.du[2 * j + 1] : 0.0, rs, w * 16u, dst + (uint32_t)j * n16);
   }
   if (wave < nw - 1) {
     // slot s % RING is free once the next wave has read step s - RING's result
@@ -461,7 +496,7 @@ __device__ __forceinline__ void sweep_step(const Rec<NV, 3 - LPC>& p, Out<NV>& o
 // not yet needed.  Loads past the last step are clamped to it.  The cell's dU goes to dst: for the forward sweep the X field of the
 // backward record of the same cell, for the backward sweep the output fields.
 template <int NV, bool VISC, bool FWD, int U, int LPC>
-__device__ __forceinline__ void sweep(__amdgpu_buffer_rsrc_t rs, uint32_t rec, uint32_t dst, uint32_t n8, int S, int R,
+__device__ __forceinline__ void sweep(__amdgpu_buffer_rsrc_t rs, uint32_t rec, uint32_t dst, uint32_t n16, int S, int R,
                                       int ncol, int nrow, int t, int k, int lane, int wave, int nw, double g,
                                       Chain<NV>& ch) {
   Ahead<NV> ah = {};  // prog = 0: step 0 needs nothing (its input is never used)
@@ -472,17 +507,18 @@ __device__ __forceinline__ void sweep(__amdgpu_buffer_rsrc_t rs, uint32_t rec, u
   Rec<NV, NS> A[U], B[U];
   // forward pairs: lane 2t reads X (-0.0) from the first trash-pad slot
   const bool xpad = FWD && LPC == 2 && k == 0;
+  const uint32_t pad16 = (uint32_t)(S * R) * 16u;
   auto load_set = [&](Rec<NV, NS>* set, int s_first) {
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      const uint32_t pos = (uint32_t)(min(s_first + u, S - 1) * R + tl);
-      load_rec<NV, VISC, NS>(set[u], rs, rec, n8, pos, xpad ? (uint32_t)(S * R) * 8u : pos * 8u, k);
+      const uint32_t v16 = (uint32_t)(min(s_first + u, S - 1) * R + tl) * 16u;
+      load_rec<NV, VISC, NS>(set[u], rs, rec, n16, v16, xpad ? pad16 : v16, k);
     }
   };
   auto run_set = [&](const Rec<NV, NS>* set, int s_first) {
 #pragma unroll
     for (int u = 0; u < U; u++)
-      sweep_step<NV, VISC, FWD, LPC>(set[u], own, rs, dst, n8, s_first + u, S, R, ncol, nrow, t, k, lane, wave,
+      sweep_step<NV, VISC, FWD, LPC>(set[u], own, rs, dst, n16, s_first + u, S, R, ncol, nrow, t, k, lane, wave,
                                      nw, g, ch, ah);
   };
   load_set(A, 0);
@@ -507,7 +543,7 @@ __global__ __launch_bounds__(MAXT) void k_lusgs_sweep(LusgsArgs a) {
   const double g = a.cps[B.case_id].gamma;
   const int ncol = B.nx - 1, nrow = B.ny - 1;
   const int S = B.sk_S, R = B.sk_R;
-  const uint32_t n8 = (uint32_t)B.sk_n * 8u;
+  const uint32_t n8 = (uint32_t)B.sk_n * 8u, n16 = 2u * n8;  // bytes of a field of doubles, of pairs
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int t = LPC == 2 ? tid >> 1 : tid, k = LPC == 2 ? tid & 1 : 0;
   // waves holding rows of this block (R is a multiple of 64 / LPC); the
@@ -517,16 +553,15 @@ __global__ __launch_bounds__(MAXT) void k_lusgs_sweep(LusgsArgs a) {
     ch[0].prog[tid] = ch[0].cons[tid] = ch[1].prog[tid] = ch[1].cons[tid] = 0;
   }
   __syncthreads();
-  // the block's record region: forward set, backward set, output fields
-  const uint32_t bytes = (uint32_t)((2 * F::NREC + NV) * B.sk_n * 8);
+  // the block's record region: forward set, backward set (its X pairs first),
+  // output pairs
+  const uint32_t bytes = (uint32_t)((2 * F::NREC + 2 * F::XP) * B.sk_n * 8);
   const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(a.sk + B.sk_base, 0, bytes, 0x00020000);
   const uint32_t fw = 0, bw = F::NREC * n8, out = 2 * F::NREC * n8;
-  if (wave < nw)
-    sweep<NV, VISC, true, U, LPC>(rs, fw, bw + F::X * n8, n8, S, R, ncol, nrow, t, k, lane, wave, nw, g, ch[0]);
+  if (wave < nw) sweep<NV, VISC, true, U, LPC>(rs, fw, bw, n16, S, R, ncol, nrow, t, k, lane, wave, nw, g, ch[0]);
   __threadfence();
   __syncthreads();
-  if (wave < nw)
-    sweep<NV, VISC, false, U, LPC>(rs, bw, out, n8, S, R, ncol, nrow, t, k, lane, wave, nw, g, ch[1]);
+  if (wave < nw) sweep<NV, VISC, false, U, LPC>(rs, bw, out, n16, S, R, ncol, nrow, t, k, lane, wave, nw, g, ch[1]);
 }
 
 // ----------------------------------------------------------------- scatter
@@ -539,12 +574,14 @@ __global__ __launch_bounds__(256) void k_lusgs_scatter(LusgsArgs a) {
   if (cidx >= ncol * nrow) return;
   const int j = cidx / ncol + 1, i = cidx - (j - 1) * ncol + 1;
   const int64_t n = B.sk_n;
-  const double* out = a.sk + B.sk_base + 2 * F::NREC * n + (int64_t)(ncol - i + nrow - j) * B.sk_R + (nrow - j);
+  // the backward record of cell (i, j): position (ncol-i + nrow-j) * R + nrow-j
+  const int64_t pos = (int64_t)(ncol - i + nrow - j) * B.sk_R + (nrow - j);
+  const double* out = a.sk + B.sk_base + 2 * F::NREC * n + 2 * pos;
   double* base = a.arena + B.base;
   const int64_t q = pidx(B, i, j);
 #pragma unroll
   for (int m = 0; m < NV; m++) {
-    const double du = out[m * n];
+    const double du = out[(int64_t)(m / 2) * 2 * n + (m & 1)];
     base[(int64_t)(a.pm.dU + m) * B.plane + q] = du;
     if (a.update) {
       double* U = base + (int64_t)(a.pm.U + m) * B.plane;
