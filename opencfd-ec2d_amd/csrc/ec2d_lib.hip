@@ -406,7 +406,8 @@ int launch_lusgs(int level, const int* dlist, int nblk, double alfa1, int update
   a.alfa1 = alfa1;
   a.update = update;
   bool visc = R.cases[0].cp.If_viscous == 1;
-  dim3 gprep((unsigned)((L.sk_max_n + 255) / 256), nblk), gsweep(nblk), blk(L.lusgs_threads);
+  // prep grid: a multiple of 8 workgroups (k_lusgs_prep spreads position ranges over the XCDs)
+  dim3 gprep((unsigned)(((L.sk_max_n + 255) / 256 + 7) / 8 * 8), nblk), gsweep(nblk), blk(L.lusgs_threads);
   dim3 gscat((unsigned)((L.max_cells + 255) / 256), nblk);
   int nv = L.o.nvar;
   // the instantiation is chosen by thread count (its launch bound): up to 4

@@ -283,12 +283,19 @@ __device__ __forceinline__ void write_rec(double* __restrict__ rb, int64_t n, in
   }
 }
 
+// Workgroups are dealt to the 8 XCDs in turn, each XCD with its own L2.  A
+// prep workgroup reads the planes along a diagonal of cells, and the
+// workgroups of the next steps read the same cache lines shifted by one
+// cell, so consecutive position ranges go to one XCD: the grid is a multiple
+// of 8 and workgroup b handles range (b % 8) * (gridDim.x / 8) + b / 8.
 template <int NV, bool VISC>
 __global__ __launch_bounds__(256) void k_lusgs_prep(LusgsArgs a) {
   using F = RecF<NV, VISC>;
   const BlockDesc B = a.blocks[a.block_list[blockIdx.y]];
   const int64_t n = B.sk_n;
-  const int64_t pos = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const unsigned per_xcd = gridDim.x / 8u;
+  const unsigned wg = (blockIdx.x % 8u) * per_xcd + blockIdx.x / 8u;
+  const int64_t pos = (int64_t)wg * 256 + threadIdx.x;
   if (pos >= (int64_t)B.sk_S * B.sk_R) return;
   const int s = (int)(pos / B.sk_R), r = (int)(pos - (int64_t)s * B.sk_R);
   const double g = a.cps[B.case_id].gamma;
