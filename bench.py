@@ -61,6 +61,23 @@ def make_workload(name, d):
                 "cells": 65536, "mach": 0.73, "aoa_deg": 2.0, "reynolds": 6.5e6, "cfl": 5.0,
                 "multigrid_levels": 3}
         return desc
+    if name == "c3":
+        info = cases.write_vortex_case(d, cases.control_vortex(iters=1000), seed=0)
+        desc = {"workload": "vortex_multiblock_8x8_euler_ausm_muscl3_lusgs", "blocks": 64,
+                "block_widths": info["W"], "block_heights": info["H"], "cells": info["cells"], "mach": 0.5,
+                "cfl": 5.0, "multigrid_levels": 1,
+                "data": "synthetic (seeded 8x8 sheared, perturbed multi-block grid; isentropic vortex + 1% noise)"}
+        return desc
+    if name == "c4":
+        prms = cases.ensemble_params(seed=0, n=256)
+        dirs = []
+        for k, prm in enumerate(prms):
+            dirs.append(cases.write_ensemble_case(os.path.join(d, f"case{k:03d}"), prm))
+        desc = {"workload": "naca_mpxx_ensemble_256_sa_rans_roe_muscl3_lusgs", "cases": len(prms),
+                "grids": "129x49 .. 257x65 (seed 0)", "cells": int(sum((p["nx"] - 1) * (p["ny"] - 1) for p in prms)),
+                "cfl": 5.0, "multigrid_levels": 1, "case_dirs": dirs,
+                "data": "synthetic (seeded NACA mpxx C-grids and flow conditions, freestream start)"}
+        return desc
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -200,12 +217,16 @@ def main():
 
     with tempfile.TemporaryDirectory() as d:
         desc = make_workload(args.workload, d)
-        case = solver.Case.from_dir(d)
+        case_dirs = desc.pop("case_dirs", [d])
+        data = desc.pop("data", "synthetic (generated NACA 0012 C-grid, freestream start)")
+        batch = [solver.Case.from_dir(cd) for cd in case_dirs]
+        case = batch[0]
         nv = case.ctl.nvar
         viscous = case.ctl.If_viscous == 1
         sa = nv == 5
-        with solver.Batch([case], tile_rows=args.tile_rows) as b:
-            cells = b.num_cells(0, 1)
+        with solver.Batch(batch, tile_rows=args.tile_rows) as b:
+            cells = b.num_cells(-1, 1)  # all cases of the batch
+            cells0 = b.num_cells(0, 1)
             b.init_flow()
             # launch shapes per level (residual tile rows, LU-SGS lanes per
             # cell), timed on this case before the measured run
@@ -229,7 +250,10 @@ def main():
                                        for k in b.KERNEL_KINDS}
             b.set_profile(False)
         kind = "port" if desc["multigrid_levels"] > 1 else "reference"
-        cpu = None if args.no_cpu else cpu_baseline(d, cells, kind=kind)
+        # one case of a batch on one core (the cases are independent)
+        cpu = None if args.no_cpu else cpu_baseline(case_dirs[0], cells0, kind=kind)
+        if cpu is not None and len(case_dirs) > 1:
+            cpu["sample"] += f"; case 0 of the {len(case_dirs)}-case batch ({cells0} cells)"
 
     ms_per_step = wall * 1000.0 / args.steps
     value = cells * args.steps / wall
@@ -252,7 +276,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (generated NACA 0012 C-grid, freestream start)",
+        "data": data,
         "config": dict(desc, device_ms_per_step=dev_ms / args.steps,
                        residual_rms_last=[float(v) for v in rrms[:nv]],
                        kernel_ms_per_level=levels, tile_rows=args.tile_rows,

@@ -225,14 +225,12 @@ __device__ __forceinline__ void write_rec(double* __restrict__ rb, int64_t n, in
     return rb[(int64_t)F::NSH * n + (int64_t)(e / 2) * 4 * n + 2 * (2 * pos + k) + (e & 1)];
   };
   auto PL = [&](int slot) { return base + (int64_t)slot * B.plane; };
-  if (r >= nrow || c < 0 || c >= ncol) {
-    // padding position: never active in a sweep
-#pragma unroll
-    for (int d = 0; d < F::NSH; d++) SH(d) = 0.0;
-#pragma unroll
-    for (int e = 0; e < F::NSD; e++) SD(0, e) = SD(1, e) = 0.0;
-    return;
-  }
+  // A padding position is never active in a sweep, and a side without a
+  // neighbour is never used (the sweep selects 0 for it): neither is
+  // written.  They keep the zeros the record arena starts with, which no
+  // sweep overwrites (sweep stores go to the X pairs of cells and the trash
+  // pad), so an inactive lane only ever reads finite values.
+  if (r >= nrow || c < 0 || c >= ncol) return;
   const int i = FWD ? c + 1 : ncol - c;
   const int j = FWD ? r + 1 : nrow - r;
   const int d = FWD ? -1 : 1;
@@ -261,11 +259,7 @@ __device__ __forceinline__ void write_rec(double* __restrict__ rb, int64_t n, in
 #pragma unroll
   for (int k = 0; k < 2; k++) {
     const bool has = k == 0 ? r != 0 : c != 0;
-    if (!has) {
-#pragma unroll
-      for (int e = 0; e < F::NSD; e++) SD(k, e) = 0.0;
-      continue;
-    }
+    if (!has) continue;
     const int64_t qn = k == 0 ? pidx(B, i, j + d) : pidx(B, i + d, j);
     const double n1 = PL(k == 0 ? pm.nj1 : pm.ni1)[q], n2 = PL(k == 0 ? pm.nj2 : pm.ni2)[q];
     double Un[NV], Fo[NV];
@@ -303,11 +297,6 @@ __global__ __launch_bounds__(256) void k_lusgs_prep(LusgsArgs a) {
   double* fw = a.sk + B.sk_base;
   write_rec<NV, VISC, true>(fw, n, pos, B, base, a.pm, s - r, r, g, a.alfa1);
   write_rec<NV, VISC, false>(fw + F::NREC * n, n, pos, B, base, a.pm, s - r, r, g, a.alfa1);
-  if (r >= B.ny - 1 || s - r < 0 || s - r >= B.nx - 1) {
-    // the backward X pairs of padding positions (valid ones come from the forward sweep)
-#pragma unroll
-    for (int d = 0; d < 2 * F::XP; d++) fw[F::NREC * n + (int64_t)(d / 2) * 2 * n + 2 * pos + (d & 1)] = 0.0;
-  }
   if (pos == 0) {
     // -0.0 in the first trash-pad slot of the forward X pairs: the X operand
     // of the j-neighbour lane of a pair (no sweep stores there)

@@ -351,3 +351,30 @@ def ensemble_params(seed: int = 0, n: int = 256):
                         Ma=rng.uniform(0.3, 0.8), aoa=rng.uniform(-2.0, 6.0),
                         Re=10 ** rng.uniform(6.0, 7.0), nx=nx, ny=ny))
     return out
+
+
+def write_vortex_case(dirpath: str, control: F.Control, seed: int = 0, nbx: int = 8, nby: int = 8, wmin: int = 33,
+                      wmax: int = 257, hmin: int = 33, hmax: int = 129, h: float = 1.0 / 64.0, bc_edit=None) -> dict:
+    """Multi-block vortex case (vortex_multiblock) with its initial field as
+    flow2d.dat (the control must have Iflag_init = 1); returns the generator's
+    info (column widths, row heights, cells)."""
+    from . import geometry as G
+
+    xs, ys, bcs, field, info = vortex_multiblock(seed=seed, nbx=nbx, nby=nby, wmin=wmin, wmax=wmax, hmin=hmin,
+                                                 hmax=hmax, h=h)
+    if bc_edit:
+        bc_edit(bcs)
+    lev = G.build_levels(xs, ys, bcs, 1)
+    cx = [b.x1[1:-1, 1:-1] for b in lev[0]]
+    cy = [b.y1[1:-1, 1:-1] for b in lev[0]]
+    write_case(dirpath, control, xs, ys, bcs, flow=(cx, cy, [field(x, y) for x, y in zip(cx, cy)]))
+    return info
+
+
+def write_ensemble_case(dirpath: str, prm: dict, iters: int = 1000, cfl: float = 5.0, ds1: float = 5e-6) -> str:
+    """One SA-RANS airfoil case of the ensemble (a dict of ensemble_params):
+    NACA mpxx C-grid of nx x ny points, Roe + MUSCL3, LU-SGS on one level."""
+    x, y, bc, _ = naca_cgrid(prm["nx"], prm["ny"], m=prm["m"], p=prm["p"], t=prm["t"], ds1=ds1)
+    ctl = control_sa_naca(iters=iters, Ma=prm["Ma"], aoa=prm["aoa"], Re=prm["Re"], cfl=cfl)
+    return write_case(dirpath, ctl, [x], [y], [bc])
+
