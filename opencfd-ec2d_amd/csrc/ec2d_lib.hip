@@ -73,6 +73,7 @@ struct DLevel {
   double* sk = nullptr;       // LU-SGS records (skewed, see ec2d_lusgs.hip)
   int64_t sk_max_n = 0;       // largest record field of a block (doubles)
   int max_cells = 0;          // largest block (cells)
+  int prep_tiles = 0;         // LU-SGS prep workgroups per block: 32 x 8 cell tiles of the largest block
   std::vector<long long> case_cells;
   bool fdm = false;
   int2 *d_ilines = nullptr, *d_jlines = nullptr, *d_frows = nullptr;
@@ -138,6 +139,7 @@ int lusgs_layout(DLevel& L, int lpc, bool upload) {
   int64_t sk_off = 0;
   L.sk_max_n = 0;
   L.max_cells = 0;
+  L.prep_tiles = 0;
   for (auto& d : L.hb) {
     const int ncol = d.nx - 1, nrow = d.ny - 1;
     d.sk_S = ncol + nrow - 1;
@@ -149,6 +151,7 @@ int lusgs_layout(DLevel& L, int lpc, bool upload) {
     sk_off += (per * d.sk_n + 31) / 32 * 32;
     L.sk_max_n = std::max(L.sk_max_n, d.sk_n);
     L.max_cells = std::max(L.max_cells, ncol * nrow);
+    L.prep_tiles = std::max(L.prep_tiles, ((ncol + 31) / 32) * ((nrow + 7) / 8));
   }
   if (L.sk) HIPCHK(hipFree(L.sk));
   L.sk = nullptr;
@@ -406,8 +409,8 @@ int launch_lusgs(int level, const int* dlist, int nblk, double alfa1, int update
   a.alfa1 = alfa1;
   a.update = update;
   bool visc = R.cases[0].cp.If_viscous == 1;
-  // prep grid: a multiple of 8 workgroups (k_lusgs_prep spreads position ranges over the XCDs)
-  dim3 gprep((unsigned)(((L.sk_max_n + 255) / 256 + 7) / 8 * 8), nblk), gsweep(nblk), blk(L.lusgs_threads);
+  // prep grid: tiles of 32 x 8 cells of the largest block
+  dim3 gprep((unsigned)L.prep_tiles, nblk), gsweep(nblk), blk(L.lusgs_threads);
   dim3 gscat((unsigned)((L.max_cells + 255) / 256), nblk);
   int nv = L.o.nvar;
   // the instantiation is chosen by thread count (its launch bound): up to 4

@@ -277,33 +277,38 @@ __device__ __forceinline__ void write_rec(double* __restrict__ rb, int64_t n, in
   }
 }
 
-// Workgroups are dealt to the 8 XCDs in turn, each XCD with its own L2.  A
-// prep workgroup reads the planes along a diagonal of cells, and the
-// workgroups of the next steps read the same cache lines shifted by one
-// cell, so consecutive position ranges go to one XCD: the grid is a multiple
-// of 8 and workgroup b handles range (b % 8) * (gridDim.x / 8) + b / 8.
+// One thread per cell, a wave per 8 x 8 tile of cells (a workgroup per 32
+// columns x 8 rows): a load of the block planes touches 8 cache lines per
+// wave and a record store about 15 (one per hyperplane through the tile);
+// one thread per record position would make the loads walk a diagonal, one
+// cache line per lane.
 template <int NV, bool VISC>
 __global__ __launch_bounds__(256) void k_lusgs_prep(LusgsArgs a) {
   using F = RecF<NV, VISC>;
   const BlockDesc B = a.blocks[a.block_list[blockIdx.y]];
   const int64_t n = B.sk_n;
-  const unsigned per_xcd = gridDim.x / 8u;
-  const unsigned wg = (blockIdx.x % 8u) * per_xcd + blockIdx.x / 8u;
-  const int64_t pos = (int64_t)wg * 256 + threadIdx.x;
-  if (pos >= (int64_t)B.sk_S * B.sk_R) return;
-  const int s = (int)(pos / B.sk_R), r = (int)(pos - (int64_t)s * B.sk_R);
+  const int ncol = B.nx - 1, nrow = B.ny - 1;
+  const int tiles_i = (ncol + 31) / 32;
+  const int tile = blockIdx.x;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int c = (tile % tiles_i) * 32 + wave * 8 + (lane & 7);  // i - 1
+  const int r = (tile / tiles_i) * 8 + (lane >> 3);              // j - 1
   const double g = a.cps[B.case_id].gamma;
   const double* base = a.arena + B.base;
   double* fw = a.sk + B.sk_base;
-  write_rec<NV, VISC, true>(fw, n, pos, B, base, a.pm, s - r, r, g, a.alfa1);
-  write_rec<NV, VISC, false>(fw + F::NREC * n, n, pos, B, base, a.pm, s - r, r, g, a.alfa1);
-  if (pos == 0) {
+  const int64_t R = B.sk_R;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
     // -0.0 in the first trash-pad slot of the forward X pairs: the X operand
     // of the j-neighbour lane of a pair (no sweep stores there)
-    const int64_t pad = (int64_t)B.sk_S * B.sk_R;
+    const int64_t pad = (int64_t)B.sk_S * R;
 #pragma unroll
     for (int d = 0; d < 2 * F::XP; d++) fw[(int64_t)(d / 2) * 2 * n + 2 * pad + (d & 1)] = -0.0;
   }
+  if (c >= ncol || r >= nrow) return;
+  // the cell's forward record at (c, r), its backward record at (ncol-1-c, nrow-1-r)
+  const int cb = ncol - 1 - c, rb = nrow - 1 - r;
+  write_rec<NV, VISC, true>(fw, n, (int64_t)(c + r) * R + r, B, base, a.pm, c, r, g, a.alfa1);
+  write_rec<NV, VISC, false>(fw + F::NREC * n, n, (int64_t)(cb + rb) * R + rb, B, base, a.pm, cb, rb, g, a.alfa1);
 }
 
 // ----------------------------------------------------------------- sweeps
