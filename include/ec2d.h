@@ -43,8 +43,9 @@ enum {
   EC2D_UN = 23,    /* Un(NVun,0:nx,0:ny) */
   EC2D_QF = 24,    /* QF(4,1:nx-1,1:ny-1) */
   EC2D_DELTU = 25, /* deltU(4,-1:nx+1,-1:ny+1) */
-  EC2D_CLOSURE_MUT = 40,  /* learned closure: mu_t of the last inference (1:nx-1,1:ny-1), SA cases */
-  EC2D_CLOSURE_MASK = 41  /* learned closure: 1 where that mu_t applies (1:nx-1,1:ny-1) */
+  EC2D_CLOSURE_MUT = 40,  /* learned closure: mu_t of the last inference (-1:nx+1,-1:ny+1; SA cases: the
+                             interior, model-4 cases: 0:nx,0:ny) */
+  EC2D_CLOSURE_MASK = 41  /* learned closure: 1 where the features lay inside the training range */
 };
 
 /* ---- life cycle --------------------------------------------------------- */

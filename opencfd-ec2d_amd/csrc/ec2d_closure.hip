@@ -213,6 +213,8 @@ struct ClosureArgs {
   PlaneMap pm;
   const ClosureNet* net;
   float* data;            // collection: 8 floats per listed cell: f0..f5, target nu_t/nu, valid
+  int standin;            // 1: the closure is the eddy-viscosity model: every listed cell gets mu_t
+                          // from its features clamped into the training range (mask: inside it)
 };
 
 __device__ __forceinline__ void unpack_cell(long long c, int& b, int& i, int& j) {
@@ -268,14 +270,19 @@ __global__ __launch_bounds__(256) void k_closure_infer(ClosureArgs a) {
       const BlockDesc B = a.blocks[b];
       ok = closure_features(f, B, a.arena + B.base, a.pm, a.cps[B.case_id], i, j);
     }
+    const bool finite = ok;
+    float xs[CL_NF];
+#pragma unroll
+    for (int k = 0; k < CL_NF; k++) {
+      const float x = (f[k] - N.mean[k]) * N.inv_std[k];
+      ok = ok && x >= N.lo[k] && x <= N.hi[k];
+      xs[k] = finite ? fminf(fmaxf(x, N.lo[k]), N.hi[k]) : 0.f;  // stand-in: clamped into the range
+    }
 #pragma unroll
     for (int k = 0; k < 16; k++) {
       float x = 0.f;
-      if (k < CL_NF) {
-        x = (f[k] - N.mean[k]) * N.inv_std[k];
-        ok = ok && x >= N.lo[k] && x <= N.hi[k];
-      }
-      sX[wave][lane * 16 + k] = f2bf(ok ? x : 0.f);
+      if (k < CL_NF) x = a.standin ? xs[k] : (ok ? xs[k] : 0.f);
+      sX[wave][lane * 16 + k] = f2bf(x);
     }
     sok[wave][lane] = ok ? 1 : 0;
   }
@@ -299,7 +306,7 @@ __global__ __launch_bounds__(256) void k_closure_infer(ClosureArgs a) {
       const double T = (PL(a.pm.U + 3)[qq] - 0.5 * d * (u * u + v * v)) / (cp.Cv * d);
       const double mu = 1.0 / cp.Re * (1.0 + cp.Tsb) * sqrt(T * T * T) / (cp.Tsb + T);
       const bool ok = sok[wave][m] != 0;
-      PL(a.pm.clo)[qq] = ok ? mu * (double)softplus(F.Z4[i]) : 0.0;
+      PL(a.pm.clo)[qq] = (ok || a.standin) ? mu * (double)softplus(F.Z4[i]) : 0.0;
       PL(a.pm.clo + 1)[qq] = ok ? 1.0 : 0.0;
     }
   }

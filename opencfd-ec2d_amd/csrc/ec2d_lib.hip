@@ -101,7 +101,7 @@ struct Runtime {
   double prof_mean[40] = {0};
   // learned closure (ec2d_closure.hip)
   struct {
-    bool ready = false, enabled = false;
+    bool ready = false, enabled = false, stalled = false;  // stalled: a model-4 batch whose residual stalled
     ClosureNet* dnet = nullptr;
     float *theta = nullptr, *m1 = nullptr, *m2 = nullptr, *part = nullptr, *losses = nullptr, *data = nullptr;
     long long* cells = nullptr;
@@ -212,7 +212,9 @@ int build_tiles(DLevel& L) {
   return 0;
 }
 
-PlaneMap make_planes(int nv, int nv_un, bool fdm_any) {
+// clo: the learned closure's planes (SA cases, and cases whose eddy viscosity
+// the closure supplies)
+PlaneMap make_planes(int nv, int nv_un, bool fdm_any, bool clo) {
   PlaneMap p{};
   int k = 0;
   p.U = k; k += nv;
@@ -239,7 +241,7 @@ PlaneMap make_planes(int nv, int nv_un, bool fdm_any) {
   } else {
     p.sst = -1;
   }
-  if (nv == 5) {
+  if (clo) {
     p.clo = k;  // learned closure: mu_t and its in-range mask
     k += 2;
   } else {
@@ -266,14 +268,18 @@ int check_ready() {
 // ---------------------------------------------------------------- launches
 int launch_pw(int level, int op, int krk = 0);
 
-// interior cells of the finest level, all cases, for the closure kernels
+// cells of the finest level, all cases, for the closure kernels: the
+// interior of SA cases; interior and ghost ring (0:nx, 0:ny, whose mu_t the
+// viscous fluxes read) when the closure supplies the eddy viscosity
 int closure_cells() {
   if (R.clo.cells) return 0;
   DLevel& L = R.lv[0];
+  const int g = L.o.turb == 4 ? 1 : 0;
   std::vector<long long> c;
   for (size_t b = 0; b < L.hb.size(); b++)
-    for (int j = 1; j <= L.hb[b].ny - 1; j++)
-      for (int i = 1; i <= L.hb[b].nx - 1; i++) c.push_back(((long long)b << 40) | ((long long)j << 20) | i);
+    for (int j = 1 - g; j <= L.hb[b].ny - 1 + g; j++)
+      for (int i = 1 - g; i <= L.hb[b].nx - 1 + g; i++)
+        c.push_back(((long long)b << 40) | ((long long)j << 20) | i);
   R.clo.ncell = (int)c.size();
   HIPCHK(hipMalloc(&R.clo.cells, sizeof(long long) * c.size()));
   HIPCHK(hipMemcpyAsync(R.clo.cells, c.data(), sizeof(long long) * c.size(), hipMemcpyHostToDevice, R.s));
@@ -292,6 +298,7 @@ ClosureArgs closure_args() {
   a.pm = L.pm;
   a.net = R.clo.dnet;
   a.data = R.clo.data;
+  a.standin = L.o.turb == 4 ? 1 : 0;
   return a;
 }
 
@@ -318,7 +325,9 @@ int launch_residual(int level, int clip) {
   a.pm = L.pm;
   a.clip = clip;
   a.write_visc = 1;
-  a.closure = (level == 0 && R.clo.enabled && L.pm.clo >= 0) ? 1 : 0;
+  // learned closure: 1 = SA case with mu_t overridden where in range,
+  // 2 = the closure supplies mu_t of every cell (turbulence model 4)
+  a.closure = (level == 0 && L.o.turb == 4) ? 2 : (level == 0 && R.clo.enabled && L.pm.clo >= 0) ? 1 : 0;
   if (a.closure && closure_infer()) return 1;
   bool visc = R.cases[0].cp.If_viscous == 1;
   int nv = L.o.nvar;
@@ -741,8 +750,8 @@ int array_map(const DLevel& L, int which, int* slot, int* ncomp, int* kind) {
     case EC2D_UN: *slot = p.Un; *ncomp = L.o.nvar_un; *kind = 3; return 0;
     case EC2D_QF: *slot = p.QF; *ncomp = 4; *kind = 2; return 0;
     case EC2D_DELTU: *slot = p.deltU; *ncomp = 4; *kind = 0; return 0;
-    case EC2D_CLOSURE_MUT: if (p.clo < 0) return 1; *slot = p.clo; *ncomp = 1; *kind = 2; return 0;
-    case EC2D_CLOSURE_MASK: if (p.clo < 0) return 1; *slot = p.clo + 1; *ncomp = 1; *kind = 2; return 0;
+    case EC2D_CLOSURE_MUT: if (p.clo < 0) return 1; *slot = p.clo; *ncomp = 1; *kind = 5; return 0;
+    case EC2D_CLOSURE_MASK: if (p.clo < 0) return 1; *slot = p.clo + 1; *ncomp = 1; *kind = 5; return 0;
   }
   return 1;
 }
@@ -907,6 +916,10 @@ int ec2d_set_tile_rows(int rows) {
 int ec2d_case_create(const ec2d_control* ctl) {
   if (!R.inited && ec2d_init()) return -1;
   if (R.committed) { fail("cases must be created before ec2d_commit"); return -1; }
+  if (ctl->Iflag_turbulence_model == 4 && (ctl->Nvar != 4 || ctl->If_viscous != 1)) {
+    fail("turbulence model 4 (learned closure) needs Nvar = 4 and If_viscous = 1");
+    return -1;
+  }
   HCase c;
   c.ctl = *ctl;
   CaseParams& p = c.cp;
@@ -999,7 +1012,8 @@ int ec2d_commit(void) {
     L.o.recon = c0.IFlag_Reconstruction;
     L.o.turb = (l == 0) ? c0.Iflag_turbulence_model : 0;
     L.fdm = fdm_any;
-    L.pm = make_planes(nv, nv, fdm_any);
+    const int turb0 = c0.Iflag_turbulence_model;
+    L.pm = make_planes(nv, nv, fdm_any, l == 0 && (turb0 == 2 || turb0 == 4));
     // blocks
     int64_t off = 0;
     std::vector<SubfaceDesc> subs;
@@ -1695,6 +1709,11 @@ int ec2d_closure_get_params(float* theta, float* norm) {
 int ec2d_closure_set_params(const float* theta, const float* norm) {
   if (check_ready()) return 1;
   if (!theta || !norm) return fail("null parameters");
+  if (R.lv[0].pm.clo < 0) return fail("the closure runs on SA and model-4 cases");
+  // the cell list exists before a run graph is captured (no allocation under capture)
+  if (closure_cells()) return 1;
+  R.clo.stalled = false;
+  R.clo.best_rms = 0.0;
   return closure_upload(theta, norm);
 }
 
@@ -1712,6 +1731,7 @@ int ec2d_closure_enable(int on) {
   if (on && closure_cells()) return 1;
   R.clo.enabled = on != 0;
   R.clo.best_rms = 0.0;
+  R.clo.stalled = false;
   if (R.graph) {
     HIPCHK(hipGraphExecDestroy(R.graph));
     R.graph = nullptr;
@@ -1721,7 +1741,10 @@ int ec2d_closure_enable(int on) {
 
 int ec2d_closure_monitor(double growth, int* active) {
   if (check_ready()) return 1;
-  if (R.clo.enabled) {
+  // model-4 cases have no SA to switch to inside the batch: a stall is
+  // reported (active = 0) and the caller restarts the case with SA
+  const bool standin = R.lv[0].o.turb == 4;
+  if (R.clo.enabled || (standin && !R.clo.stalled)) {
     double worst = 0.0;
     for (int c = 0; c < (int)R.cases.size(); c++) {
       double mx[6], rms[6];
@@ -1732,14 +1755,15 @@ int ec2d_closure_monitor(double growth, int* active) {
     if (R.clo.best_rms == 0.0 || worst < R.clo.best_rms) R.clo.best_rms = worst;
     else if (worst > growth * R.clo.best_rms || !std::isfinite(worst)) {
       // the residual stalls or grows: fall back to SA everywhere
-      R.clo.enabled = false;
+      if (standin) R.clo.stalled = true;
+      else R.clo.enabled = false;
       if (R.graph) {
         HIPCHK(hipGraphExecDestroy(R.graph));
         R.graph = nullptr;
       }
     }
   }
-  if (active) *active = R.clo.enabled ? 1 : 0;
+  if (active) *active = standin ? (R.clo.stalled ? 0 : 1) : (R.clo.enabled ? 1 : 0);
   return 0;
 }
 

@@ -33,7 +33,8 @@ struct ResArgs {
   PlaneMap pm;
   int clip;                   // apply force_vt_kw to interior nu_t on load
   int write_visc;             // store Amu / Amu_t planes
-  int closure;                // SA: take mu_t of interior cells from the closure planes where in range
+  int closure;                // 1 (SA): take mu_t of interior cells from the closure planes where in range;
+                              // 2 (4 equations): mu_t of every ring cell from the closure plane
 };
 
 template <int TX, int TY, int NV, bool VISC, bool SA>
@@ -115,12 +116,17 @@ __global__ __launch_bounds__(TX* TY) void k_residual(ResArgs a) {
       // SST: mu_t as the model left it (interior and wall ghosts set by
       // k_sst_res, other ghosts untouched)
       if constexpr (NV == 6) at = ring ? PL(pm.Amu_t)[q] : 0.0;
+      // the learned closure as the eddy-viscosity model
+      if constexpr (NV == 4) {
+        if (a.closure == 2 && ring) at = PL(pm.clo)[q];
+      }
       sAt[lj][li] = at;
     }
   }
   __syncthreads();
-  if constexpr (SA) {
+  if (SA || (NV == 4 && VISC && a.closure == 2)) {
     // Amut_boundary: wall ghost cells of layer 0 take -Amu_t of the adjacent interior cell
+    // (sub_turbulence_SA.f90:200-216; the closure's mu_t follows the same rule)
     const double* flag = PL(pm.flag);
     for (int idx = tid; idx < RW * RH; idx += NT) {
       int li = idx % RW, lj = idx / RW;

@@ -32,6 +32,8 @@ RECON_ORIGINAL, RECON_CONSERVATIVE, RECON_CHARACTERISTIC = 0, 1, 2
 BC_WALL, BC_FARFIELD, BC_SYMMETRY, BC_INLET, BC_OUTLET = -10, -20, -30, -40, -50
 TIME_LU_SGS, TIME_EULER1, TIME_RK3, TIME_DUAL_LU_SGS = 0, 1, 3, -1
 TURB_NONE, TURB_BL, TURB_SA, TURB_SST = 0, 1, 2, 3
+# the learned closure standing in for SA's transport (no reference counterpart): 4 equations, mu_t from the MLP
+TURB_CLOSURE = 4
 LAP = 2
 PI = 3.1415926535897932
 

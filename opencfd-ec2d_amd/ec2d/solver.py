@@ -50,13 +50,13 @@ class Case:
         xs, ys = F.read_mesh2d(os.path.join(case_dir, "Mesh2d.dat"))
         bcs = F.read_bcin(os.path.join(case_dir, "bc2d_1.in"))
         fdm = F.read_fdm_in(os.path.join(case_dir, "FDM.in"))
-        need_dw = ctl.Iflag_turbulence_model in (F.TURB_SA, F.TURB_SST)
+        need_dw = ctl.Iflag_turbulence_model in (F.TURB_SA, F.TURB_SST, F.TURB_CLOSURE)
         levels = G.build_levels(xs, ys, bcs, ctl.Num_Mesh, fdm, need_dw)
         return cls(ctl, levels, case_dir)
 
     @classmethod
     def from_arrays(cls, ctl: F.Control, xs, ys, bcs, fdm=(), case_dir=None) -> "Case":
-        need_dw = ctl.Iflag_turbulence_model in (F.TURB_SA, F.TURB_SST)
+        need_dw = ctl.Iflag_turbulence_model in (F.TURB_SA, F.TURB_SST, F.TURB_CLOSURE)
         return cls(ctl, G.build_levels(xs, ys, bcs, ctl.Num_Mesh, fdm, need_dw), case_dir)
 
     @property
@@ -146,8 +146,8 @@ class Batch:
         capi.EC2D_UN: lambda B, nv: (B.ny + 1, B.nx + 1, nv),
         capi.EC2D_QF: lambda B, nv: (B.ny - 1, B.nx - 1, 4),
         capi.EC2D_DELTU: lambda B, nv: (B.ny + 3, B.nx + 3, 4),
-        capi.EC2D_CLOSURE_MUT: lambda B, nv: (B.ny - 1, B.nx - 1),
-        capi.EC2D_CLOSURE_MASK: lambda B, nv: (B.ny - 1, B.nx - 1),
+        capi.EC2D_CLOSURE_MUT: lambda B, nv: (B.ny + 3, B.nx + 3),
+        capi.EC2D_CLOSURE_MASK: lambda B, nv: (B.ny + 3, B.nx + 3),
     }
     NAMES = {"U": capi.EC2D_U, "Res": capi.EC2D_RES, "dt": capi.EC2D_DT, "dU": capi.EC2D_DU, "Lci": capi.EC2D_LCI,
              "Lcj": capi.EC2D_LCJ, "Lvi": capi.EC2D_LVI, "Lvj": capi.EC2D_LVJ, "Amu": capi.EC2D_AMU,

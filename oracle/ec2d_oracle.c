@@ -1028,10 +1028,17 @@ int o_residual(OMesh *M, const OParams *P, OError *err) {
     if (comput_duvtpckw(M, P, B, &f, mb + 1, err)) { flow_free(&f); return 1; }
     if (P->If_viscous == 1) {
       get_viscous(P, B, &f);
-      memset(B->Amu_t, 0, sizeof(double) * (size_t)(B->nx + 3) * (B->ny + 3));
-      if (M->turb == 2) turbulence_sa(M, P, B, &f);
-      else if (M->turb == 3) turbulence_sst(M, P, B, &f);
-      else if (M->turb == 1) { flow_free(&f); return 3; } /* Baldwin-Lomax is not on this path */
+      if (M->turb == 4) {
+        /* eddy viscosity supplied by the caller in Amu_t (0:nx,0:ny), as the
+         * learned closure supplies it (no reference counterpart); its wall
+         * ghosts follow SA's rule (sub_turbulence_SA.f90:200-216) */
+        amut_boundary(B);
+      } else {
+        memset(B->Amu_t, 0, sizeof(double) * (size_t)(B->nx + 3) * (B->ny + 3));
+        if (M->turb == 2) turbulence_sa(M, P, B, &f);
+        else if (M->turb == 3) turbulence_sst(M, P, B, &f);
+        else if (M->turb == 1) { flow_free(&f); return 3; } /* Baldwin-Lomax is not on this path */
+      }
     }
     residual_fvm(M, P, B, &f);
     if (B->fvm_fdm == 1) residual_fdm(M, P, B, &f);

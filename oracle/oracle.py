@@ -150,7 +150,7 @@ class OracleCase:
             xs, ys = F.read_mesh2d(os.path.join(case_dir, "Mesh2d.dat"))
             bcs = F.read_bcin(os.path.join(case_dir, "bc2d_1.in"))
             fdm = F.read_fdm_in(os.path.join(case_dir, "FDM.in"))
-            need_dw = control.Iflag_turbulence_model in (F.TURB_SA, F.TURB_SST)
+            need_dw = control.Iflag_turbulence_model in (F.TURB_SA, F.TURB_SST, F.TURB_CLOSURE)
             levels = G.build_levels(xs, ys, bcs, control.Num_Mesh, fdm, need_dw)
         self.dir = case_dir
         self.ctl = control
