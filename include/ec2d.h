@@ -173,6 +173,10 @@ int ec2d_closure_get_params(float *theta, float *norm);
 int ec2d_closure_set_params(const float *theta, const float *norm);
 int ec2d_closure_infer(void);             /* fills EC2D_CLOSURE_MUT / _MASK from the current state */
 int ec2d_closure_enable(int on);          /* use the closure in every following residual */
+/* Model-4 cases: under-relax the network's mu_t from one residual to the next,
+ * mu_t <- mu_t + w (mu_t(network) - mu_t), w in (0, 1] (1, the default: none;
+ * the plane starts at 0, so mu_t ramps up from laminar). */
+int ec2d_closure_relaxation(double w);
 /* Fallback on a stall: if the largest density Res_rms of the batch exceeds
  * growth times the smallest seen since enabling, switch the closure off
  * (SA everywhere).  active: 1 while the closure is on.  Synchronises. */

@@ -215,6 +215,7 @@ struct ClosureArgs {
   float* data;            // collection: 8 floats per listed cell: f0..f5, target nu_t/nu, valid
   int standin;            // 1: the closure is the eddy-viscosity model: every listed cell gets mu_t
                           // from its features clamped into the training range (mask: inside it)
+  double relax;           // model 4: mu_t <- mu_t + relax (mu_t(network) - mu_t); 1: no relaxation
 };
 
 __device__ __forceinline__ void unpack_cell(long long c, int& b, int& i, int& j) {
@@ -306,7 +307,9 @@ __global__ __launch_bounds__(256) void k_closure_infer(ClosureArgs a) {
       const double T = (PL(a.pm.U + 3)[qq] - 0.5 * d * (u * u + v * v)) / (cp.Cv * d);
       const double mu = 1.0 / cp.Re * (1.0 + cp.Tsb) * sqrt(T * T * T) / (cp.Tsb + T);
       const bool ok = sok[wave][m] != 0;
-      PL(a.pm.clo)[qq] = (ok || a.standin) ? mu * (double)softplus(F.Z4[i]) : 0.0;
+      const double mt = (ok || a.standin) ? mu * (double)softplus(F.Z4[i]) : 0.0;
+      double* out = PL(a.pm.clo) + qq;
+      *out = (a.standin && a.relax < 1.0) ? *out + a.relax * (mt - *out) : mt;
       PL(a.pm.clo + 1)[qq] = ok ? 1.0 : 0.0;
     }
   }

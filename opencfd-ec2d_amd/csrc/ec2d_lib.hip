@@ -102,6 +102,7 @@ struct Runtime {
   // learned closure (ec2d_closure.hip)
   struct {
     bool ready = false, enabled = false, stalled = false;  // stalled: a model-4 batch whose residual stalled
+    double relax = 1.0;                                    // model 4: under-relaxation of mu_t
     ClosureNet* dnet = nullptr;
     float *theta = nullptr, *m1 = nullptr, *m2 = nullptr, *part = nullptr, *losses = nullptr, *data = nullptr;
     long long* cells = nullptr;
@@ -299,6 +300,7 @@ ClosureArgs closure_args() {
   a.net = R.clo.dnet;
   a.data = R.clo.data;
   a.standin = L.o.turb == 4 ? 1 : 0;
+  a.relax = R.clo.relax;
   return a;
 }
 
@@ -1715,6 +1717,16 @@ int ec2d_closure_set_params(const float* theta, const float* norm) {
   R.clo.stalled = false;
   R.clo.best_rms = 0.0;
   return closure_upload(theta, norm);
+}
+
+int ec2d_closure_relaxation(double w) {
+  if (!(w > 0.0 && w <= 1.0)) return fail("relaxation factor must lie in (0, 1]");
+  R.clo.relax = w;
+  if (R.graph) {  // the factor is a kernel argument of the captured graph
+    HIPCHK(hipGraphExecDestroy(R.graph));
+    R.graph = nullptr;
+  }
+  return 0;
 }
 
 int ec2d_closure_infer(void) {

@@ -296,6 +296,10 @@ class Batch:
     def closure_enable(self, on=True):
         capi.call("ec2d_closure_enable", 1 if on else 0)
 
+    def closure_relaxation(self, w: float):
+        """Model-4 cases: under-relax mu_t between residuals (see ec2d_closure_relaxation)."""
+        capi.call("ec2d_closure_relaxation", float(w))
+
     def closure_monitor(self, growth=10.0) -> bool:
         """Fall back to SA if the density residual grew past growth x its best; returns whether the closure is on."""
         a = C.c_int()
