@@ -165,7 +165,7 @@ __device__ __forceinline__ void corner(double* const* U, int64_t q1, int64_t q2,
 }
 
 template <int NV>
-__global__ __launch_bounds__(256) void k_ghost(GhostArgs a) {
+__global__ __launch_bounds__(1024) void k_ghost(GhostArgs a) {
   const int bidx = a.block_list[blockIdx.x];
   const BlockDesc B = a.blocks[bidx];
   const CaseParams cp = a.cps[B.case_id];

@@ -474,7 +474,8 @@ int launch_ghost(int level, int do_bc, int do_exchange) {
   a.nvar = L.o.nvar;
   a.do_bc = do_bc;
   a.do_exchange = do_exchange;
-  dim3 grid((int)L.hb.size()), blk(256);
+  // 1024 threads: a subface of up to 1024 points in one pass per ordered phase
+  dim3 grid((int)L.hb.size()), blk(1024);
   switch (L.o.nvar) {
     case 4: hipLaunchKernelGGL(k_ghost<4>, grid, blk, 0, R.s, a); break;
     case 5: hipLaunchKernelGGL(k_ghost<5>, grid, blk, 0, R.s, a); break;
