@@ -67,6 +67,8 @@ struct DLevel {
   std::vector<int> case_first_block;
   int lusgs_threads = 64;
   int lusgs_lpc = 1;          // LU-SGS lanes per cell: 2 when every block has at most 32 rows
+  int lusgs_split = 1;        // LU-SGS sweep workgroups per block (2: rows split between two CUs)
+  unsigned long long* dgring = nullptr;  // split sweeps: the rings between the two workgroups of a block
   int4* dwall = nullptr;      // wall faces (block, face, i, j), cases in order
   int* dcase_wall0 = nullptr;
   double* dforces = nullptr;  // 4 per case
@@ -86,6 +88,7 @@ struct Runtime {
   std::vector<DLevel> lv;
   CaseParams* dcps = nullptr;
   unsigned long long* derr = nullptr;
+  int* dsweep_err = nullptr;  // a split LU-SGS sweep ran out of its wait budget
   hipStream_t s = nullptr;
   std::string err;
   int num_mesh = 1;
@@ -129,10 +132,16 @@ int fail(const std::string& m) {
 
 // LU-SGS record layout of a level for lpc lanes per cell (see ec2d_lusgs.hip);
 // upload: also refresh the device block table (after ec2d_commit).
-int lusgs_layout(DLevel& L, int lpc, bool upload) {
+int lusgs_layout(DLevel& L, int lpc, int split, bool upload) {
   int max_rows = 1;
   for (auto& d : L.hb) max_rows = std::max(max_rows, d.ny - 1);
-  const int threads = 64 * ((max_rows * lpc + 63) / 64);
+  const int rq = 64 / lpc;  // rows per wave
+  if (split < 1 || split > 2) return fail("LU-SGS split must be 1 or 2");
+  if (split == 2 && (max_rows <= rq || 2 * (int)L.hb.size() > 256))
+    return fail("a split LU-SGS sweep needs blocks taller than one wave and at most 128 blocks");
+  // rows rounded to whole waves per workgroup; split: one communication wave more
+  const int waves = (max_rows + rq * split - 1) / (rq * split) + (split > 1 ? 1 : 0);
+  const int threads = 64 * waves;
   if (threads > 1024) return fail("blocks with more than 1024 rows of cells are not supported by the LU-SGS kernel");
   const int nv = L.o.nvar;
   const bool visc = R.cases[0].cp.If_viscous == 1;
@@ -144,8 +153,7 @@ int lusgs_layout(DLevel& L, int lpc, bool upload) {
   for (auto& d : L.hb) {
     const int ncol = d.nx - 1, nrow = d.ny - 1;
     d.sk_S = ncol + nrow - 1;
-    const int rq = 64 / lpc;  // rows per wave
-    d.sk_R = rq * ((nrow + rq - 1) / rq);
+    d.sk_R = rq * split * ((nrow + rq * split - 1) / (rq * split));
     d.sk_n = (int64_t)d.sk_S * d.sk_R + 64;
     d.sk_base = sk_off;
     if (per * d.sk_n * 8 >= (int64_t)1 << 31) return fail("block too large for the LU-SGS record buffer (2 GiB)");
@@ -159,7 +167,18 @@ int lusgs_layout(DLevel& L, int lpc, bool upload) {
   HIPCHK(hipMalloc(&L.sk, sizeof(double) * (size_t)std::max<int64_t>(sk_off, 1)));
   HIPCHK(hipMemset(L.sk, 0, sizeof(double) * (size_t)std::max<int64_t>(sk_off, 1)));
   L.lusgs_lpc = lpc;
+  L.lusgs_split = split;
   L.lusgs_threads = threads;
+  if (L.dgring) HIPCHK(hipFree(L.dgring));
+  L.dgring = nullptr;
+  if (split > 1) {
+    // the rings between the halves of every block, both sweep directions, all sentinel
+    const int64_t n = (int64_t)L.hb.size() * 2 * GRING * 2 * L.o.nvar;
+    HIPCHK(hipMalloc(&L.dgring, sizeof(unsigned long long) * n));
+    hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, R.s, L.dgring, n, GSENT);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(R.s));
+  }
   if (upload) HIPCHK(hipMemcpy(L.db, L.hb.data(), sizeof(BlockDesc) * L.hb.size(), hipMemcpyHostToDevice));
   return 0;
 }
@@ -419,9 +438,13 @@ int launch_lusgs(int level, const int* dlist, int nblk, double alfa1, int update
   a.pm = L.pm;
   a.alfa1 = alfa1;
   a.update = update;
+  a.split = L.lusgs_split;
+  a.dir = 0;
+  a.gring = L.dgring;
+  a.err = R.dsweep_err;
   bool visc = R.cases[0].cp.If_viscous == 1;
   // prep grid: tiles of 32 x 8 cells of the largest block
-  dim3 gprep((unsigned)L.prep_tiles, nblk), gsweep(nblk), blk(L.lusgs_threads);
+  dim3 gprep((unsigned)L.prep_tiles, nblk), gsweep(nblk * L.lusgs_split), blk(L.lusgs_threads);
   dim3 gscat((unsigned)((L.max_cells + 255) / 256), nblk);
   int nv = L.o.nvar;
   // the instantiation is chosen by thread count (its launch bound): up to 4
@@ -437,6 +460,8 @@ int launch_lusgs(int level, const int* dlist, int nblk, double alfa1, int update
   hipLaunchKernelGGL((k_lusgs_prep<NV, V>), gprep, dim3(256), 0, R.s, a);                   \
   prof_mark(-(4 + pk));                                                                     \
   prof_mark(5 + pk);                                                                        \
+  for (int d = (a.split > 1 ? 1 : 0); d <= (a.split > 1 ? 2 : 0); d++) {                  \
+  a.dir = d;                                                                                \
   if (pair) {                                                                                \
     if (nth <= 64) hipLaunchKernelGGL((k_lusgs_sweep<NV, V, 2, 64, 2>), gsweep, blk, 0, R.s, a); \
     else if (nth <= 256) hipLaunchKernelGGL((k_lusgs_sweep<NV, V, 2, 256, 2>), gsweep, blk, 0, R.s, a); \
@@ -446,6 +471,7 @@ int launch_lusgs(int level, const int* dlist, int nblk, double alfa1, int update
     if (nth <= 256) hipLaunchKernelGGL((k_lusgs_sweep<NV, V, 2, 256, 1>), gsweep, blk, 0, R.s, a); \
     else if (nth <= 512) hipLaunchKernelGGL((k_lusgs_sweep<NV, V, 1, 512, 1>), gsweep, blk, 0, R.s, a); \
     else hipLaunchKernelGGL((k_lusgs_sweep<NV, V, 1, 1024, 1>), gsweep, blk, 0, R.s, a);       \
+  }                                                                                         \
   }                                                                                         \
   prof_mark(-(5 + pk));                                                                     \
   prof_mark(6 + pk);                                                                        \
@@ -889,12 +915,14 @@ int ec2d_finalize(void) {
     hipFree(L.arena); hipFree(L.db); hipFree(L.ds); hipFree(L.dnorm); hipFree(L.dt); hipFree(L.dperm);
     hipFree(L.dcase_tile0); hipFree(L.dncell); hipFree(L.dpartials); hipFree(L.dnorms); hipFree(L.drows);
     hipFree(L.dblocks_all); hipFree(L.sk); hipFree(L.dwall); hipFree(L.dcase_wall0); hipFree(L.dforces);
+    hipFree(L.dgring);
     hipFree(L.d_ilines); hipFree(L.d_jlines); hipFree(L.d_frows);
   }
   hipFree(R.clo.dnet); hipFree(R.clo.theta); hipFree(R.clo.m1); hipFree(R.clo.m2); hipFree(R.clo.part);
   hipFree(R.clo.losses); hipFree(R.clo.data); hipFree(R.clo.cells); hipFree(R.clo.idx);
   hipFree(R.dcps);
   hipFree(R.derr);
+  hipFree(R.dsweep_err);
   for (auto e : R.pev) hipEventDestroy(e);
   hipEventDestroy(R.ev0);
   hipEventDestroy(R.ev1);
@@ -997,6 +1025,8 @@ int ec2d_commit(void) {
   HIPCHK(hipMalloc(&R.dcps, sizeof(CaseParams) * cps.size()));
   HIPCHK(hipMemcpy(R.dcps, cps.data(), sizeof(CaseParams) * cps.size(), hipMemcpyHostToDevice));
   HIPCHK(hipMalloc(&R.derr, sizeof(unsigned long long)));
+  HIPCHK(hipMalloc(&R.dsweep_err, sizeof(int)));
+  HIPCHK(hipMemset(R.dsweep_err, 0, sizeof(int)));
   unsigned long long none = ~0ull;
   HIPCHK(hipMemcpy(R.derr, &none, sizeof(none), hipMemcpyHostToDevice));
   const ec2d_control& c0 = R.cases[0].ctl;
@@ -1116,7 +1146,7 @@ int ec2d_commit(void) {
     L.arena_n = off;
     // two lanes per cell when one wave holds every block's rows that way;
     // otherwise one lane per row (fewer waves, fewer barriers per step)
-    if (lusgs_layout(L, max_rows <= 32 ? 2 : 1, false)) return 1;
+    if (lusgs_layout(L, max_rows <= 32 ? 2 : 1, 1, false)) return 1;
     // host image of the arena
     std::vector<double> img((size_t)off, 0.0);
     init_default_state(L, l, img);
@@ -1503,23 +1533,36 @@ int ec2d_timing(double* run_ms, double* residual_ms, double* lusgs_ms, double* g
   return 0;
 }
 
+int ec2d_lusgs_shape(int nMesh, int lanes_per_cell, int workgroups_per_block) {
+  if (check_ready() || nMesh < 1 || nMesh > R.num_mesh) return fail("bad mesh level");
+  if (lanes_per_cell != 1 && lanes_per_cell != 2) return fail("lanes per cell must be 1 or 2");
+  HIPCHK(hipStreamSynchronize(R.s));
+  if (lusgs_layout(R.lv[nMesh - 1], lanes_per_cell, workgroups_per_block, true)) return 1;
+  if (R.graph) {  // a captured graph holds the old launch shape
+    HIPCHK(hipGraphExecDestroy(R.graph));
+    R.graph = nullptr;
+  }
+  return 0;
+}
+
 int ec2d_autotune(const char* cache_path, int reps, char* report, int report_len) {
   if (check_ready()) return 1;
   if (R.profile) return fail("autotune needs the profile mode off");
   if (reps < 1) reps = 5;
   HIPCHK(hipStreamSynchronize(R.s));  // the layouts below are rewritten with null-stream copies
-  // cache: one line per level configuration, "key tile_rows lpc"
-  std::vector<std::pair<std::string, std::pair<int, int>>> cache;
+  // cache: one line per level configuration, "key tile_rows lpc split"
+  struct Choice { int tr, lp, sp; };
+  std::vector<std::pair<std::string, Choice>> cache;
   if (cache_path && cache_path[0]) {
     if (FILE* f = std::fopen(cache_path, "r")) {
       char key[512];
-      int tr, lp;
-      while (std::fscanf(f, "%511s %d %d", key, &tr, &lp) == 3) cache.push_back({key, {tr, lp}});
+      int tr, lp, sp;
+      while (std::fscanf(f, "%511s %d %d %d", key, &tr, &lp, &sp) == 4) cache.push_back({key, {tr, lp, sp}});
       std::fclose(f);
     }
   }
   std::string rep_txt;
-  std::vector<std::pair<std::string, std::pair<int, int>>> added;
+  std::vector<std::pair<std::string, Choice>> added;
   for (int l = 0; l < R.num_mesh; l++) {
     DLevel& L = R.lv[l];
     // key: the level's block shapes, equations and discretisation
@@ -1527,9 +1570,9 @@ int ec2d_autotune(const char* cache_path, int reps, char* report, int report_len
                       std::to_string(L.o.flux) + "_r" + std::to_string(L.o.recon) + "_t" + std::to_string(L.o.turb) +
                       "_v" + std::to_string(R.cases[0].cp.If_viscous) + (L.fdm ? "_fdm" : "");
     for (auto& b : L.hb) key += "_" + std::to_string(b.nx) + "x" + std::to_string(b.ny);
-    int best_tr = -1, best_lp = -1;
+    int best_tr = -1, best_lp = -1, best_sp = 1;
     for (auto& e : cache)
-      if (e.first == key) { best_tr = e.second.first; best_lp = e.second.second; }
+      if (e.first == key) { best_tr = e.second.tr; best_lp = e.second.lp; best_sp = e.second.sp; }
     char line[256];
     if (best_tr < 0) {
       // residual tile rows
@@ -1541,31 +1584,36 @@ int ec2d_autotune(const char* cache_path, int reps, char* report, int report_len
         if (time_launches(reps, [&] { return launch_residual(l, 0); }, &ms)) return 1;
         if (ms < best) { best = ms; best_tr = tr; }
       }
-      // LU-SGS lanes per cell (the state after a residual; dU only is written)
+      // LU-SGS lanes per cell and workgroups per block (the state after a
+      // residual; dU only is written)
       best = 1e30;
-      for (int lp : {1, 2}) {
-        if (lusgs_layout(L, lp, true)) {
-          R.err.clear();
-          continue;  // layout not possible for these blocks
+      for (int sp : {1, 2})
+        for (int lp : {1, 2}) {
+          if (lusgs_layout(L, lp, sp, true)) {
+            R.err.clear();
+            continue;  // layout not possible for these blocks
+          }
+          double ms;
+          if (time_launches(reps, [&] { return launch_lusgs(l, L.dblocks_all, (int)L.hb.size(), 0.0, 0); }, &ms))
+            return 1;
+          if (ms < best) { best = ms; best_lp = lp; best_sp = sp; }
         }
-        double ms;
-        if (time_launches(reps, [&] { return launch_lusgs(l, L.dblocks_all, (int)L.hb.size(), 0.0, 0); }, &ms)) return 1;
-        if (ms < best) { best = ms; best_lp = lp; }
-      }
       if (best_lp < 0) return fail("no LU-SGS layout fits this level");
-      added.push_back({key, {best_tr, best_lp}});
-      std::snprintf(line, sizeof(line), "level %d: measured tile_rows %d, lanes per cell %d\n", l + 1, best_tr, best_lp);
+      added.push_back({key, {best_tr, best_lp, best_sp}});
+      std::snprintf(line, sizeof(line), "level %d: measured tile_rows %d, lanes per cell %d, workgroups per block %d\n",
+                    l + 1, best_tr, best_lp, best_sp);
     } else {
-      std::snprintf(line, sizeof(line), "level %d: cached tile_rows %d, lanes per cell %d\n", l + 1, best_tr, best_lp);
+      std::snprintf(line, sizeof(line), "level %d: cached tile_rows %d, lanes per cell %d, workgroups per block %d\n",
+                    l + 1, best_tr, best_lp, best_sp);
     }
     rep_txt += line;
     L.tile_rows = best_tr;
     if (build_tiles(L)) return 1;
-    if (lusgs_layout(L, best_lp, true)) return 1;
+    if (lusgs_layout(L, best_lp, best_sp, true)) return 1;
   }
   if (!added.empty() && cache_path && cache_path[0]) {
     if (FILE* f = std::fopen(cache_path, "a")) {
-      for (auto& e : added) std::fprintf(f, "%s %d %d\n", e.first.c_str(), e.second.first, e.second.second);
+      for (auto& e : added) std::fprintf(f, "%s %d %d %d\n", e.first.c_str(), e.second.tr, e.second.lp, e.second.sp);
       std::fclose(f);
     }
   }
@@ -1844,7 +1892,10 @@ int ec2d_res_norms(int case_id, int nMesh, double* res_max, double* res_rms) {
   HIPCHK(hipMemcpyAsync(buf, L.dnorms + 12 * case_id, sizeof(buf), hipMemcpyDeviceToHost, R.s));
   unsigned long long e = 0;
   HIPCHK(hipMemcpyAsync(&e, R.derr, sizeof(e), hipMemcpyDeviceToHost, R.s));
+  int se = 0;
+  HIPCHK(hipMemcpyAsync(&se, R.dsweep_err, sizeof(se), hipMemcpyDeviceToHost, R.s));
   HIPCHK(hipStreamSynchronize(R.s));
+  if (se) return fail("a split LU-SGS sweep timed out waiting for its other half (results invalid)");
   for (int m = 0; m < 6; m++) {
     res_max[m] = buf[m];
     res_rms[m] = buf[6 + m];

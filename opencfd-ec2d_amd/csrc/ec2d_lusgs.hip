@@ -69,6 +69,10 @@ struct LusgsArgs {
   PlaneMap pm;
   double alfa1;
   int update;             // scatter: 1 also adds dU to U
+  int split;              // sweep: workgroups per block (1 or 2: rows split between two CUs)
+  int dir;                // sweep: 0 both sweeps, 1 forward only, 2 backward only (split launches)
+  unsigned long long* gring;  // sweep, split: rings between the two workgroups of a block
+  int* err;               // sweep, split: set when a cross-workgroup wait ran out of its budget
 };
 
 // Field layout of one record set (one sweep direction) of a block.  A
@@ -353,12 +357,16 @@ __device__ __forceinline__ void side_term(double* d, const Rec<NV, NS>& p, int h
 // read, so it saw the slot written), and read again after a wait otherwise.
 // The producer likewise reads cons one step ahead; a stale value only makes
 // it wait longer.
+//
+// ring[w] is the input of wave w: its producer is wave w-1 (or, in the upper
+// workgroup of a split block, the communication wave), prog[w] that
+// producer's progress and cons[w] wave w's consumption.
 constexpr int RING = 8;
 template <int NV>
 struct Chain {
-  double ring[15][RING][2 * NV];  // wave w -> w+1: (dU, U + dU) of its last row
-  int prog[16];                   // steps finished by wave w
-  int cons[16];                   // steps whose input wave w has read
+  double ring[17][RING][2 * NV];  // input of wave w: (dU, U + dU) of the row below it
+  int prog[17];                   // steps finished by the producer of ring[w]
+  int cons[17];                   // steps whose input wave w has read
 };
 
 // per-wave read-ahead state: the next input (lane 0's) and the counters
@@ -387,8 +395,8 @@ __device__ __forceinline__ void wait_at_least(const int* p, int v) {
 template <int NV, bool VISC, bool FWD, int LPC>
 __device__ __forceinline__ void sweep_step(const Rec<NV, 3 - LPC>& p, Out<NV>& own, __amdgpu_buffer_rsrc_t rs,
                                            uint32_t dst, uint32_t n16, int s, int S, int R, int ncol, int nrow, int t,
-                                           int k, int lane, int wave, int nw, double g, Chain<NV>& ch,
-                                           Ahead<NV>& ah) {
+                                           int k, int lane, int wave, bool cin, bool cout, double g,
+                                           Chain<NV>& ch, Ahead<NV>& ah) {
   // the j-neighbour's result: row t-1's, one lane down (the last lane of the
   // previous wave through the LDS ring); the i-neighbour's is the own row's
   Out<NV> up;
@@ -397,12 +405,12 @@ __device__ __forceinline__ void sweep_step(const Rec<NV, 3 - LPC>& p, Out<NV>& o
     up.du[m] = shr1(own.du[m]);
     up.W[m] = shr1(own.W[m]);
   }
-  if (wave > 0) {
-    // the previous wave's last row at step s-1 (an LDS broadcast), lane 0
-    // keeps it; read one step ahead unless the producer was not there yet
+  if (cin) {
+    // the row below this wave at step s-1 (an LDS broadcast), lane 0 keeps
+    // it; read one step ahead unless the producer was not there yet
     if (ah.prog < s) {
-      wait_at_least(&ch.prog[wave - 1], s);
-      const double* x = ch.ring[wave - 1][(s - 1) & (RING - 1)];
+      wait_at_least(&ch.prog[wave], s);
+      const double* x = ch.ring[wave][(s - 1) & (RING - 1)];
 #pragma unroll
       for (int m = 0; m < NV; m++) {
         ah.du[m] = x[m];
@@ -417,9 +425,9 @@ __device__ __forceinline__ void sweep_step(const Rec<NV, 3 - LPC>& p, Out<NV>& o
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
     lds_put(&ch.cons[wave], s);
     // the input of step s+1: prog first, the slot behind it
-    ah.prog = lds_get(&ch.prog[wave - 1]);
+    ah.prog = lds_get(&ch.prog[wave]);
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    const double* x = ch.ring[wave - 1][s & (RING - 1)];
+    const double* x = ch.ring[wave][s & (RING - 1)];
 #pragma unroll
     for (int m = 0; m < NV; m++) {
       ah.du[m] = x[m];
@@ -470,16 +478,16 @@ __device__ __forceinline__ void sweep_step(const Rec<NV, 3 - LPC>& p, Out<NV>& o
     for (int j = 0; j < (NV + 1) / 2; j++)
       bstore2(o.du[2 * j], 2 * j + 1 < NV ? o.du[2 * j + 1] : 0.0, rs, w * 16u, dst + (uint32_t)j * n16);
   }
-  if (wave < nw - 1) {
+  if (cout) {
     // slot s % RING is free once the next wave has read step s - RING's result
     if (ah.cons < s - RING + 1) wait_at_least(&ch.cons[wave + 1], s - RING + 1);
     if (lane == 63) {
-      double* x = ch.ring[wave][s & (RING - 1)];
+      double* x = ch.ring[wave + 1][s & (RING - 1)];
 #pragma unroll
       for (int m = 0; m < NV; m++) { x[m] = own.du[m]; x[NV + m] = own.W[m]; }
     }
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    lds_put(&ch.prog[wave], s + 1);
+    lds_put(&ch.prog[wave + 1], s + 1);
     ah.cons = lds_get(&ch.cons[wave + 1]);
   }
 }
@@ -498,8 +506,8 @@ __device__ __forceinline__ void sweep_step(const Rec<NV, 3 - LPC>& p, Out<NV>& o
 // backward record of the same cell, for the backward sweep the output fields.
 template <int NV, bool VISC, bool FWD, int U, int LPC>
 __device__ __forceinline__ void sweep(__amdgpu_buffer_rsrc_t rs, uint32_t rec, uint32_t dst, uint32_t n16, int S, int R,
-                                      int ncol, int nrow, int t, int k, int lane, int wave, int nw, double g,
-                                      Chain<NV>& ch) {
+                                      int ncol, int nrow, int t, int k, int lane, int wave, bool cin, bool cout,
+                                      double g, Chain<NV>& ch) {
   Ahead<NV> ah = {};  // prog = 0: step 0 needs nothing (its input is never used)
   const int npad = (S + 2 * U - 1) / (2 * U) * (2 * U);
   const int tl = min(t, R - 1);  // loads of lanes beyond the block stay in its records
@@ -520,7 +528,7 @@ __device__ __forceinline__ void sweep(__amdgpu_buffer_rsrc_t rs, uint32_t rec, u
 #pragma unroll
     for (int u = 0; u < U; u++)
       sweep_step<NV, VISC, FWD, LPC>(set[u], own, rs, dst, n16, s_first + u, S, R, ncol, nrow, t, k, lane, wave,
-                                     nw, g, ch, ah);
+                                     cin, cout, g, ch, ah);
   };
   load_set(A, 0);
   for (int s0 = 0; s0 < npad; s0 += 2 * U) {
@@ -535,22 +543,128 @@ __device__ __forceinline__ void sweep(__amdgpu_buffer_rsrc_t rs, uint32_t rec, u
   }
 }
 
-// LPC lanes per cell (1 or 2), U steps per register set, MAXT threads at most
+// ---- a block's rows split between two workgroups (two CUs) ----------------
+// The lower workgroup's last wave feeds an extra communication wave through
+// the LDS ring as if it were the next wave; that wave forwards each step's
+// row through a ring in global memory to the upper workgroup's communication
+// wave, which feeds its first compute wave through the LDS ring.  Compute
+// waves never poll global memory (a wait on a global load would also wait
+// for their record prefetch).  Each global slot double goes sentinel ->
+// value -> sentinel: the forwarder writes a value once it reads the sentinel
+// back, the receiver takes it once it reads a value and writes the sentinel;
+// every double is its own 8-byte single-producer single-consumer handshake,
+// so no ordering between slots is needed.  The sentinel is a NaN payload that
+// arithmetic never produces (it yields the default NaN).  Steps 0 .. npad-2
+// are forwarded: the upper workgroup never needs the last one, and every
+// slot is back to the sentinel when the sweep ends.  Global waits are
+// bounded: past SPIN_MAX polls the wave sets err and stops waiting for the
+// rest of the sweep, so the kernel always ends within one budget (its
+// results are then wrong, and the host reports err).
+constexpr unsigned long long GSENT = 0x7FF4DEADBEEF0001ull;
+constexpr int GRING = 16;
+constexpr int SPIN_MAX = 1 << 20;
+
+__device__ __forceinline__ unsigned long long gload(unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void gstore(unsigned long long* p, unsigned long long v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// sets the global rings to the sentinel once, when the layout is made
+__global__ void k_fill_u64(unsigned long long* p, int64_t n, unsigned long long v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// The communication waves move CK = 64 / (2 NV) steps at a time, one double
+// per lane, so that one round trip to global memory covers CK steps.
+template <int NV>
+struct CommLane {
+  static constexpr int CK = 64 / (2 * NV);
+  int kk, f;
+  bool on;  // this lane carries double f of step kk of the batch
+  __device__ CommLane() : kk((int)(threadIdx.x & 63) / (2 * NV)), f((int)(threadIdx.x & 63) % (2 * NV)), on(kk < CK) {}
+};
+
+// lower workgroup: take ring[nw] (the last compute wave's row) and forward it
+template <int NV>
+__device__ void comm_forward(Chain<NV>& ch, int nw, unsigned long long* gr, int npad, int* err) {
+  const CommLane<NV> c;
+  bool gone = false;  // the other half stopped answering: wait no more
+  for (int s0 = 0; s0 < npad - 1; s0 += c.CK) {
+    const int nst = min(c.CK, npad - 1 - s0);
+    wait_at_least(&ch.prog[nw], s0 + nst);
+    const bool on = c.on && c.kk < nst;
+    const double v = on ? ch.ring[nw][(s0 + c.kk) & (RING - 1)][c.f] : 0.0;
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    lds_put(&ch.cons[nw], s0 + nst);
+    if (on && !gone) {
+      unsigned long long* slot = gr + ((s0 + c.kk) % GRING) * (2 * NV) + c.f;
+      int n = 0;
+      while (gload(slot) != GSENT && ++n < SPIN_MAX) __builtin_amdgcn_s_sleep(1);
+      if (n >= SPIN_MAX) {
+        atomicOr(err, 1);
+        gone = true;
+      }
+      gstore(slot, __builtin_bit_cast(unsigned long long, v));
+    }
+  }
+}
+
+// upper workgroup: receive each batch of rows and feed ring[0]
+template <int NV>
+__device__ void comm_receive(Chain<NV>& ch, unsigned long long* gr, int npad, int* err) {
+  const CommLane<NV> c;
+  static_assert(CommLane<NV>::CK <= RING, "a batch must fit the LDS ring");
+  bool gone = false;  // the other half stopped answering: wait no more
+  for (int s0 = 0; s0 < npad - 1; s0 += c.CK) {
+    const int nst = min(c.CK, npad - 1 - s0);
+    const bool on = c.on && c.kk < nst;
+    double v = 0.0;
+    if (on && !gone) {
+      unsigned long long* slot = gr + ((s0 + c.kk) % GRING) * (2 * NV) + c.f;
+      unsigned long long x;
+      int n = 0;
+      while ((x = gload(slot)) == GSENT && ++n < SPIN_MAX) __builtin_amdgcn_s_sleep(1);
+      if (n >= SPIN_MAX) {
+        atomicOr(err, 1);
+        gone = true;
+      }
+      gstore(slot, GSENT);
+      v = __builtin_bit_cast(double, x);
+    }
+    // the slots of steps s0 .. s0+nst-1 are free once wave 0 read step s0+nst-1-RING
+    wait_at_least(&ch.cons[0], s0 + nst - RING);
+    if (on) ch.ring[0][(s0 + c.kk) & (RING - 1)][c.f] = v;
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    lds_put(&ch.prog[0], s0 + nst);
+  }
+}
+
+// LPC lanes per cell (1 or 2), U steps per register set, MAXT threads at most.
+// a.split workgroups per block (blockIdx.x = block * split + half); with
+// split 2 each half has its compute waves plus one communication wave and
+// the two sweeps are separate launches (a.dir), since the backward sweep of
+// one half needs forward results of the other.
 template <int NV, bool VISC, int U, int MAXT, int LPC>
 __global__ __launch_bounds__(MAXT) void k_lusgs_sweep(LusgsArgs a) {
   using F = RecF<NV, VISC>;
   __shared__ Chain<NV> ch[2];  // one per sweep
-  const BlockDesc B = a.blocks[a.block_list[blockIdx.x]];
+  const int G = a.split, bi = blockIdx.x / G, half = blockIdx.x % G;
+  const BlockDesc B = a.blocks[a.block_list[bi]];
   const double g = a.cps[B.case_id].gamma;
   const int ncol = B.nx - 1, nrow = B.ny - 1;
   const int S = B.sk_S, R = B.sk_R;
   const uint32_t n8 = (uint32_t)B.sk_n * 8u, n16 = 2u * n8;  // bytes of a field of doubles, of pairs
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int t = LPC == 2 ? tid >> 1 : tid, k = LPC == 2 ? tid & 1 : 0;
-  // waves holding rows of this block (R is a multiple of 64 / LPC); the
-  // launch may carry more for the level's tallest block, they sit out
-  const int nw = R * LPC / 64;
-  if (tid < 16) {
+  // compute waves holding rows of this half (R is a multiple of split * 64 /
+  // LPC); the launch may carry more for the level's tallest block, they sit out
+  const int nw = R * LPC / (64 * G);
+  const int t = half * (R / G) + (LPC == 2 ? tid >> 1 : tid), k = LPC == 2 ? tid & 1 : 0;
+  const bool has_in = half > 0, has_out = half < G - 1;
+  const bool cin = wave > 0 || has_in, cout = wave < nw - 1 || has_out;
+  if (tid < 17) {
     ch[0].prog[tid] = ch[0].cons[tid] = ch[1].prog[tid] = ch[1].cons[tid] = 0;
   }
   __syncthreads();
@@ -559,10 +673,23 @@ __global__ __launch_bounds__(MAXT) void k_lusgs_sweep(LusgsArgs a) {
   const uint32_t bytes = (uint32_t)((2 * F::NREC + 2 * F::XP) * B.sk_n * 8);
   const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(a.sk + B.sk_base, 0, bytes, 0x00020000);
   const uint32_t fw = 0, bw = F::NREC * n8, out = 2 * F::NREC * n8;
-  if (wave < nw) sweep<NV, VISC, true, U, LPC>(rs, fw, bw, n16, S, R, ncol, nrow, t, k, lane, wave, nw, g, ch[0]);
-  __threadfence();
-  __syncthreads();
-  if (wave < nw) sweep<NV, VISC, false, U, LPC>(rs, bw, out, n16, S, R, ncol, nrow, t, k, lane, wave, nw, g, ch[1]);
+  const int npad = (S + 2 * U - 1) / (2 * U) * (2 * U);
+  // global ring of this block's boundary for sweep direction d
+  auto gr = [&](int d) { return a.gring + ((int64_t)bi * 2 + d) * GRING * 2 * NV; };
+  if (a.dir != 2) {
+    if (wave < nw) sweep<NV, VISC, true, U, LPC>(rs, fw, bw, n16, S, R, ncol, nrow, t, k, lane, wave, cin, cout, g, ch[0]);
+    else if (wave == nw && has_out) comm_forward<NV>(ch[0], nw, gr(0), npad, a.err);
+    else if (wave == nw && has_in) comm_receive<NV>(ch[0], gr(0), npad, a.err);
+  }
+  if (a.dir == 0) {
+    __threadfence();
+    __syncthreads();
+  }
+  if (a.dir != 1) {
+    if (wave < nw) sweep<NV, VISC, false, U, LPC>(rs, bw, out, n16, S, R, ncol, nrow, t, k, lane, wave, cin, cout, g, ch[1]);
+    else if (wave == nw && has_out) comm_forward<NV>(ch[1], nw, gr(1), npad, a.err);
+    else if (wave == nw && has_in) comm_receive<NV>(ch[1], gr(1), npad, a.err);
+  }
 }
 
 // ----------------------------------------------------------------- scatter

@@ -76,6 +76,7 @@ SIGNATURES = {
     "ec2d_closure_set_params": [C.POINTER(C.c_float), C.POINTER(C.c_float)],
     "ec2d_closure_infer": [],
     "ec2d_closure_enable": [_i],
+    "ec2d_lusgs_shape": [_i, _i, _i],
     "ec2d_closure_relaxation": [C.c_double],
     "ec2d_closure_monitor": [C.c_double, _ip],
     "ec2d_set_tile_rows": [_i],

@@ -256,6 +256,10 @@ class Batch:
         capi.call("ec2d_force_coefficients", case_id, C.byref(cl), C.byref(cd), f.ctypes.data_as(C.POINTER(C.c_double)))
         return cl.value, cd.value, f
 
+    def lusgs_shape(self, nmesh: int, lanes_per_cell: int, workgroups_per_block: int = 1):
+        """LU-SGS launch shape of one level by hand (see ec2d_lusgs_shape)."""
+        capi.call("ec2d_lusgs_shape", nmesh, lanes_per_cell, workgroups_per_block)
+
     def autotune(self, cache_path: str = "", reps: int = 5) -> str:
         """Time and pick residual tile rows and LU-SGS lanes per cell per level
         (see ec2d_autotune); returns the per-level report."""

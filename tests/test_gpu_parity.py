@@ -288,3 +288,66 @@ def test_autotune_keeps_results_and_caches(tmp_path):
         assert first_diff(b.get("U"), ref.block().U) is None
         rep2 = b.autotune(cache, reps=2)
         assert rep2.count("cached") == 2, rep2
+
+
+@pytest.mark.parametrize("lpc", [1, 2])
+def test_split_sweep_tall_block_multigrid(lpc):
+    """Rows of a block split between two workgroups (two CUs): the halves
+    exchange the boundary row through a ring in global memory; results stay
+    bit-identical to the oracle (LU-SGS-smoothed 2-level V-cycle)."""
+    c = cases.control_euler_naca(iters=10)
+    c.Time_Method, c.Num_Mesh, c.CFL = F.TIME_LU_SGS, 2, 2.0
+    d = naca_case(f"g_split_{lpc}", c, nx=129, ny=73)
+    ref = O.OracleCase(d)
+    ref.init_flow()
+    with solver.Batch([solver.Case.from_dir(d)]) as b:
+        b.init_flow()
+        b.lusgs_shape(1, lpc, 2)
+        b.lusgs_shape(2, lpc, 1)
+        for k in range(3):
+            ref.main_step()
+            b.main_step()
+            assert first_diff(b.get("U"), ref.block().U) is None, f"step {k}"
+        b.run(3)
+        for _ in range(3):
+            ref.main_step()
+        assert first_diff(b.get("U"), ref.block().U) is None
+        b.res_norms()  # reports a timed-out cross-workgroup wait, if any
+
+
+def test_split_sweep_sa():
+    c = cases.control_sa_naca(iters=10, Re=1e5, Ma=0.5)
+    d = naca_case("g_split_sa", c, nx=129, ny=65, ds1=1e-4)
+    ref = O.OracleCase(d)
+    ref.init_flow()
+    with solver.Batch([solver.Case.from_dir(d)]) as b:
+        b.init_flow()
+        b.lusgs_shape(1, 2, 2)
+        for k in range(4):
+            ref.main_step()
+            b.main_step()
+            np.testing.assert_allclose(b.get("U"), ref.block().U, rtol=1e-11, atol=1e-300, err_msg=f"step {k}")
+        b.res_norms()
+
+
+def test_split_sweep_uneven_multiblock():
+    """64 uneven blocks, each split in two: some upper halves hold padding
+    rows only and still relay nothing but the protocol."""
+    from helpers import case_dir
+
+    d = case_dir("g_split_mb")
+    cases.write_vortex_case(d, cases.control_vortex(iters=10), seed=5, nbx=8, nby=8, wmin=9, wmax=41, hmin=9,
+                            hmax=49, h=0.25)
+    ref = O.OracleCase(d)
+    ref.init_flow()
+    case = solver.Case.from_dir(d)
+    with solver.Batch([case]) as b:
+        b.init_flow()
+        b.lusgs_shape(1, 2, 2)
+        for k in range(3):
+            ref.main_step()
+            b.main_step()
+            for mb in range(1, 65):
+                diff = first_diff(b.get("U", 0, 1, mb), ref.block(1, mb).U)
+                assert diff is None, f"step {k} block {mb}: {diff}"
+        b.res_norms()
