@@ -207,14 +207,14 @@ int ec2d_set_profile(int on);
 
 /* No reference counterpart: the LU-SGS launch shape of one level, chosen by
  * hand (ec2d_autotune chooses it by timing otherwise): lanes per cell 1 or 2,
- * workgroups per block 1 or 2 (2: each block's rows split between two CUs,
- * for levels of at most 128 blocks taller than one wave).  Results do not
- * depend on it. */
+ * workgroups per block 1, 2 or 4 (each block's rows split between that many
+ * CUs, for levels of at most 256/split blocks taller than split/2 waves).
+ * Results do not depend on it. */
 int ec2d_lusgs_shape(int nMesh, int lanes_per_cell, int workgroups_per_block);
 
 /* Autotune the launch shapes of every mesh level on the committed batch:
  * residual tile rows (1, 2, 4, 8 x 64 cells), LU-SGS lanes per cell (1 or 2)
- * and workgroups per block (1 or 2, see ec2d_lusgs_shape), each timed with
+ * and workgroups per block (1, 2 or 4, see ec2d_lusgs_shape), each timed with
  * HIP events over `reps` launches on the current state;
  * the fastest is kept.  Results are cached per level configuration (equations,
  * scheme, flux, reconstruction, turbulence model, viscosity, block shapes) in

@@ -67,8 +67,9 @@ struct DLevel {
   std::vector<int> case_first_block;
   int lusgs_threads = 64;
   int lusgs_lpc = 1;          // LU-SGS lanes per cell: 2 when every block has at most 32 rows
-  int lusgs_split = 1;        // LU-SGS sweep workgroups per block (2: rows split between two CUs)
-  unsigned long long* dgring = nullptr;  // split sweeps: the rings between the two workgroups of a block
+  int lusgs_split = 1;        // LU-SGS sweep workgroups per block (2 or 4: a block's rows split between CUs)
+  unsigned long long* dgring = nullptr;  // split sweeps: one ring per boundary between workgroups and direction
+  int64_t gstride = 0;                   // slots of one ring
   int4* dwall = nullptr;      // wall faces (block, face, i, j), cases in order
   int* dcase_wall0 = nullptr;
   double* dforces = nullptr;  // 4 per case
@@ -136,11 +137,13 @@ int lusgs_layout(DLevel& L, int lpc, int split, bool upload) {
   int max_rows = 1;
   for (auto& d : L.hb) max_rows = std::max(max_rows, d.ny - 1);
   const int rq = 64 / lpc;  // rows per wave
-  if (split < 1 || split > 2) return fail("LU-SGS split must be 1 or 2");
-  if (split == 2 && (max_rows <= rq || 2 * (int)L.hb.size() > 256))
-    return fail("a split LU-SGS sweep needs blocks taller than one wave and at most 128 blocks");
-  // rows rounded to whole waves per workgroup; split: one communication wave more
-  const int waves = (max_rows + rq * split - 1) / (rq * split) + (split > 1 ? 1 : 0);
+  if (split != 1 && split != 2 && split != 4) return fail("LU-SGS split must be 1, 2 or 4");
+  // every workgroup of a split launch must be resident at once: one per CU at most
+  if (split > 1 && (max_rows <= rq * (split / 2) || split * (int)L.hb.size() > 256))
+    return fail("a split LU-SGS sweep needs blocks taller than split/2 waves and at most 256/split blocks");
+  // rows rounded to whole waves per workgroup; split: one communication wave
+  // more, two for the middle parts of a split into four
+  const int waves = (max_rows + rq * split - 1) / (rq * split) + (split > 2 ? 2 : split - 1);
   const int threads = 64 * waves;
   if (threads > 1024) return fail("blocks with more than 1024 rows of cells are not supported by the LU-SGS kernel");
   const int nv = L.o.nvar;
@@ -171,9 +174,14 @@ int lusgs_layout(DLevel& L, int lpc, int split, bool upload) {
   L.lusgs_threads = threads;
   if (L.dgring) HIPCHK(hipFree(L.dgring));
   L.dgring = nullptr;
+  L.gstride = 0;
   if (split > 1) {
-    // the rings between the halves of every block, both sweep directions, all sentinel
-    const int64_t n = (int64_t)L.hb.size() * 2 * GRING * 2 * L.o.nvar;
+    // one ring per boundary between parts of a block and sweep direction, a
+    // slot per step (npad <= S + 3) and double, all sentinel
+    int max_S = 1;
+    for (auto& d : L.hb) max_S = std::max(max_S, d.sk_S);
+    L.gstride = ((int64_t)(max_S + 4) * 2 * L.o.nvar + 31) / 32 * 32;
+    const int64_t n = (int64_t)L.hb.size() * (split - 1) * 2 * L.gstride;
     HIPCHK(hipMalloc(&L.dgring, sizeof(unsigned long long) * n));
     hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, R.s, L.dgring, n, GSENT);
     HIPCHK(hipGetLastError());
@@ -441,6 +449,7 @@ int launch_lusgs(int level, const int* dlist, int nblk, double alfa1, int update
   a.split = L.lusgs_split;
   a.dir = 0;
   a.gring = L.dgring;
+  a.gstride = L.gstride;
   a.err = R.dsweep_err;
   bool visc = R.cases[0].cp.If_viscous == 1;
   // prep grid: tiles of 32 x 8 cells of the largest block
@@ -1587,7 +1596,7 @@ int ec2d_autotune(const char* cache_path, int reps, char* report, int report_len
       // LU-SGS lanes per cell and workgroups per block (the state after a
       // residual; dU only is written)
       best = 1e30;
-      for (int sp : {1, 2})
+      for (int sp : {1, 2, 4})
         for (int lp : {1, 2}) {
           if (lusgs_layout(L, lp, sp, true)) {
             R.err.clear();

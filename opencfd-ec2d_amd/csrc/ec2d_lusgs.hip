@@ -69,9 +69,10 @@ struct LusgsArgs {
   PlaneMap pm;
   double alfa1;
   int update;             // scatter: 1 also adds dU to U
-  int split;              // sweep: workgroups per block (1 or 2: rows split between two CUs)
+  int split;              // sweep: workgroups per block (1, 2 or 4: a block's rows split between CUs)
   int dir;                // sweep: 0 both sweeps, 1 forward only, 2 backward only (split launches)
-  unsigned long long* gring;  // sweep, split: rings between the two workgroups of a block
+  unsigned long long* gring;  // sweep, split: one ring per boundary between workgroups and direction
+  int64_t gstride;        // sweep, split: slots of one ring (at least npad * 2 * NV)
   int* err;               // sweep, split: set when a cross-workgroup wait ran out of its budget
 };
 
@@ -543,25 +544,25 @@ __device__ __forceinline__ void sweep(__amdgpu_buffer_rsrc_t rs, uint32_t rec, u
   }
 }
 
-// ---- a block's rows split between two workgroups (two CUs) ----------------
-// The lower workgroup's last wave feeds an extra communication wave through
-// the LDS ring as if it were the next wave; that wave forwards each step's
-// row through a ring in global memory to the upper workgroup's communication
-// wave, which feeds its first compute wave through the LDS ring.  Compute
-// waves never poll global memory (a wait on a global load would also wait
-// for their record prefetch).  Each global slot double goes sentinel ->
-// value -> sentinel: the forwarder writes a value once it reads the sentinel
-// back, the receiver takes it once it reads a value and writes the sentinel;
-// every double is its own 8-byte single-producer single-consumer handshake,
-// so no ordering between slots is needed.  The sentinel is a NaN payload that
-// arithmetic never produces (it yields the default NaN).  Steps 0 .. npad-2
-// are forwarded: the upper workgroup never needs the last one, and every
-// slot is back to the sentinel when the sweep ends.  Global waits are
-// bounded: past SPIN_MAX polls the wave sets err and stops waiting for the
-// rest of the sweep, so the kernel always ends within one budget (its
-// results are then wrong, and the host reports err).
+// ---- a block's rows split between workgroups (one CU each) --------------
+// The last compute wave of a workgroup feeds an extra communication wave
+// through the LDS ring as if it were the next wave; that wave forwards each
+// step's row through global memory to a communication wave of the next
+// workgroup, which feeds its first compute wave through the LDS ring.
+// Compute waves never poll global memory (a wait on a global load would also
+// wait for their record prefetch).  Every step of a sweep has its own global
+// slot (steps 0 .. npad-2; the next workgroup never needs the last one), so
+// the forwarder only stores: no flow control, no round trip.  A slot double
+// holds the sentinel until the forwarder writes the value; the receiver
+// polls each double until it differs from the sentinel (every double is its
+// own 8-byte handshake, so no ordering between doubles is needed) and, when
+// its sweep is done, sets all its slots back to the sentinel for the next
+// launch.  The sentinel is a NaN payload that arithmetic never produces (it
+// yields the default NaN).  Global waits are bounded: past SPIN_MAX polls
+// the wave sets err and stops waiting for the rest of the sweep, so the
+// kernel always ends within one budget (its results are then wrong, and the
+// host reports err).
 constexpr unsigned long long GSENT = 0x7FF4DEADBEEF0001ull;
-constexpr int GRING = 16;
 constexpr int SPIN_MAX = 1 << 20;
 
 __device__ __forceinline__ unsigned long long gload(unsigned long long* p) {
@@ -587,11 +588,10 @@ struct CommLane {
   __device__ CommLane() : kk((int)(threadIdx.x & 63) / (2 * NV)), f((int)(threadIdx.x & 63) % (2 * NV)), on(kk < CK) {}
 };
 
-// lower workgroup: take ring[nw] (the last compute wave's row) and forward it
+// sending side: take ring[nw] (the last compute wave's row) and store it
 template <int NV>
-__device__ void comm_forward(Chain<NV>& ch, int nw, unsigned long long* gr, int npad, int* err) {
+__device__ void comm_forward(Chain<NV>& ch, int nw, unsigned long long* gr, int npad) {
   const CommLane<NV> c;
-  bool gone = false;  // the other half stopped answering: wait no more
   for (int s0 = 0; s0 < npad - 1; s0 += c.CK) {
     const int nst = min(c.CK, npad - 1 - s0);
     wait_at_least(&ch.prog[nw], s0 + nst);
@@ -599,31 +599,22 @@ __device__ void comm_forward(Chain<NV>& ch, int nw, unsigned long long* gr, int 
     const double v = on ? ch.ring[nw][(s0 + c.kk) & (RING - 1)][c.f] : 0.0;
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
     lds_put(&ch.cons[nw], s0 + nst);
-    if (on && !gone) {
-      unsigned long long* slot = gr + ((s0 + c.kk) % GRING) * (2 * NV) + c.f;
-      int n = 0;
-      while (gload(slot) != GSENT && ++n < SPIN_MAX) __builtin_amdgcn_s_sleep(1);
-      if (n >= SPIN_MAX) {
-        atomicOr(err, 1);
-        gone = true;
-      }
-      gstore(slot, __builtin_bit_cast(unsigned long long, v));
-    }
+    if (on) gstore(gr + (s0 + c.kk) * (2 * NV) + c.f, __builtin_bit_cast(unsigned long long, v));
   }
 }
 
-// upper workgroup: receive each batch of rows and feed ring[0]
+// receiving side: take each batch of rows and feed ring[0]; then reset
 template <int NV>
 __device__ void comm_receive(Chain<NV>& ch, unsigned long long* gr, int npad, int* err) {
   const CommLane<NV> c;
   static_assert(CommLane<NV>::CK <= RING, "a batch must fit the LDS ring");
-  bool gone = false;  // the other half stopped answering: wait no more
+  bool gone = false;  // the other workgroup stopped answering: wait no more
   for (int s0 = 0; s0 < npad - 1; s0 += c.CK) {
     const int nst = min(c.CK, npad - 1 - s0);
     const bool on = c.on && c.kk < nst;
     double v = 0.0;
     if (on && !gone) {
-      unsigned long long* slot = gr + ((s0 + c.kk) % GRING) * (2 * NV) + c.f;
+      unsigned long long* slot = gr + (s0 + c.kk) * (2 * NV) + c.f;
       unsigned long long x;
       int n = 0;
       while ((x = gload(slot)) == GSENT && ++n < SPIN_MAX) __builtin_amdgcn_s_sleep(1);
@@ -631,7 +622,6 @@ __device__ void comm_receive(Chain<NV>& ch, unsigned long long* gr, int npad, in
         atomicOr(err, 1);
         gone = true;
       }
-      gstore(slot, GSENT);
       v = __builtin_bit_cast(double, x);
     }
     // the slots of steps s0 .. s0+nst-1 are free once wave 0 read step s0+nst-1-RING
@@ -640,25 +630,28 @@ __device__ void comm_receive(Chain<NV>& ch, unsigned long long* gr, int npad, in
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
     lds_put(&ch.prog[0], s0 + nst);
   }
+  // every value of this launch has been read: back to the sentinel
+  for (int i = (int)(threadIdx.x & 63); i < (npad - 1) * 2 * NV; i += 64) gstore(gr + i, GSENT);
 }
 
 // LPC lanes per cell (1 or 2), U steps per register set, MAXT threads at most.
-// a.split workgroups per block (blockIdx.x = block * split + half); with
-// split 2 each half has its compute waves plus one communication wave and
+// a.split workgroups per block (blockIdx.x = block * split + part); with a
+// split each part has its compute waves, then a receiving communication wave
+// (all parts but the first), then a sending one (all parts but the last), and
 // the two sweeps are separate launches (a.dir), since the backward sweep of
-// one half needs forward results of the other.
+// one part needs forward results of the others.
 template <int NV, bool VISC, int U, int MAXT, int LPC>
 __global__ __launch_bounds__(MAXT) void k_lusgs_sweep(LusgsArgs a) {
   using F = RecF<NV, VISC>;
   __shared__ Chain<NV> ch[2];  // one per sweep
-  const int G = a.split, bi = blockIdx.x / G, half = blockIdx.x % G;
+  const int G = a.split, bi = blockIdx.x / G, half = blockIdx.x % G;  // half: the part of the block
   const BlockDesc B = a.blocks[a.block_list[bi]];
   const double g = a.cps[B.case_id].gamma;
   const int ncol = B.nx - 1, nrow = B.ny - 1;
   const int S = B.sk_S, R = B.sk_R;
   const uint32_t n8 = (uint32_t)B.sk_n * 8u, n16 = 2u * n8;  // bytes of a field of doubles, of pairs
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  // compute waves holding rows of this half (R is a multiple of split * 64 /
+  // compute waves holding rows of this part (R is a multiple of split * 64 /
   // LPC); the launch may carry more for the level's tallest block, they sit out
   const int nw = R * LPC / (64 * G);
   const int t = half * (R / G) + (LPC == 2 ? tid >> 1 : tid), k = LPC == 2 ? tid & 1 : 0;
@@ -674,12 +667,13 @@ __global__ __launch_bounds__(MAXT) void k_lusgs_sweep(LusgsArgs a) {
   const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(a.sk + B.sk_base, 0, bytes, 0x00020000);
   const uint32_t fw = 0, bw = F::NREC * n8, out = 2 * F::NREC * n8;
   const int npad = (S + 2 * U - 1) / (2 * U) * (2 * U);
-  // global ring of this block's boundary for sweep direction d
-  auto gr = [&](int d) { return a.gring + ((int64_t)bi * 2 + d) * GRING * 2 * NV; };
+  // global ring of boundary b (between parts b and b+1) for sweep direction d
+  auto gr = [&](int b, int d) { return a.gring + (((int64_t)bi * (G - 1) + b) * 2 + d) * a.gstride; };
+  const bool rcv = has_in && wave == nw, snd = has_out && wave == nw + (has_in ? 1 : 0);
   if (a.dir != 2) {
     if (wave < nw) sweep<NV, VISC, true, U, LPC>(rs, fw, bw, n16, S, R, ncol, nrow, t, k, lane, wave, cin, cout, g, ch[0]);
-    else if (wave == nw && has_out) comm_forward<NV>(ch[0], nw, gr(0), npad, a.err);
-    else if (wave == nw && has_in) comm_receive<NV>(ch[0], gr(0), npad, a.err);
+    else if (rcv) comm_receive<NV>(ch[0], gr(half - 1, 0), npad, a.err);
+    else if (snd) comm_forward<NV>(ch[0], nw, gr(half, 0), npad);
   }
   if (a.dir == 0) {
     __threadfence();
@@ -687,8 +681,8 @@ __global__ __launch_bounds__(MAXT) void k_lusgs_sweep(LusgsArgs a) {
   }
   if (a.dir != 1) {
     if (wave < nw) sweep<NV, VISC, false, U, LPC>(rs, bw, out, n16, S, R, ncol, nrow, t, k, lane, wave, cin, cout, g, ch[1]);
-    else if (wave == nw && has_out) comm_forward<NV>(ch[1], nw, gr(1), npad, a.err);
-    else if (wave == nw && has_in) comm_receive<NV>(ch[1], gr(1), npad, a.err);
+    else if (rcv) comm_receive<NV>(ch[1], gr(half - 1, 1), npad, a.err);
+    else if (snd) comm_forward<NV>(ch[1], nw, gr(half, 1), npad);
   }
 }
 

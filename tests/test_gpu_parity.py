@@ -290,19 +290,20 @@ def test_autotune_keeps_results_and_caches(tmp_path):
         assert rep2.count("cached") == 2, rep2
 
 
-@pytest.mark.parametrize("lpc", [1, 2])
-def test_split_sweep_tall_block_multigrid(lpc):
-    """Rows of a block split between two workgroups (two CUs): the halves
-    exchange the boundary row through a ring in global memory; results stay
-    bit-identical to the oracle (LU-SGS-smoothed 2-level V-cycle)."""
+@pytest.mark.parametrize("lpc,split,ny", [(1, 2, 73), (2, 2, 73), (2, 4, 73), (2, 4, 129)])
+def test_split_sweep_tall_block_multigrid(lpc, split, ny):
+    """Rows of a block split between two or four workgroups (one CU each):
+    neighbouring parts pass the boundary row through global memory; results
+    stay bit-identical to the oracle (LU-SGS-smoothed 2-level V-cycle).  With
+    72 rows split in four the last part holds padding rows only."""
     c = cases.control_euler_naca(iters=10)
     c.Time_Method, c.Num_Mesh, c.CFL = F.TIME_LU_SGS, 2, 2.0
-    d = naca_case(f"g_split_{lpc}", c, nx=129, ny=73)
+    d = naca_case(f"g_split_{lpc}_{split}_{ny}", c, nx=129, ny=ny)
     ref = O.OracleCase(d)
     ref.init_flow()
     with solver.Batch([solver.Case.from_dir(d)]) as b:
         b.init_flow()
-        b.lusgs_shape(1, lpc, 2)
+        b.lusgs_shape(1, lpc, split)
         b.lusgs_shape(2, lpc, 1)
         for k in range(3):
             ref.main_step()
@@ -315,14 +316,15 @@ def test_split_sweep_tall_block_multigrid(lpc):
         b.res_norms()  # reports a timed-out cross-workgroup wait, if any
 
 
-def test_split_sweep_sa():
+@pytest.mark.parametrize("split,ny", [(2, 65), (4, 97)])
+def test_split_sweep_sa(split, ny):
     c = cases.control_sa_naca(iters=10, Re=1e5, Ma=0.5)
-    d = naca_case("g_split_sa", c, nx=129, ny=65, ds1=1e-4)
+    d = naca_case(f"g_split_sa_{split}", c, nx=129, ny=ny, ds1=1e-4)
     ref = O.OracleCase(d)
     ref.init_flow()
     with solver.Batch([solver.Case.from_dir(d)]) as b:
         b.init_flow()
-        b.lusgs_shape(1, 2, 2)
+        b.lusgs_shape(1, 2, split)
         for k in range(4):
             ref.main_step()
             b.main_step()
@@ -330,20 +332,22 @@ def test_split_sweep_sa():
         b.res_norms()
 
 
-def test_split_sweep_uneven_multiblock():
-    """64 uneven blocks, each split in two: some upper halves hold padding
-    rows only and still relay nothing but the protocol."""
+@pytest.mark.parametrize("split,hmax", [(2, 49), (4, 81)])
+def test_split_sweep_uneven_multiblock(split, hmax):
+    """64 uneven blocks, each split in two or four (256 workgroups): some
+    upper parts hold padding rows only and still relay nothing but the
+    protocol."""
     from helpers import case_dir
 
-    d = case_dir("g_split_mb")
+    d = case_dir(f"g_split_mb_{split}")
     cases.write_vortex_case(d, cases.control_vortex(iters=10), seed=5, nbx=8, nby=8, wmin=9, wmax=41, hmin=9,
-                            hmax=49, h=0.25)
+                            hmax=hmax, h=0.25)
     ref = O.OracleCase(d)
     ref.init_flow()
     case = solver.Case.from_dir(d)
     with solver.Batch([case]) as b:
         b.init_flow()
-        b.lusgs_shape(1, 2, 2)
+        b.lusgs_shape(1, 2, split)
         for k in range(3):
             ref.main_step()
             b.main_step()
