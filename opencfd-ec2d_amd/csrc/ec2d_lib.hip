@@ -61,6 +61,7 @@ struct DLevel {
   double* dncell = nullptr;
   double* dpartials = nullptr;
   double* dnorms = nullptr;
+  bool residual_done = false;  // a residual launch has written dpartials (reduced on demand)
   int2* drows = nullptr;
   int nrows = 0;
   int* dblocks_all = nullptr;
@@ -191,6 +192,8 @@ int lusgs_layout(DLevel& L, int lpc, int split, bool upload) {
   return 0;
 }
 
+int reduce_norms(DLevel& L);
+
 // Residual tiles of a level (64 x L.tile_rows cells), their XCD-aware launch
 // order and the per-tile norm partials.
 int build_tiles(DLevel& L) {
@@ -225,6 +228,12 @@ int build_tiles(DLevel& L) {
     if (v >= 0) perm2.push_back(v);
   perm = perm2;
   L.nlaunch = (int)perm.size();
+  if (L.residual_done) {
+    // keep the last residual's norms: reduce them before the partials go
+    if (reduce_norms(L)) return 1;
+    HIPCHK(hipStreamSynchronize(R.s));
+    L.residual_done = false;
+  }
   hipFree(L.dt);
   hipFree(L.dperm);
   hipFree(L.dcase_tile0);
@@ -425,7 +434,17 @@ int launch_residual(int level, int clip) {
     else hipLaunchKernelGGL(k_tile_norms<6>, dim3(L.ntiles), dim3(256), 0, R.s, L.dt, L.db, L.arena, L.pm, L.dpartials, L.tile_rows);
     HIPCHK(hipGetLastError());
   }
-  int ncase = (int)R.cases.size();
+  // the per-tile partials hold this residual's norms; they are reduced when
+  // the host asks for them (reduce_norms), since nothing on the device reads
+  // them and every residual launch of the level rewrites all of them
+  L.residual_done = true;
+  return 0;
+}
+
+// Res_max / Res_rms of every case of a level from the per-tile partials of
+// the level's last residual launch.
+int reduce_norms(DLevel& L) {
+  const int nv = L.o.nvar, ncase = (int)R.cases.size();
   if (nv == 4) hipLaunchKernelGGL(k_norms<4>, dim3(ncase), dim3(256), 0, R.s, L.dpartials, L.dcase_tile0, L.dncell, L.dnorms);
   else if (nv == 5) hipLaunchKernelGGL(k_norms<5>, dim3(ncase), dim3(256), 0, R.s, L.dpartials, L.dcase_tile0, L.dncell, L.dnorms);
   else hipLaunchKernelGGL(k_norms<6>, dim3(ncase), dim3(256), 0, R.s, L.dpartials, L.dcase_tile0, L.dncell, L.dnorms);
@@ -1898,6 +1917,7 @@ int ec2d_res_norms(int case_id, int nMesh, double* res_max, double* res_rms) {
   if (case_id < 0 || case_id >= (int)R.cases.size() || nMesh < 1 || nMesh > R.num_mesh) return fail("bad case / mesh");
   DLevel& L = R.lv[nMesh - 1];
   double buf[12];
+  if (L.residual_done && reduce_norms(L)) return 1;
   HIPCHK(hipMemcpyAsync(buf, L.dnorms + 12 * case_id, sizeof(buf), hipMemcpyDeviceToHost, R.s));
   unsigned long long e = 0;
   HIPCHK(hipMemcpyAsync(&e, R.derr, sizeof(e), hipMemcpyDeviceToHost, R.s));
