@@ -54,7 +54,7 @@ struct SubfaceDesc {
   int nb_block;        // neighbour block index in the level (0-based), or -1
   int nb_face, nb_ist, nb_iend, nb_jst, nb_jend;
   int norm_off;        // offset of per-point outward normals (farfield / symmetry), -1 if none
-  int pad;
+  int sync;            // ghost fill: 1 = a barrier before this subface (it depends on the ones since the last)
 };
 
 struct TileDesc {

@@ -8,7 +8,9 @@
 // ghost cells that an earlier subface of the same block wrote (the point
 // loops include the subface end points, :325-327, :478-479), and interfaces
 // overwrite what boundary conditions wrote.  One workgroup per block keeps
-// that order with a barrier between dependent stages; blocks run in parallel
+// that order with a barrier before every subface that depends on the ones
+// since the last barrier (SubfaceDesc::sync, from the host's analysis of the
+// cells each subface reads and writes), and before the corners; blocks run in parallel
 // because every stage reads only the own block's cells, except the interface
 // copy, which reads neighbour INTERIOR cells that this kernel never writes.
 #include "ec2d_device.h"
@@ -177,19 +179,20 @@ __global__ __launch_bounds__(1024) void k_ghost(GhostArgs a) {
     for (int k = 0; k < B.nsub; k++) {
       const SubfaceDesc sf = a.subs[B.bc_off + k];
       if (sf.neighb >= 0) continue;
+      if (sf.sync) __syncthreads();
       int ni = sf.iend - sf.ist + 1, nj = sf.jend - sf.jst + 1;
       int npt = ni * nj;
       for (int p = tid; p < npt; p += blockDim.x) {
         int i = sf.ist + p % ni, j = sf.jst + p / ni;
         bc_point<NV>(a, B, cp, sf, i, j, p, U);
       }
-      __syncthreads();
     }
   }
   if (a.do_exchange) {
     for (int k = 0; k < B.nsub; k++) {
       const SubfaceDesc sf = a.subs[B.bc_off + k];
       if (sf.neighb < 0) continue;
+      if (sf.sync) __syncthreads();
       const BlockDesc B1 = a.blocks[sf.nb_block];
       const double* U1[NV];
 #pragma unroll
@@ -216,13 +219,13 @@ __global__ __launch_bounds__(1024) void k_ghost(GhostArgs a) {
       }
     }
     __syncthreads();
-    if (tid == 0) {
-      const int nx = B.nx, ny = B.ny;
-      corner<NV>(U, pidx(B, 1, 0), pidx(B, 1, 1), pidx(B, 0, 1), pidx(B, 0, 0), cp.Cv);
-      corner<NV>(U, pidx(B, 1, ny), pidx(B, 1, ny - 1), pidx(B, 0, ny - 1), pidx(B, 0, ny), cp.Cv);
-      corner<NV>(U, pidx(B, nx - 1, 0), pidx(B, nx - 1, 1), pidx(B, nx, 1), pidx(B, nx, 0), cp.Cv);
-      corner<NV>(U, pidx(B, nx - 1, ny), pidx(B, nx - 1, ny - 1), pidx(B, nx, ny - 1), pidx(B, nx, ny), cp.Cv);
-    }
+    // the four corners are independent: each writes its corner cell and
+    // reads three cells next to it, none of them a corner cell (nx, ny >= 2)
+    const int nx = B.nx, ny = B.ny;
+    if (tid == 0) corner<NV>(U, pidx(B, 1, 0), pidx(B, 1, 1), pidx(B, 0, 1), pidx(B, 0, 0), cp.Cv);
+    else if (tid == 64) corner<NV>(U, pidx(B, 1, ny), pidx(B, 1, ny - 1), pidx(B, 0, ny - 1), pidx(B, 0, ny), cp.Cv);
+    else if (tid == 128) corner<NV>(U, pidx(B, nx - 1, 0), pidx(B, nx - 1, 1), pidx(B, nx, 1), pidx(B, nx, 0), cp.Cv);
+    else if (tid == 192) corner<NV>(U, pidx(B, nx - 1, ny), pidx(B, nx - 1, ny - 1), pidx(B, nx, ny - 1), pidx(B, nx, ny), cp.Cv);
   }
 }
 

@@ -132,6 +132,72 @@ int fail(const std::string& m) {
     if (e_ != hipSuccess) return fail(std::string(#x) + ": " + hipGetErrorString(e_));       \
   } while (0)
 
+// Ghost-fill phases of one block (k_ghost): the fill runs the boundary
+// subfaces in file order, then the interfaces, then the corners.  A subface
+// joins the running phase (no barrier) when it writes no cell that the phase
+// read or wrote and reads no cell that the phase wrote; otherwise it starts a
+// new phase (sync = 1).  The cell sets follow bc_point and the interface copy
+// in ec2d_ghost.hip exactly; interfaces read neighbour interior cells, which
+// no fill writes, so only their writes count.  The corners always follow a
+// barrier.
+void ghost_phases(const BlockDesc& B, SubfaceDesc* s) {
+  std::vector<int> wph((size_t)B.plane, 0), rph((size_t)B.plane, 0);
+  int ph = 1;
+  std::vector<int64_t> rd, wr;
+  auto q = [&](int i, int j) { return pidx(B, i, j); };
+  auto join = [&](SubfaceDesc& sf) {
+    bool dep = false;
+    for (int64_t c : wr) dep = dep || c < 0 || c >= B.plane || wph[c] == ph || rph[c] == ph;
+    for (int64_t c : rd) dep = dep || c < 0 || c >= B.plane || wph[c] == ph;
+    sf.sync = dep ? 1 : 0;
+    if (dep) ph++;
+    for (int64_t c : wr) if (c >= 0 && c < B.plane) wph[c] = ph;
+    for (int64_t c : rd) if (c >= 0 && c < B.plane) rph[c] = ph;
+  };
+  for (int k = 0; k < B.nsub; k++) {
+    SubfaceDesc& sf = s[k];
+    if (sf.neighb >= 0) continue;
+    rd.clear();
+    wr.clear();
+    for (int j = sf.jst; j <= sf.jend; j++)
+      for (int i = sf.ist; i <= sf.iend; i++) {
+        const int f = sf.face;
+        if (sf.neighb == -10 || sf.neighb == -30) {
+          for (int l = 1; l <= EC2D_LAP; l++) {
+            if (f == 1) { rd.push_back(q(i + l - 1, j)); wr.push_back(q(i - l, j)); }
+            else if (f == 2) { rd.push_back(q(i, j + l - 1)); wr.push_back(q(i, j - l)); }
+            else if (f == 3) { rd.push_back(q(i - l, j)); wr.push_back(q(i + l - 1, j)); }
+            else { rd.push_back(q(i, j - l)); wr.push_back(q(i, j + l - 1)); }
+          }
+        } else {
+          rd.push_back(q(f == 3 ? i - 1 : i, f == 4 ? j - 1 : j));
+          for (int l = 1; l <= EC2D_LAP; l++) {
+            if (f == 1) wr.push_back(q(i - l, j));
+            else if (f == 2) wr.push_back(q(i, j - l));
+            else if (f == 3) wr.push_back(q(i + l - 1, j));
+            else wr.push_back(q(i, j + l - 1));
+          }
+        }
+      }
+    join(sf);
+  }
+  for (int k = 0; k < B.nsub; k++) {
+    SubfaceDesc& sf = s[k];
+    if (sf.neighb < 0) continue;
+    rd.clear();
+    wr.clear();
+    const int n1 = sf.iend - sf.ist + sf.jend - sf.jst;
+    for (int l = 1; l <= EC2D_LAP; l++)
+      for (int kk = 1; kk <= n1; kk++) {
+        if (sf.face == 1) wr.push_back(q(sf.ist - EC2D_LAP - 1 + l, sf.jst + kk - 1));
+        else if (sf.face == 2) wr.push_back(q(sf.ist + kk - 1, sf.jst - EC2D_LAP - 1 + l));
+        else if (sf.face == 3) wr.push_back(q(sf.iend + l - 1, sf.jst + kk - 1));
+        else wr.push_back(q(sf.ist + kk - 1, sf.jend + l - 1));
+      }
+    join(sf);
+  }
+}
+
 // LU-SGS record layout of a level for lpc lanes per cell (see ec2d_lusgs.hip);
 // upload: also refresh the device block table (after ec2d_commit).
 int lusgs_layout(DLevel& L, int lpc, int split, bool upload) {
@@ -1214,6 +1280,7 @@ int ec2d_commit(void) {
     HIPCHK(hipMemcpy(L.arena, img.data(), sizeof(double) * (size_t)off, hipMemcpyHostToDevice));
     HIPCHK(hipMalloc(&L.db, sizeof(BlockDesc) * L.hb.size()));
     HIPCHK(hipMemcpy(L.db, L.hb.data(), sizeof(BlockDesc) * L.hb.size(), hipMemcpyHostToDevice));
+    for (auto& b : L.hb) ghost_phases(b, subs.data() + b.bc_off);
     HIPCHK(hipMalloc(&L.ds, sizeof(SubfaceDesc) * std::max<size_t>(subs.size(), 1)));
     HIPCHK(hipMemcpy(L.ds, subs.data(), sizeof(SubfaceDesc) * subs.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMalloc(&L.dnorm, sizeof(double) * std::max<size_t>(norms.size(), 2)));
