@@ -204,6 +204,8 @@ def main():
     ap.add_argument("--tile-rows", type=int, default=4)
     ap.add_argument("--calibrate-copy", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--tune-cache", default="", help="autotune cache file: choices found there are reused without "
+                    "timing launches, new ones are appended (profiled runs see the same shapes as the timed run)")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy()
@@ -230,7 +232,7 @@ def main():
             b.init_flow()
             # launch shapes per level (residual tile rows, LU-SGS lanes per
             # cell), timed on this case before the measured run
-            tuning = b.autotune(reps=5) if not args.no_tune else "off"
+            tuning = b.autotune(args.tune_cache, reps=5) if not args.no_tune else "off"
             b.run(max(args.warmup, 1))
             b.synchronize()
             t0 = time.perf_counter()
