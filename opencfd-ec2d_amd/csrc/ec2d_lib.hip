@@ -230,7 +230,7 @@ int lusgs_layout(DLevel& L, int lpc, int split, bool upload) {
     sk_off += (per * d.sk_n + 31) / 32 * 32;
     L.sk_max_n = std::max(L.sk_max_n, d.sk_n);
     L.max_cells = std::max(L.max_cells, ncol * nrow);
-    L.prep_tiles = std::max(L.prep_tiles, ((ncol + 31) / 32) * ((nrow + 7) / 8));
+    L.prep_tiles = std::max(L.prep_tiles, ((d.sk_S + 31) / 32) * ((nrow + 7) / 8));
   }
   if (L.sk) HIPCHK(hipFree(L.sk));
   L.sk = nullptr;
@@ -537,9 +537,9 @@ int launch_lusgs(int level, const int* dlist, int nblk, double alfa1, int update
   a.gstride = L.gstride;
   a.err = R.dsweep_err;
   bool visc = R.cases[0].cp.If_viscous == 1;
-  // prep grid: tiles of 32 x 8 cells of the largest block
+  // prep and scatter grid: tiles of 32 steps x 8 rows of the largest block's skewed space
   dim3 gprep((unsigned)L.prep_tiles, nblk), gsweep(nblk * L.lusgs_split), blk(L.lusgs_threads);
-  dim3 gscat((unsigned)((L.max_cells + 255) / 256), nblk);
+  dim3 gscat((unsigned)L.prep_tiles, nblk);
   int nv = L.o.nvar;
   // the instantiation is chosen by thread count (its launch bound): up to 4
   // waves one wave per SIMD and two steps per register set; more waves (up

@@ -293,11 +293,17 @@ __global__ __launch_bounds__(256) void k_lusgs_prep(LusgsArgs a) {
   const BlockDesc B = a.blocks[a.block_list[blockIdx.y]];
   const int64_t n = B.sk_n;
   const int ncol = B.nx - 1, nrow = B.ny - 1;
-  const int tiles_i = (ncol + 31) / 32;
+  // tiles of the skewed (step, row) space: a workgroup covers 32 steps x 8
+  // rows, a wave 8 x 8, eight lanes one step; its cells (i-1, j-1) =
+  // (s - r, r) read 8 consecutive cells of 8 rows, and every record store of
+  // the eight lanes of a step fills 8 consecutive positions of both sets
+  const int S = B.sk_S;
+  const int tiles_s = (S + 31) / 32;
   const int tile = blockIdx.x;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int c = (tile % tiles_i) * 32 + wave * 8 + (lane & 7);  // i - 1
-  const int r = (tile / tiles_i) * 8 + (lane >> 3);              // j - 1
+  const int s = (tile % tiles_s) * 32 + wave * 8 + (lane >> 3);  // step
+  const int r = (tile / tiles_s) * 8 + (lane & 7);               // j - 1
+  const int c = s - r;                                           // i - 1
   const double g = a.cps[B.case_id].gamma;
   const double* base = a.arena + B.base;
   double* fw = a.sk + B.sk_base;
@@ -309,7 +315,7 @@ __global__ __launch_bounds__(256) void k_lusgs_prep(LusgsArgs a) {
 #pragma unroll
     for (int d = 0; d < 2 * F::XP; d++) fw[(int64_t)(d / 2) * 2 * n + 2 * pad + (d & 1)] = -0.0;
   }
-  if (c >= ncol || r >= nrow) return;
+  if (s >= S || r >= nrow || c < 0 || c >= ncol) return;  // padding positions stay unwritten
   // the cell's forward record at (c, r), its backward record at (ncol-1-c, nrow-1-r)
   const int cb = ncol - 1 - c, rb = nrow - 1 - r;
   write_rec<NV, VISC, true>(fw, n, (int64_t)(c + r) * R + r, B, base, a.pm, c, r, g, a.alfa1);
@@ -692,12 +698,19 @@ __global__ __launch_bounds__(256) void k_lusgs_scatter(LusgsArgs a) {
   using F = RecF<NV, VISC>;
   const BlockDesc B = a.blocks[a.block_list[blockIdx.y]];
   const int ncol = B.nx - 1, nrow = B.ny - 1;
-  const int cidx = blockIdx.x * 256 + threadIdx.x;
-  if (cidx >= ncol * nrow) return;
-  const int j = cidx / ncol + 1, i = cidx - (j - 1) * ncol + 1;
+  // the tiles of k_lusgs_prep in the backward set's (step, row) space: eight
+  // lanes read 8 consecutive output positions, a wave writes 8 cells of 8 rows
+  const int S = B.sk_S;
+  const int tiles_s = (S + 31) / 32;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int sb = (blockIdx.x % tiles_s) * 32 + wave * 8 + (lane >> 3);
+  const int rb = (blockIdx.x / tiles_s) * 8 + (lane & 7);
+  const int cb = sb - rb;
+  if (sb >= S || rb >= nrow || cb < 0 || cb >= ncol) return;
+  const int i = ncol - cb, j = nrow - rb;
   const int64_t n = B.sk_n;
   // the backward record of cell (i, j): position (ncol-i + nrow-j) * R + nrow-j
-  const int64_t pos = (int64_t)(ncol - i + nrow - j) * B.sk_R + (nrow - j);
+  const int64_t pos = (int64_t)sb * B.sk_R + rb;
   const double* out = a.sk + B.sk_base + 2 * F::NREC * n + 2 * pos;
   double* base = a.arena + B.base;
   const int64_t q = pidx(B, i, j);
