@@ -10,7 +10,9 @@
 // overwrite what boundary conditions wrote.  One workgroup per block keeps
 // that order with a barrier before every subface that depends on the ones
 // since the last barrier (SubfaceDesc::sync, from the host's analysis of the
-// cells each subface reads and writes), and before the corners; blocks run in parallel
+// cells each subface reads and writes), and before the corners; between
+// barriers the points of all subfaces form one index space over the
+// workgroup's threads; blocks run in parallel
 // because every stage reads only the own block's cells, except the interface
 // copy, which reads neighbour INTERIOR cells that this kernel never writes.
 #include "ec2d_device.h"
@@ -22,6 +24,7 @@ struct GhostArgs {
   const BlockDesc* blocks;
   const int* block_list;
   const SubfaceDesc* subs;
+  const int* units;       // per block (at bc_off): its subfaces in fill order, boundary conditions first
   const double* normals;  // per boundary point (n1, n2)
   const CaseParams* cps;
   double* arena;
@@ -175,32 +178,41 @@ __global__ __launch_bounds__(1024) void k_ghost(GhostArgs a) {
 #pragma unroll
   for (int m = 0; m < NV; m++) U[m] = a.arena + B.base + (int64_t)(a.pm.U + m) * B.plane;
   const int tid = threadIdx.x;
-  if (a.do_bc) {
-    for (int k = 0; k < B.nsub; k++) {
-      const SubfaceDesc sf = a.subs[B.bc_off + k];
-      if (sf.neighb >= 0) continue;
-      if (sf.sync) __syncthreads();
-      int ni = sf.iend - sf.ist + 1, nj = sf.jend - sf.jst + 1;
-      int npt = ni * nj;
-      for (int p = tid; p < npt; p += blockDim.x) {
-        int i = sf.ist + p % ni, j = sf.jst + p / ni;
-        bc_point<NV>(a, B, cp, sf, i, j, p, U);
+  // Phases: a unit with sync = 1 starts a new one.  The points of all units
+  // of a phase are independent of each other, so the threads take them as
+  // one index space (one memory round trip per phase, not per subface).
+  auto on = [&](const SubfaceDesc& sf) { return sf.neighb < 0 ? a.do_bc != 0 : a.do_exchange != 0; };
+  auto npoints = [&](const SubfaceDesc& sf) {
+    if (!on(sf)) return 0;
+    if (sf.neighb < 0) return (sf.iend - sf.ist + 1) * (sf.jend - sf.jst + 1);
+    return (sf.iend - sf.ist + sf.jend - sf.jst) * EC2D_LAP;
+  };
+  const int* un = a.units + B.bc_off;
+  for (int u0 = 0; u0 < B.nsub;) {
+    int u1 = u0 + 1;
+    while (u1 < B.nsub && !a.subs[un[u1]].sync) u1++;
+    int total = 0;
+    for (int u = u0; u < u1; u++) total += npoints(a.subs[un[u]]);
+    for (int p = tid; p < total; p += blockDim.x) {
+      int u = u0, lp = p;
+      for (;;) {
+        const int np = npoints(a.subs[un[u]]);
+        if (lp < np) break;
+        lp -= np;
+        u++;
       }
-    }
-  }
-  if (a.do_exchange) {
-    for (int k = 0; k < B.nsub; k++) {
-      const SubfaceDesc sf = a.subs[B.bc_off + k];
-      if (sf.neighb < 0) continue;
-      if (sf.sync) __syncthreads();
-      const BlockDesc B1 = a.blocks[sf.nb_block];
-      const double* U1[NV];
+      const SubfaceDesc sf = a.subs[un[u]];
+      if (sf.neighb < 0) {
+        const int ni = sf.iend - sf.ist + 1;
+        bc_point<NV>(a, B, cp, sf, sf.ist + lp % ni, sf.jst + lp / ni, lp, U);
+      } else {
+        const BlockDesc B1 = a.blocks[sf.nb_block];
+        const double* U1[NV];
 #pragma unroll
-      for (int m = 0; m < NV; m++) U1[m] = a.arena + B1.base + (int64_t)(a.pm.U + m) * B1.plane;
-      int n1 = sf.iend - sf.ist + sf.jend - sf.jst;
-      for (int w = tid; w < n1 * EC2D_LAP; w += blockDim.x) {
-        int kk = w % n1 + 1, l = w / n1 + 1;  // cell along the face (1..n1), layer (1..LAP)
-        int k1 = (sf.orient == 2) ? kk : n1 + 1 - kk;
+        for (int m = 0; m < NV; m++) U1[m] = a.arena + B1.base + (int64_t)(a.pm.U + m) * B1.plane;
+        const int n1 = sf.iend - sf.ist + sf.jend - sf.jst;
+        const int kk = lp % n1 + 1, l = lp / n1 + 1;  // cell along the face (1..n1), layer (1..LAP)
+        const int k1 = (sf.orient == 2) ? kk : n1 + 1 - kk;
         // source cell: slot (k1, l) of the neighbour's buffer (sub_boundary.f90:76-108)
         int si_, sj_;
         if (sf.nb_face == 1) { si_ = sf.nb_ist + l - 1; sj_ = sf.nb_jst + k1 - 1; }
@@ -213,11 +225,15 @@ __global__ __launch_bounds__(1024) void k_ghost(GhostArgs a) {
         else if (sf.face == 2) { di = sf.ist + kk - 1; dj = sf.jst - EC2D_LAP - 1 + l; }
         else if (sf.face == 3) { di = sf.iend + l - 1; dj = sf.jst + kk - 1; }
         else { di = sf.ist + kk - 1; dj = sf.jend + l - 1; }
-        int64_t qs = pidx(B1, si_, sj_), qd = pidx(B, di, dj);
+        const int64_t qs = pidx(B1, si_, sj_), qd = pidx(B, di, dj);
 #pragma unroll
         for (int m = 0; m < NV; m++) U[m][qd] = U1[m][qs];
       }
     }
+    u0 = u1;
+    if (u0 < B.nsub) __syncthreads();
+  }
+  if (a.do_exchange) {
     __syncthreads();
     // the four corners are independent: each writes its corner cell and
     // reads three cells next to it, none of them a corner cell (nx, ny >= 2)

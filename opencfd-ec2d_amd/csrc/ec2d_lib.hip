@@ -50,6 +50,7 @@ struct DLevel {
   std::vector<int> blk_case, blk_local;
   BlockDesc* db = nullptr;
   SubfaceDesc* ds = nullptr;
+  int* dunits = nullptr;  // ghost fill order per block: subface indices, boundary conditions first
   double* dnorm = nullptr;
   std::vector<TileDesc> ht;
   TileDesc* dt = nullptr;
@@ -133,19 +134,22 @@ int fail(const std::string& m) {
   } while (0)
 
 // Ghost-fill phases of one block (k_ghost): the fill runs the boundary
-// subfaces in file order, then the interfaces, then the corners.  A subface
+// subfaces in file order, then the interfaces, then the corners; units gets
+// that order (level subface indices).  A subface
 // joins the running phase (no barrier) when it writes no cell that the phase
 // read or wrote and reads no cell that the phase wrote; otherwise it starts a
 // new phase (sync = 1).  The cell sets follow bc_point and the interface copy
 // in ec2d_ghost.hip exactly; interfaces read neighbour interior cells, which
 // no fill writes, so only their writes count.  The corners always follow a
 // barrier.
-void ghost_phases(const BlockDesc& B, SubfaceDesc* s) {
+void ghost_phases(const BlockDesc& B, SubfaceDesc* s, int* units) {
   std::vector<int> wph((size_t)B.plane, 0), rph((size_t)B.plane, 0);
   int ph = 1;
   std::vector<int64_t> rd, wr;
   auto q = [&](int i, int j) { return pidx(B, i, j); };
+  int nu = 0;
   auto join = [&](SubfaceDesc& sf) {
+    units[nu++] = B.bc_off + (int)(&sf - s);
     bool dep = false;
     for (int64_t c : wr) dep = dep || c < 0 || c >= B.plane || wph[c] == ph || rph[c] == ph;
     for (int64_t c : rd) dep = dep || c < 0 || c >= B.plane || wph[c] == ph;
@@ -587,6 +591,7 @@ int launch_ghost(int level, int do_bc, int do_exchange) {
   a.blocks = L.db;
   a.block_list = L.dblocks_all;
   a.subs = L.ds;
+  a.units = L.dunits;
   a.normals = L.dnorm;
   a.cps = R.dcps;
   a.arena = L.arena;
@@ -1006,7 +1011,7 @@ int ec2d_finalize(void) {
   hipStreamSynchronize(R.s);
   if (R.graph) hipGraphExecDestroy(R.graph);
   for (auto& L : R.lv) {
-    hipFree(L.arena); hipFree(L.db); hipFree(L.ds); hipFree(L.dnorm); hipFree(L.dt); hipFree(L.dperm);
+    hipFree(L.arena); hipFree(L.db); hipFree(L.ds); hipFree(L.dunits); hipFree(L.dnorm); hipFree(L.dt); hipFree(L.dperm);
     hipFree(L.dcase_tile0); hipFree(L.dncell); hipFree(L.dpartials); hipFree(L.dnorms); hipFree(L.drows);
     hipFree(L.dblocks_all); hipFree(L.sk); hipFree(L.dwall); hipFree(L.dcase_wall0); hipFree(L.dforces);
     hipFree(L.dgring);
@@ -1280,7 +1285,10 @@ int ec2d_commit(void) {
     HIPCHK(hipMemcpy(L.arena, img.data(), sizeof(double) * (size_t)off, hipMemcpyHostToDevice));
     HIPCHK(hipMalloc(&L.db, sizeof(BlockDesc) * L.hb.size()));
     HIPCHK(hipMemcpy(L.db, L.hb.data(), sizeof(BlockDesc) * L.hb.size(), hipMemcpyHostToDevice));
-    for (auto& b : L.hb) ghost_phases(b, subs.data() + b.bc_off);
+    std::vector<int> units(std::max<size_t>(subs.size(), 1), 0);
+    for (auto& b : L.hb) ghost_phases(b, subs.data() + b.bc_off, units.data() + b.bc_off);
+    HIPCHK(hipMalloc(&L.dunits, sizeof(int) * units.size()));
+    HIPCHK(hipMemcpy(L.dunits, units.data(), sizeof(int) * units.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMalloc(&L.ds, sizeof(SubfaceDesc) * std::max<size_t>(subs.size(), 1)));
     HIPCHK(hipMemcpy(L.ds, subs.data(), sizeof(SubfaceDesc) * subs.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMalloc(&L.dnorm, sizeof(double) * std::max<size_t>(norms.size(), 2)));
