@@ -30,6 +30,8 @@ typedef struct {
   int If_viscous, Iflag_turbulence_model, Iflag_init, Iflag_local_dt, Time_Method;
   int Iflag_Scheme, Iflag_Flux, IFlag_Reconstruction, Num_Mesh, Nvar;
   int Pre_Step_Mesh[3];
+  int Nstep_Inner_Limit;  /* dual-time stepping (Time_Method -1): inner iterations at most */
+  double Res_Inner_Limit; /* and the max_m Res_rms(m) that ends them (opencfd_ec2d_v1.5.4.f90:209) */
 } ec2d_control;
 
 /* array selectors for ec2d_put_array / ec2d_get_array (reference bounds) */
@@ -43,6 +45,7 @@ enum {
   EC2D_UN = 23,    /* Un(NVun,0:nx,0:ny) */
   EC2D_QF = 24,    /* QF(4,1:nx-1,1:ny-1) */
   EC2D_DELTU = 25, /* deltU(4,-1:nx+1,-1:ny+1) */
+  EC2D_UN1 = 31,   /* Un1(NV,0:nx,0:ny), dual-time stepping only (interior set) */
   EC2D_CLOSURE_MUT = 40,  /* learned closure: mu_t of the last inference (-1:nx+1,-1:ny+1; SA cases: the
                              interior, model-4 cases: 0:nx,0:ny) */
   EC2D_CLOSURE_MASK = 41  /* learned closure: 1 where the features lay inside the training range */
@@ -135,6 +138,12 @@ int ec2d_res_norms(int case_id, int nMesh, double *res_max, double *res_rms);
 
 /* Kstep and tt of a case's mesh level (Mesh_TYPE, opencfd_ec2d_v1.5.4.f90:88-90) */
 int ec2d_time(int case_id, int nMesh, int *kstep, double *tt);
+
+/* Dual-time stepping (Time_Method -1): the inner-loop index kt_in of the last
+ * physical step, the value Dual_time_LU_SGS prints (sub_NS_singlegrid.f90:277;
+ * Nstep_Inner_Limit + 1 when the inner loop ran out).  No reference
+ * counterpart as an interface: the reference only prints it. */
+int ec2d_dual_inner(int *kt_in);
 
 int ec2d_synchronize(void);
 

@@ -45,7 +45,9 @@ struct BlockDesc {
 // plane indices within a block (computed per level from nvar)
 struct PlaneMap {
   int U, Res, dU, dt, Lci, Lcj, Lvi, Lvj, vol, si, sj, ni1, ni2, nj1, nj2, x1, y1, Amu, Amu_t, dw,
-      Un, QF, deltU, Akx, Aky, Aix, Aiy, Ajac, flag, fx, fy, ev, sst, clo, nplanes;
+      Un, QF, deltU, Akx, Aky, Aix, Aiy, Ajac, flag, fx, fy, ev, sst, clo,
+      Un1,  // U(n-1) of dual-time stepping (fine level, Time_Method -1), else -1
+      nplanes;
 };
 
 // One subface of bc2d_1.in, resolved for the device.

@@ -97,6 +97,12 @@ struct Runtime {
   int num_mesh = 1;
   int tile_rows = 4;
   bool clip_pending[3] = {false, false, false};
+  // dual-time stepping: the reference's first-call flags (integer, save) of
+  // Comput_Residual_one_mesh (Sfac, sub_Residual.f90:18,28-34) and of
+  // Dual_time_LU_SGS (alfa1, sub_NS_singlegrid.f90:241-247), counted from
+  // ec2d_init_flow; the last step's inner-loop index as the reference prints it
+  long long residual_calls = 0, dual_steps = 0;
+  int dual_kt = 0;
   hipGraphExec_t graph = nullptr;
   bool graph_ok = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -321,7 +327,7 @@ int build_tiles(DLevel& L) {
 
 // clo: the learned closure's planes (SA cases, and cases whose eddy viscosity
 // the closure supplies)
-PlaneMap make_planes(int nv, int nv_un, bool fdm_any, bool clo) {
+PlaneMap make_planes(int nv, int nv_un, bool fdm_any, bool clo, bool dual) {
   PlaneMap p{};
   int k = 0;
   p.U = k; k += nv;
@@ -353,6 +359,12 @@ PlaneMap make_planes(int nv, int nv_un, bool fdm_any, bool clo) {
     k += 2;
   } else {
     p.clo = -1;
+  }
+  if (dual) {
+    p.Un1 = k;  // dual-time stepping: U(n-1)
+    k += nv;
+  } else {
+    p.Un1 = -1;
   }
   p.nplanes = k;
   return p;
@@ -620,6 +632,7 @@ int launch_pw(int level, int op, int krk) {
   a.nvar = L.o.nvar;
   a.op = op;
   a.krk = krk;
+  a.sfac = 0.0;
   const double Ra[3] = {1.0, 3.0 / 4.0, 1.0 / 3.0}, Rb[3] = {1.0, 1.0 / 4.0, 2.0 / 3.0}, Rg[3] = {0.0, 1.0 / 4.0, 2.0 / 3.0};
   if (krk >= 1) {
     a.ra = Ra[krk - 1];
@@ -678,6 +691,29 @@ int residual(int level) {
   R.clip_pending[level] = false;
   prof_mark(1 + 10 * level);
   int rc = launch_residual(level, clip);
+  if (!rc && R.cases[0].ctl.Time_Method == -1) {
+    // dual-time source after the whole residual and before the norms:
+    // Sfac = 1/(3 dt) on the first residual call, 1/(2 dt) afterwards
+    DLevel& L = R.lv[level];
+    const double dtg = R.cases[0].ctl.dt_global;
+    PwArgs a;
+    a.blocks = L.db;
+    a.rows = L.drows;
+    a.arena = L.arena;
+    a.pm = L.pm;
+    a.nvar = L.o.nvar;
+    a.op = PW_DUAL_SRC;
+    a.krk = 0;
+    a.ra = a.rb = a.rg = 0.0;
+    a.sfac = R.residual_calls == 0 ? 1.0 / (3.0 * dtg) : 1.0 / (2.0 * dtg);
+    hipLaunchKernelGGL(k_pointwise, dim3(L.nrows), dim3(256), 0, R.s, a);
+    const int nv = L.o.nvar;
+    if (nv == 4) hipLaunchKernelGGL(k_tile_norms<4>, dim3(L.ntiles), dim3(256), 0, R.s, L.dt, L.db, L.arena, L.pm, L.dpartials, L.tile_rows);
+    else if (nv == 5) hipLaunchKernelGGL(k_tile_norms<5>, dim3(L.ntiles), dim3(256), 0, R.s, L.dt, L.db, L.arena, L.pm, L.dpartials, L.tile_rows);
+    else hipLaunchKernelGGL(k_tile_norms<6>, dim3(L.ntiles), dim3(256), 0, R.s, L.dt, L.db, L.arena, L.pm, L.dpartials, L.tile_rows);
+    rc = hipGetLastError() == hipSuccess ? 0 : fail("dual-time source launch failed");
+  }
+  R.residual_calls++;
   prof_mark(-(1 + 10 * level));
   return rc;
 }
@@ -728,13 +764,46 @@ int advance_rk3(int level) {
   return 0;
 }
 
+// Dual_time_LU_SGS (sub_NS_singlegrid.f90:231-310): inner LU-SGS iterations
+// with alfa1 = 1/dt (first physical step) or 3/(2 dt), until max_m Res_rms(m)
+// <= Res_Inner_Limit or Nstep_Inner_Limit iterations; then Un1 = Un, Un = U.
+// The exit test reads the norms on the host after every inner iteration.
+int dual_time_lusgs() {
+  DLevel& L = R.lv[0];
+  const ec2d_control& k = R.cases[0].ctl;
+  const double dtg = k.dt_global;
+  const double alfa1 = R.dual_steps == 0 ? 1.0 / dtg : 3.0 / (2.0 * dtg);
+  R.dual_steps++;
+  int kt = 1;
+  for (; kt <= k.Nstep_Inner_Limit; kt++) {
+    if (residual(0)) return 1;
+    prof_mark(2);
+    if (launch_lusgs(0, L.dblocks_all, (int)L.hb.size(), alfa1, 1)) return 1;
+    prof_mark(-2);
+    prof_mark(3);
+    if (launch_ghost(0, 1, 1)) return 1;
+    prof_mark(-3);
+    double mx[6], rms[6];
+    const int rc = ec2d_res_norms(0, 1, mx, rms);
+    if (rc) return rc;
+    double mr = rms[0];
+    for (int m = 0; m < L.o.nvar; m++) mr = std::max(mr, rms[m]);
+    if (mr <= k.Res_Inner_Limit) break;
+  }
+  R.dual_kt = kt;  // Nstep_Inner_Limit + 1 when the loop ran out, as the reference prints it
+  if (launch_pw(0, PW_DUAL_SHIFT)) return 1;
+  tick(0);
+  return 0;
+}
+
 int time_advance(int level) {
   int tm = R.cases[0].ctl.Time_Method;
   int rc;
   if (tm == 0) rc = advance_lusgs(level, false);
   else if (tm == 1) rc = advance_euler1(level);
   else if (tm == 3) rc = advance_rk3(level);
-  else return fail("time method not on this path (dual-time LU-SGS is out of scope)");
+  else if (tm == -1 && level == 0) rc = dual_time_lusgs();
+  else return fail("time method not on this path");
   if (rc) return rc;
   if (R.lv[level].o.nvar >= 5) R.clip_pending[level] = true;
   return 0;
@@ -878,6 +947,7 @@ int array_map(const DLevel& L, int which, int* slot, int* ncomp, int* kind) {
     case EC2D_UN: *slot = p.Un; *ncomp = L.o.nvar_un; *kind = 3; return 0;
     case EC2D_QF: *slot = p.QF; *ncomp = 4; *kind = 2; return 0;
     case EC2D_DELTU: *slot = p.deltU; *ncomp = 4; *kind = 0; return 0;
+    case EC2D_UN1: if (p.Un1 < 0) return 1; *slot = p.Un1; *ncomp = L.o.nvar; *kind = 3; return 0;
     case EC2D_CLOSURE_MUT: if (p.clo < 0) return 1; *slot = p.clo; *ncomp = 1; *kind = 5; return 0;
     case EC2D_CLOSURE_MASK: if (p.clo < 0) return 1; *slot = p.clo + 1; *ncomp = 1; *kind = 5; return 0;
   }
@@ -1113,6 +1183,15 @@ int ec2d_commit(void) {
   if (R.cases.empty()) return fail("no case defined");
   if (R.committed) return fail("already committed");
   R.num_mesh = R.cases[0].num_mesh;
+  if (R.cases[0].ctl.Time_Method == -1) {
+    // Dual_time_LU_SGS: one mesh (opencfd_ec2d_v1.5.4.f90:216-221); the inner
+    // loop ends per case, so a batch holds one case; pre-steps would run
+    // dual-time steps before Init_flow sets Un and Un1 (sub_init.f90:396)
+    if (R.num_mesh != 1) return fail("dual-time LU-SGS needs Num_Mesh = 1");
+    if (R.cases.size() != 1) return fail("dual-time LU-SGS runs one case per batch");
+    if (R.cases[0].ctl.Pre_Step_Mesh[0] != 0) return fail("dual-time LU-SGS with pre-steps is not supported");
+    if (R.cases[0].ctl.Nstep_Inner_Limit < 1) return fail("dual-time LU-SGS needs Nstep_Inner_Limit >= 1");
+  }
   for (auto& c : R.cases)
     for (int l = 0; l < c.num_mesh; l++) {
       if (c.levels[l].empty()) return fail("mesh level without blocks");
@@ -1145,7 +1224,7 @@ int ec2d_commit(void) {
     L.o.turb = (l == 0) ? c0.Iflag_turbulence_model : 0;
     L.fdm = fdm_any;
     const int turb0 = c0.Iflag_turbulence_model;
-    L.pm = make_planes(nv, nv, fdm_any, l == 0 && (turb0 == 2 || turb0 == 4));
+    L.pm = make_planes(nv, nv, fdm_any, l == 0 && (turb0 == 2 || turb0 == 4), l == 0 && c0.Time_Method == -1);
     // blocks
     int64_t off = 0;
     std::vector<SubfaceDesc> subs;
@@ -1412,9 +1491,14 @@ int ec2d_init_flow(void) {
   if (check_ready()) return 1;
   HIPCHK(hipStreamSynchronize(R.s));  // the plane fills below copy on the null stream
   const ec2d_control& c0 = R.cases[0].ctl;
+  R.residual_calls = 0;
+  R.dual_steps = 0;
+  R.dual_kt = 0;
+  // Init_flow ends with Un = Un1 = U for dual-time stepping (sub_init.f90:396-409)
+  auto dual_init = [&]() { return c0.Time_Method == -1 ? launch_pw(0, PW_DUAL_INIT) : 0; };
   if (c0.Iflag_init != 0) {
     if (launch_ghost(0, 1, 1)) return 1;
-    return 0;
+    return dual_init();
   }
   // init_flow_zero (sub_init.f90:423-495)
   int nm = R.num_mesh;
@@ -1457,6 +1541,12 @@ int ec2d_init_flow(void) {
       if (launch_ghost(n - 2, 1, 1)) return 1;
     }
   }
+  return dual_init();
+}
+
+int ec2d_dual_inner(int* kt_in) {
+  if (check_ready()) return 1;
+  if (kt_in) *kt_in = R.dual_kt;
   return 0;
 }
 
@@ -1550,7 +1640,8 @@ int ec2d_run(int n_iter) {
   if (n_iter <= 0) return 0;
   R.res_ms = R.lusgs_ms = R.ghost_ms = 0.0;
   HIPCHK(hipEventRecord(R.ev0, R.s));
-  if (R.profile) {
+  // dual-time stepping reads the norms on the host inside every step: eager
+  if (R.profile || R.cases[0].ctl.Time_Method == -1) {
     for (auto e : R.pev) hipEventDestroy(e);
     R.pev.clear();
     R.pev_kind.clear();

@@ -19,7 +19,8 @@
 
 namespace ec2d {
 
-enum PwOp { PW_UPDATE, PW_CLIP, PW_EULER1, PW_COPY_UN, PW_RK3, PW_FORCE, PW_ADD_FORCE, PW_DELT_U, PW_SET_UN, PW_NEW_U };
+enum PwOp { PW_UPDATE, PW_CLIP, PW_EULER1, PW_COPY_UN, PW_RK3, PW_FORCE, PW_ADD_FORCE, PW_DELT_U, PW_SET_UN, PW_NEW_U,
+            PW_DUAL_SRC, PW_DUAL_INIT, PW_DUAL_SHIFT };
 
 struct PwArgs {
   const BlockDesc* blocks;
@@ -28,6 +29,7 @@ struct PwArgs {
   PlaneMap pm;
   int nvar, op, krk;
   double ra, rb, rg;  // RK3 stage coefficients
+  double sfac;        // dual-time source factor of this residual call
 };
 
 __global__ __launch_bounds__(256) void k_pointwise(PwArgs a) {
@@ -104,6 +106,29 @@ __global__ __launch_bounds__(256) void k_pointwise(PwArgs a) {
       for (int i = 1 + threadIdx.x; i <= nx - 1; i += blockDim.x) {
         int64_t q = pidx(B, i, j);
         for (int m = 0; m < 4; m++) PL(pm.Res + m)[q] = PL(pm.Res + m)[q] + PL(pm.QF + m)[q];
+      }
+      break;
+    case PW_DUAL_SRC:
+      // dual-time source (sub_Residual.f90:55-64): Res - (3 U - 4 Un + Un1) vol Sfac,
+      // evaluated left to right
+      if (!interior_row) return;
+      for (int i = 1 + threadIdx.x; i <= nx - 1; i += blockDim.x) {
+        int64_t q = pidx(B, i, j);
+        for (int m = 0; m < a.nvar; m++) {
+          double t = (3.0 * PL(pm.U + m)[q] - 4.0 * PL(pm.Un + m)[q]) + PL(pm.Un1 + m)[q];
+          PL(pm.Res + m)[q] = PL(pm.Res + m)[q] - (t * PL(pm.vol)[q]) * a.sfac;
+        }
+      }
+      break;
+    case PW_DUAL_INIT:  // Init_flow: Un = Un1 = U (sub_init.f90:396-409)
+    case PW_DUAL_SHIFT:  // end of a physical step: Un1 = Un, Un = U (sub_NS_singlegrid.f90:282-299)
+      if (!interior_row) return;
+      for (int i = 1 + threadIdx.x; i <= nx - 1; i += blockDim.x) {
+        int64_t q = pidx(B, i, j);
+        for (int m = 0; m < a.nvar; m++) {
+          PL(pm.Un1 + m)[q] = a.op == PW_DUAL_INIT ? PL(pm.U + m)[q] : PL(pm.Un + m)[q];
+          PL(pm.Un + m)[q] = PL(pm.U + m)[q];
+        }
       }
       break;
     case PW_NEW_U:

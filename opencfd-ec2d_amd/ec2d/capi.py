@@ -16,6 +16,7 @@ EC2D_U, EC2D_RES, EC2D_DT, EC2D_DU = 1, 2, 3, 4
 EC2D_LCI, EC2D_LCJ, EC2D_LVI, EC2D_LVJ = 5, 6, 7, 8
 EC2D_AMU, EC2D_AMU_T = 9, 10
 EC2D_UN, EC2D_QF, EC2D_DELTU = 23, 24, 25
+EC2D_UN1 = 31
 EC2D_CLOSURE_MUT, EC2D_CLOSURE_MASK = 40, 41
 CLOSURE_NPARAM = 6 * 64 + 64 + 64 * 64 + 64 + 64 * 64 + 64 + 64 + 1  # W1 b1 W2 b2 W3 b3 W4 b4
 CLOSURE_NNORM = 24  # mean, inv_std, lo, hi of the 6 features
@@ -27,7 +28,8 @@ class Control(C.Structure):
                                           "Wt_inf")] + \
                [(n, C.c_int) for n in ("If_viscous", "Iflag_turbulence_model", "Iflag_init", "Iflag_local_dt",
                                        "Time_Method", "Iflag_Scheme", "Iflag_Flux", "IFlag_Reconstruction",
-                                       "Num_Mesh", "Nvar")] + [("Pre_Step_Mesh", C.c_int * 3)]
+                                       "Num_Mesh", "Nvar")] + [("Pre_Step_Mesh", C.c_int * 3)] + \
+               [("Nstep_Inner_Limit", C.c_int), ("Res_Inner_Limit", C.c_double)]
 
 
 # exported entry points and their argument types
@@ -62,6 +64,7 @@ SIGNATURES = {
     "ec2d_run": [_i],
     "ec2d_res_norms": [_i, _i, _dp, _dp],
     "ec2d_time": [_i, _i, _ip, _dp],
+    "ec2d_dual_inner": [_ip],
     "ec2d_synchronize": [],
     "ec2d_num_cells": [_i, _i],
     "ec2d_timing": [_dp, _dp, _dp, _dp],

@@ -76,6 +76,7 @@ class Case:
         k.Num_Mesh, k.Nvar = self.num_mesh, c.nvar
         for n in range(3):
             k.Pre_Step_Mesh[n] = int(c.Pre_Step_Mesh[n])
+        k.Nstep_Inner_Limit, k.Res_Inner_Limit = int(c.Nstep_Inner_Limit), float(c.Res_Inner_Limit)
         return k
 
     def read_restart(self) -> List[np.ndarray]:
@@ -144,6 +145,7 @@ class Batch:
         capi.EC2D_AMU: lambda B, nv: (B.ny + 3, B.nx + 3),
         capi.EC2D_AMU_T: lambda B, nv: (B.ny + 3, B.nx + 3),
         capi.EC2D_UN: lambda B, nv: (B.ny + 1, B.nx + 1, nv),
+        capi.EC2D_UN1: lambda B, nv: (B.ny + 1, B.nx + 1, nv),
         capi.EC2D_QF: lambda B, nv: (B.ny - 1, B.nx - 1, 4),
         capi.EC2D_DELTU: lambda B, nv: (B.ny + 3, B.nx + 3, 4),
         capi.EC2D_CLOSURE_MUT: lambda B, nv: (B.ny + 3, B.nx + 3),
@@ -151,7 +153,8 @@ class Batch:
     }
     NAMES = {"U": capi.EC2D_U, "Res": capi.EC2D_RES, "dt": capi.EC2D_DT, "dU": capi.EC2D_DU, "Lci": capi.EC2D_LCI,
              "Lcj": capi.EC2D_LCJ, "Lvi": capi.EC2D_LVI, "Lvj": capi.EC2D_LVJ, "Amu": capi.EC2D_AMU,
-             "Amu_t": capi.EC2D_AMU_T, "Un": capi.EC2D_UN, "QF": capi.EC2D_QF, "deltU": capi.EC2D_DELTU,
+             "Amu_t": capi.EC2D_AMU_T, "Un": capi.EC2D_UN, "Un1": capi.EC2D_UN1, "QF": capi.EC2D_QF,
+             "deltU": capi.EC2D_DELTU,
              "closure_mut": capi.EC2D_CLOSURE_MUT, "closure_mask": capi.EC2D_CLOSURE_MASK}
 
     def _nv(self, case: Case, nmesh: int) -> int:
@@ -221,6 +224,12 @@ class Batch:
         rrms = np.zeros(6)
         capi.call("ec2d_res_norms", self.cases[case].id, nmesh, _ptr(rmax), _ptr(rrms))
         return rmax, rrms
+
+    def dual_inner(self) -> int:
+        """kt_in of the last dual-time step (see ec2d_dual_inner)."""
+        k = C.c_int()
+        capi.call("ec2d_dual_inner", C.byref(k))
+        return k.value
 
     def time(self, case: int = 0, nmesh: int = 1):
         k = C.c_int()

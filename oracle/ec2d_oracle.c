@@ -1020,7 +1020,6 @@ static void comput_dt(const OParams *P, OBlock *B) {
 int o_residual(OMesh *M, const OParams *P, OError *err) {
   int NV = M->nvar;
   for (int m = 0; m < 6; m++) { M->res_max[m] = 0.0; M->res_rms[m] = 0.0; }
-  if (P->Time_Method == -1) return 2; /* dual-time stepping is not on this path */
   for (int mb = 0; mb < M->nblock; mb++) {
     OBlock *B = &M->blocks[mb];
     Flow f;
@@ -1042,6 +1041,17 @@ int o_residual(OMesh *M, const OParams *P, OError *err) {
     }
     residual_fvm(M, P, B, &f);
     if (B->fvm_fdm == 1) residual_fdm(M, P, B, &f);
+    if (P->Time_Method == -1) {
+      /* dual-time source (Time_dual_LU_SGS = -1), sub_Residual.f90:55-64:
+       * Res - (3 U - 4 Un + Un1) vol Sfac, evaluated left to right */
+      for (int j = 1; j <= B->ny - 1; j++)
+        for (int i = 1; i <= B->nx - 1; i++)
+          for (int m = 1; m <= NV; m++) {
+            size_t r = RI(B, NV, m, i, j), w = UNI(B, M->nvar_un, m, i, j);
+            double t = (3.0 * B->U[UI(B, NV, m, i, j)] - 4.0 * B->Un[w]) + B->Un1[w];
+            B->Res[r] = B->Res[r] - (t * B->vol[PI_(B, i, j)]) * P->sfac;
+          }
+    }
     double rmax[6] = {0}, rrms[6] = {0};
     for (int j = 1; j <= B->ny - 1; j++)
       for (int i = 1; i <= B->nx - 1; i++)
@@ -1494,6 +1504,34 @@ void o_set_un(OMesh *M) {
     for (int j = 0; j <= B->ny; j++)
       for (int i = 0; i <= B->nx; i++)
         for (int m = 1; m <= 4; m++) B->Un[UNI(B, M->nvar_un, m, i, j)] = B->U[UI(B, M->nvar, m, i, j)];
+  }
+}
+
+/* Init_flow, dual-time stepping: Un = Un1 = U (sub_init.f90:396-409) */
+void o_dual_init(OMesh *M) {
+  for (int mb = 0; mb < M->nblock; mb++) {
+    OBlock *B = &M->blocks[mb];
+    for (int j = 1; j <= B->ny - 1; j++)
+      for (int i = 1; i <= B->nx - 1; i++)
+        for (int m = 1; m <= M->nvar; m++) {
+          size_t w = UNI(B, M->nvar_un, m, i, j);
+          B->Un[w] = B->U[UI(B, M->nvar, m, i, j)];
+          B->Un1[w] = B->U[UI(B, M->nvar, m, i, j)];
+        }
+  }
+}
+
+/* end of a dual-time step: Un1 = Un, Un = U (sub_NS_singlegrid.f90:282-299) */
+void o_dual_shift(OMesh *M) {
+  for (int mb = 0; mb < M->nblock; mb++) {
+    OBlock *B = &M->blocks[mb];
+    for (int j = 1; j <= B->ny - 1; j++)
+      for (int i = 1; i <= B->nx - 1; i++)
+        for (int m = 1; m <= M->nvar; m++) {
+          size_t w = UNI(B, M->nvar_un, m, i, j);
+          B->Un1[w] = B->Un[w];
+          B->Un[w] = B->U[UI(B, M->nvar, m, i, j)];
+        }
   }
 }
 

@@ -23,6 +23,7 @@ typedef struct {
   const double *vol, *si, *sj, *ni1, *ni2, *nj1, *nj2, *x, *y, *x1, *y1, *dw;
   double *Un, *QF, *deltU;
   const double *Akx, *Aky, *Aix, *Aiy, *Ajac;
+  double *Un1; /* (NVun,0:nx,0:ny), dual-time stepping only (sub_init.f90:154-156) */
 } OBlock;
 
 typedef struct {
@@ -35,6 +36,7 @@ typedef struct {
   double Ma, Re, gamma, AoA, Pr, Cv, Cp, dt_global, CFL, dtmax, dtmin, P_outlet, T_inf, Twall, vt_inf,
       Kt_inf, Wt_inf;
   int If_viscous, Iflag_local_dt, Time_Method, Iflag_Scheme; /* Iflag_Scheme: the global one */
+  double sfac; /* dual-time source factor of this residual call (sub_Residual.f90:28-34), Time_Method -1 */
 } OParams;
 
 typedef struct { int code, i, j, block; double T; } OError;
@@ -55,6 +57,8 @@ void o_add_force(OMesh *M);
 void o_delt_u(OMesh *M);
 void o_set_un(OMesh *M);
 void o_new_u(OMesh *M);
+void o_dual_init(OMesh *M);
+void o_dual_shift(OMesh *M);
 
 /* wall forces of one mesh: F[0..3] = pressure (Fx, Fy), viscous (Fx, Fy) */
 void o_forces(const OMesh *M, const OParams *P, double *F);

@@ -39,7 +39,7 @@ class OBlock(C.Structure):
                 ("bc", C.POINTER(C.c_int))] + [(n, _dp) for n in (
                     "U", "Res", "dt", "dU", "Lci", "Lcj", "Lvi", "Lvj", "Amu", "Amu_t",
                     "vol", "si", "sj", "ni1", "ni2", "nj1", "nj2", "x", "y", "x1", "y1", "dw",
-                    "Un", "QF", "deltU", "Akx", "Aky", "Aix", "Aiy", "Ajac")]
+                    "Un", "QF", "deltU", "Akx", "Aky", "Aix", "Aiy", "Ajac", "Un1")]
 
 
 class OMesh(C.Structure):
@@ -51,7 +51,8 @@ class OMesh(C.Structure):
 class OParams(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("Ma", "Re", "gamma", "AoA", "Pr", "Cv", "Cp", "dt_global", "CFL", "dtmax",
                                           "dtmin", "P_outlet", "T_inf", "Twall", "vt_inf", "Kt_inf", "Wt_inf")] + \
-               [(n, C.c_int) for n in ("If_viscous", "Iflag_local_dt", "Time_Method", "Iflag_Scheme")]
+               [(n, C.c_int) for n in ("If_viscous", "Iflag_local_dt", "Time_Method", "Iflag_Scheme")] + \
+               [("sfac", C.c_double)]
 
 
 class OError(C.Structure):
@@ -74,7 +75,7 @@ def lib():
         L.o_residual.restype = C.c_int
         L.o_lusgs_block.argtypes = [M, P, C.c_int, C.c_double]
         for n in ("o_update_lusgs", "o_force_vt_kw", "o_euler1_update", "o_copy_un", "o_force_function",
-                  "o_add_force", "o_delt_u", "o_set_un", "o_new_u"):
+                  "o_add_force", "o_delt_u", "o_set_un", "o_new_u", "o_dual_init", "o_dual_shift"):
             getattr(L, n).argtypes = [M]
         L.o_rk3_stage.argtypes = [M, C.c_int]
         L.o_bc.argtypes = [M, P]
@@ -115,6 +116,7 @@ class _Blk:
         self.Un[..., 0] = 1.0
         if not fine:
             self.Un[..., 3] = 1.0
+        self.Un1 = np.zeros((ny + 1, nx + 1, nvar_un))  # dual-time stepping, U(n-1)
         self.QF = np.zeros((ny - 1, nx - 1, 4))
         self.deltU = np.zeros((ny + 3, nx + 3, 4))
         self.bc = np.ascontiguousarray(np.array(g.bc, dtype=np.int32).ravel())
@@ -131,7 +133,7 @@ class _Blk:
         def p(a):
             return None if a is None else a.ctypes.data_as(_dp)
 
-        for n in ("U", "Res", "dt", "dU", "Lci", "Lcj", "Lvi", "Lvj", "Amu", "Amu_t", "Un", "QF", "deltU"):
+        for n in ("U", "Res", "dt", "dU", "Lci", "Lcj", "Lvi", "Lvj", "Amu", "Amu_t", "Un", "QF", "deltU", "Un1"):
             setattr(b, n, p(getattr(self, n)))
         for n, a in self.geo.items():
             setattr(b, n, p(a))
@@ -183,6 +185,13 @@ class OracleCase:
         self.P.Time_Method, self.P.Iflag_Scheme = c.Time_Method, c.Iflag_Scheme
         self.kstep = [0] * self.num_mesh
         self.tt = [0.0] * self.num_mesh
+        # the reference's first-call flags (integer, save): Sfac of
+        # Comput_Residual_one_mesh (sub_Residual.f90:18, 28-34) and alfa1 of
+        # Dual_time_LU_SGS (sub_NS_singlegrid.f90:241-247)
+        self.residual_calls = 0
+        self.dual_steps = 0
+        self.inner_iterations = []  # per dual-time step, kt_in as the reference prints it (limit + 1: no early exit)
+        self.inner_max_res = []  # per inner iteration, max_m Res_rms(m), the exit test's value
 
     # ---- accessors ----------------------------------------------------
     def M(self, nmesh=1):
@@ -209,6 +218,10 @@ class OracleCase:
 
     # ---- pieces -------------------------------------------------------
     def residual(self, nmesh=1):
+        if self.ctl.Time_Method == F.TIME_DUAL_LU_SGS:
+            dt = self.ctl.dt_global
+            self.P.sfac = 1.0 / (3.0 * dt) if self.residual_calls == 0 else 1.0 / (2.0 * dt)
+        self.residual_calls += 1
         err = OError()
         rc = lib().o_residual(self.M(nmesh), C.byref(self.P), C.byref(err))
         if rc == 1:
@@ -269,6 +282,34 @@ class OracleCase:
         self.tt[nmesh - 1] = self.tt[nmesh - 1] + self.ctl.dt_global
         self.kstep[nmesh - 1] += 1
 
+    def advance_dual(self):
+        """Dual_time_LU_SGS (sub_NS_singlegrid.f90:231-310), single mesh."""
+        dt = self.ctl.dt_global
+        alfa1 = 1.0 / dt if self.dual_steps == 0 else 3.0 / (2.0 * dt)
+        self.dual_steps += 1
+        nvar = self.ctl.nvar
+        kt = 0
+        for kt in range(1, self.ctl.Nstep_Inner_Limit + 1):
+            self.residual(1)
+            for mb in range(1, len(self.meshes[0]["blocks"]) + 1):
+                self.lusgs(1, mb, alfa1)
+            self.update(1)
+            self.bc(1)
+            self.exchange(1)
+            rms = self.norms(1)[1]
+            max_res = rms[0]
+            for m in range(nvar):
+                max_res = max(max_res, rms[m])
+            self.inner_max_res.append(max_res)
+            if max_res <= self.ctl.Res_Inner_Limit:
+                break
+        else:
+            kt += 1  # the DO variable after a completed loop, as the reference prints it
+        self.inner_iterations.append(kt)
+        lib().o_dual_shift(self.M(1))
+        self.tt[0] = self.tt[0] + dt
+        self.kstep[0] += 1
+
     def advance(self, nmesh=1):
         """NS_Time_advance (sub_NS_singlegrid.f90:8-26)."""
         tm = self.ctl.Time_Method
@@ -278,6 +319,8 @@ class OracleCase:
             self.advance_rk3(nmesh)
         elif tm == F.TIME_LU_SGS:
             self.advance_lusgs(nmesh)
+        elif tm == F.TIME_DUAL_LU_SGS and nmesh == 1 and self.num_mesh == 1:
+            self.advance_dual()
         else:
             raise SolverError("time method not on this path")
         self.clip(nmesh)
@@ -360,10 +403,15 @@ class OracleCase:
     def init_flow(self):
         """Init_flow (sub_init.f90:383-414) for Iflag_init 0 and 1."""
         c = self.ctl
+        if c.Time_Method == F.TIME_DUAL_LU_SGS and any(c.Pre_Step_Mesh[: c.Num_Mesh]):
+            # pre-steps would run dual-time steps before Un, Un1 are set
+            raise SolverError("dual-time stepping with pre-steps is not on this path")
         if c.Iflag_init == 0:
             self._init_zero()
         else:
             self._init_read()
+        if c.Time_Method == F.TIME_DUAL_LU_SGS:
+            lib().o_dual_init(self.M(1))
 
     def _init_zero(self):
         c = self.ctl

@@ -107,6 +107,7 @@ contains
     case (28); if (associated(B%Aix)) p2 => B%Aix
     case (29); if (associated(B%Aiy)) p2 => B%Aiy
     case (30); if (associated(B%Ajac)) p2 => B%Ajac
+    case (31); if (associated(B%Un1)) p3 => B%Un1
     end select
   end subroutine
 

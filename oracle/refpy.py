@@ -22,7 +22,7 @@ ARRAYS = {
     "U": 1, "Res": 2, "dt": 3, "dU": 4, "Lci": 5, "Lcj": 6, "Lvi": 7, "Lvj": 8, "Amu": 9, "Amu_t": 10,
     "vol": 11, "si": 12, "sj": 13, "ni1": 14, "ni2": 15, "nj1": 16, "nj2": 17, "x": 18, "y": 19,
     "x1": 20, "y1": 21, "dw": 22, "Un": 23, "QF": 24, "deltU": 25,
-    "Akx": 26, "Aky": 27, "Aix": 28, "Aiy": 29, "Ajac": 30,
+    "Akx": 26, "Aky": 27, "Aix": 28, "Aiy": 29, "Ajac": 30, "Un1": 31,
 }
 OPS = {
     "residual": 1, "lusgs": 2, "update": 3, "bc": 4, "exchange": 5, "clip": 6, "advance": 7,

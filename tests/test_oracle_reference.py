@@ -8,6 +8,9 @@ the reference build the tests are skipped; tests/test_golden.py then pins the
 oracle against stored reference outputs.
 """
 import itertools
+import os
+import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -16,7 +19,7 @@ import oracle as O
 import refpy
 from ec2d import cases
 from ec2d import formats as F
-from helpers import first_diff, naca_case, vortex_case
+from helpers import dual_case, first_diff, naca_case, vortex_case
 
 pytestmark = pytest.mark.skipif(not refpy.available(), reason="reference build (oracle/_ref) not present")
 
@@ -127,6 +130,47 @@ def test_lusgs_with_dual_time_diagonal():
         R.op("lusgs", 1, 1, c8)
         oc.lusgs(1, 1, c8 / 8.0)
         assert first_diff(oc.block().dU, R.get("dU")) is None, c8
+
+
+def ref_dual(d, nsteps, tmp_path):
+    """The reference's dual-time passes in a fresh process (ref_dual_run.py):
+    saved arrays per pass and the inner-iteration counts it prints."""
+    out = tmp_path / "ref.npz"
+    r = subprocess.run([sys.executable, os.path.join(os.path.dirname(__file__), "ref_dual_run.py"), d, str(nsteps),
+                        str(out)], capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr[-2000:]
+    inner = [int(line.split()[-1]) for line in r.stdout.splitlines() if "Inner step" in line]
+    return np.load(out), inner
+
+
+@pytest.mark.parametrize("sa,limit", [(False, 1e-30), (False, 3.5e-3), (True, 1e-30)])
+def test_dual_time_lusgs(sa, limit, tmp_path):
+    """Dual_time_LU_SGS (sub_NS_singlegrid.f90:231-310) and its source term
+    (sub_Residual.f90:28-34, 55-64), with both first-call flags: Sfac =
+    1/(3 dt) on the process's first residual only, alfa1 = 1/dt on the first
+    physical step only.  The 3.5e-3 inner limit makes inner loops exit early
+    (the recorded max_m Res_rms values lie at least 7% from it); Un and Un1
+    are compared on the interior, the cells the reference sets."""
+    n_inner = 8 if limit > 1e-20 else 6
+    d = dual_case(f"r_dual_{int(sa)}_{int(limit > 1e-20)}", sa, limit, n_inner)
+    z, inner = ref_dual(d, 3, tmp_path)
+    oc = O.OracleCase(d)
+    oc.init_flow()
+    for k in range(3):
+        oc.main_step()
+        b = oc.block()
+        for a in ("U", "Res", "dt"):
+            diff = first_diff(getattr(b, a), z[f"{a}_{k}"])
+            assert diff is None, f"step {k} {a}: {diff}"
+        for a in ("Un", "Un1"):
+            diff = first_diff(getattr(b, a)[1:-1, 1:-1], z[f"{a}_{k}"][1:-1, 1:-1])
+            assert diff is None, f"step {k} {a}: {diff}"
+        assert np.array_equal(oc.norms(1)[1], z[f"rms_{k}"]), f"step {k} rms"
+    assert oc.inner_iterations == inner
+    if limit > 1e-20:
+        assert min(inner) <= n_inner and max(inner) >= 2, inner  # early exits, not on the first iteration
+    else:
+        assert inner == [n_inner + 1] * 3
 
 
 @pytest.mark.parametrize("fdm", [False, True])
