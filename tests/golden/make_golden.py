@@ -1,6 +1,6 @@
 """Write the golden fixtures of tests/test_golden.py from the reference build.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [fixture names]
 
 Runs the flang build of the unmodified reference (oracle/_ref, built by
 oracle/Makefile.ref from the reference sources) on small generated cases and
@@ -62,12 +62,45 @@ def specs():
     return out
 
 
+def dual_fixture(name, steps=3):
+    """Dual-time LU-SGS (Time_Method -1), Euler with early inner exits.  The
+    reference's first-call flags (SAVE variables) need a process that has not
+    run a residual yet, so tests/ref_dual_run.py runs it in a new one; the
+    inner-loop indices come from the reference's own "Inner step" lines."""
+    import subprocess
+
+    from helpers import dual_case
+
+    d = dual_case("golden_dual", False, 3.5e-3, 8)
+    files = {f: open(os.path.join(d, f)).read() for f in FILES if os.path.exists(os.path.join(d, f))}
+    with tempfile.TemporaryDirectory() as t:
+        out = os.path.join(t, "ref.npz")
+        r = subprocess.run([sys.executable, os.path.join(ROOT, "tests", "ref_dual_run.py"), d, str(steps), out],
+                           capture_output=True, text=True, check=True)
+        z = np.load(out)
+        data = {f"file:{k}": np.array(v) for k, v in files.items()}
+        data["steps"] = np.array(steps)
+        data["nblock"] = np.array(1)
+        data["res_rms"] = np.array([z[f"rms_{k}"] for k in range(steps)])
+        data["res_max"] = np.array([z[f"max_{k}"] for k in range(steps)])
+        data["inner"] = np.array([int(line.split()[-1]) for line in r.stdout.splitlines() if "Inner step" in line])
+        for a in ("U", "Res", "dt"):
+            data[f"{a}:1"] = z[f"{a}_{steps - 1}"]
+    np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **data)
+    print(name, "steps", steps, "inner", data["inner"])
+
+
 def main():
     import refpy
 
     if not refpy.available():
         raise SystemExit("reference build missing: make -C oracle -f Makefile.ref")
+    only = set(sys.argv[1:])  # fixture names to write (default: all)
+    if not only or "euler_dual_time_lusgs" in only:
+        dual_fixture("euler_dual_time_lusgs")
     for name, build, steps, arrays in specs():
+        if only and name not in only:
+            continue
         with tempfile.TemporaryDirectory() as d:
             build(d)
             files = {f: open(os.path.join(d, f)).read() for f in FILES if os.path.exists(os.path.join(d, f))}

@@ -5,8 +5,8 @@ Comput_Residual_one_mesh, sub_Residual.f90:18,28-34; alfa1 of Dual_time_LU_SGS,
 sub_NS_singlegrid.f90:241-247), so its dual-time stepping depends on whether
 the process has run a residual before.  This script sets the case up in a new
 process, runs main-loop passes and saves U, Un, Un1, Res, dt and Res_rms after
-each; the reference prints its inner-iteration counts ("Inner step ...") on
-stdout.
+each (and Res_max); the reference prints its inner-iteration counts ("Inner
+step ...") on stdout.
 
     python ref_dual_run.py CASE_DIR NSTEPS OUT.npz
 """
@@ -30,7 +30,7 @@ def main():
         R.op("main_step")
         for a in ("U", "Un", "Un1", "Res", "dt"):
             arrs[f"{a}_{k}"] = R.get(a)
-        arrs[f"rms_{k}"] = R.norms(1)[1]
+        arrs[f"max_{k}"], arrs[f"rms_{k}"] = R.norms(1)
     np.savez(out, **arrs)
 
 

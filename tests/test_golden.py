@@ -39,6 +39,8 @@ def test_oracle_reproduces_reference(path, tmp_path):
         rmax, rrms = oc.norms(1)
         assert np.array_equal(rrms, z["res_rms"][k]), f"step {k} rms {rrms} vs {z['res_rms'][k]}"
         assert np.array_equal(rmax, z["res_max"][k]), f"step {k} max"
+    if "inner" in z.files:  # dual-time stepping: the reference's printed inner-loop indices
+        assert oc.inner_iterations == [int(v) for v in z["inner"]]
     for key in z.files:
         if ":" not in key or key.startswith("file:"):
             continue
