@@ -47,6 +47,7 @@ struct PlaneMap {
   int U, Res, dU, dt, Lci, Lcj, Lvi, Lvj, vol, si, sj, ni1, ni2, nj1, nj2, x1, y1, Amu, Amu_t, dw,
       Un, QF, deltU, Akx, Aky, Aix, Aiy, Ajac, flag, fx, fy, ev, sst, clo,
       Un1,  // U(n-1) of dual-time stepping (fine level, Time_Method -1), else -1
+      bl,   // Baldwin-Lomax scratch (fine level, model 1): vorticity, first-write flag; else -1
       nplanes;
 };
 

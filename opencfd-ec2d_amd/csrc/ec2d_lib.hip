@@ -20,6 +20,7 @@
 #include "ec2d_closure.hip"
 #include "ec2d_pointwise.hip"
 #include "ec2d_residual.hip"
+#include "ec2d_bl.hip"
 
 using namespace ec2d;
 
@@ -327,7 +328,7 @@ int build_tiles(DLevel& L) {
 
 // clo: the learned closure's planes (SA cases, and cases whose eddy viscosity
 // the closure supplies)
-PlaneMap make_planes(int nv, int nv_un, bool fdm_any, bool clo, bool dual) {
+PlaneMap make_planes(int nv, int nv_un, bool fdm_any, bool clo, bool dual, bool bl) {
   PlaneMap p{};
   int k = 0;
   p.U = k; k += nv;
@@ -365,6 +366,12 @@ PlaneMap make_planes(int nv, int nv_un, bool fdm_any, bool clo, bool dual) {
     k += nv;
   } else {
     p.Un1 = -1;
+  }
+  if (bl) {
+    p.bl = k;  // Baldwin-Lomax: vorticity and first-write flag
+    k += 2;
+  } else {
+    p.bl = -1;
   }
   p.nplanes = k;
   return p;
@@ -448,7 +455,21 @@ int launch_residual(int level, int clip) {
   // learned closure: 1 = SA case with mu_t overridden where in range,
   // 2 = the closure supplies mu_t of every cell (turbulence model 4)
   a.closure = (level == 0 && L.o.turb == 4) ? 2 : (level == 0 && R.clo.enabled && L.pm.clo >= 0) ? 1 : 0;
-  if (a.closure && closure_infer()) return 1;
+  if (L.pm.bl >= 0) {
+    // Baldwin-Lomax: mu_t of every cell of the level before the mean-flow
+    // residual, which takes it as given (turbulence_model_BL, called from
+    // Residual, sub_Residual.f90:168-171)
+    a.closure = 3;
+    BlArgs ba;
+    ba.blocks = L.db;
+    ba.subs = L.ds;
+    ba.cps = R.dcps;
+    ba.arena = L.arena;
+    ba.pm = L.pm;
+    hipLaunchKernelGGL(k_bl, dim3((unsigned)L.hb.size()), dim3(256), 0, R.s, ba);
+    HIPCHK(hipGetLastError());
+  }
+  if ((a.closure == 1 || a.closure == 2) && closure_infer()) return 1;
   bool visc = R.cases[0].cp.If_viscous == 1;
   int nv = L.o.nvar;
   int turb = L.o.turb;
@@ -1224,7 +1245,8 @@ int ec2d_commit(void) {
     L.o.turb = (l == 0) ? c0.Iflag_turbulence_model : 0;
     L.fdm = fdm_any;
     const int turb0 = c0.Iflag_turbulence_model;
-    L.pm = make_planes(nv, nv, fdm_any, l == 0 && (turb0 == 2 || turb0 == 4), l == 0 && c0.Time_Method == -1);
+    L.pm = make_planes(nv, nv, fdm_any, l == 0 && (turb0 == 2 || turb0 == 4), l == 0 && c0.Time_Method == -1,
+                       l == 0 && turb0 == 1 && c0.If_viscous == 1);
     // blocks
     int64_t off = 0;
     std::vector<SubfaceDesc> subs;

@@ -34,7 +34,8 @@ struct ResArgs {
   int clip;                   // apply force_vt_kw to interior nu_t on load
   int write_visc;             // store Amu / Amu_t planes
   int closure;                // 1 (SA): take mu_t of interior cells from the closure planes where in range;
-                              // 2 (4 equations): mu_t of every ring cell from the closure plane
+                              // 2 (4 equations): mu_t of every ring cell from the closure plane;
+                              // 3 (4 equations, Baldwin-Lomax): mu_t of every ring cell from Amu_t as k_bl left it
 };
 
 template <int TX, int TY, int NV, bool VISC, bool SA>
@@ -119,6 +120,8 @@ __global__ __launch_bounds__(TX* TY) void k_residual(ResArgs a) {
       // the learned closure as the eddy-viscosity model
       if constexpr (NV == 4) {
         if (a.closure == 2 && ring) at = PL(pm.clo)[q];
+        // Baldwin-Lomax (k_bl wrote every cell, ghosts included, by its own rules)
+        if (a.closure == 3 && ring) at = PL(pm.Amu_t)[q];
       }
       sAt[lj][li] = at;
     }

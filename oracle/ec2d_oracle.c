@@ -421,6 +421,153 @@ static double ipow(double x, int n) {
   return r;
 }
 
+/* ------------------------------------------------------------------------
+ * Baldwin-Lomax: sub_turbulence_BL.f90:6-275
+ * ---------------------------------------------------------------------- */
+/* BL_model_1d (:218-275) on one wall-normal line, points 1..ny-1 (1-based
+ * arrays of ny+1).  The real*8 parameters AP, Ccp, Ckleb, AKT, AK are
+ * initialised from default-real (single precision) literals (:222), so they
+ * carry the float values; Cwk is a double literal. */
+static void bl_model_1d(int ny, const double *yy, const double *Amu, double *Amu_t, const double *d,
+                        const double *u, const double *v, const double *om) {
+  const double AP = (double)26.0f, Ccp = (double)1.6f, Ckleb = (double)0.3f, Cwk = 0.25, AKT = (double)0.4f,
+               AK = (double)0.0168f;
+  double TW = fabs(Amu[1] * om[1]);
+  for (int j = 1; j <= ny - 1; j++)
+    if (fabs(Amu[j] * om[j]) > TW) TW = fabs(Amu[j] * om[j]);
+  const double Ret = sqrt(d[1] * TW) / Amu[1];
+  double Fmax = 0.0, etamax = 0.0, Udif = 0.0;
+  const int nb = ny - 1; /* Ny_boundary (:236) */
+  for (int j = 1; j <= nb; j++) {
+    double uu = sqrt(u[j] * u[j] + v[j] * v[j]);
+    if (uu > Udif) Udif = uu;
+    double etap = yy[j] * Ret;
+    double FF = yy[j] * fabs(om[j]) * (1.0 - exp(-etap / AP));
+    if (FF > Fmax) {
+      Fmax = FF;
+      etamax = yy[j];
+    }
+  }
+  const double a = etamax * Fmax, b = Cwk * etamax * Udif * Udif / Fmax;
+  const double Fwak = b < a ? b : a; /* dmin1; only Fmax = 0 makes b NaN, and then every Amu_t is 0 */
+  int iflag = 0;
+  for (int j = 1; j <= ny - 1; j++) {
+    double etap = Ret * yy[j];
+    double bl = AKT * yy[j] * (1.0 - exp(-etap / AP));
+    double visti = d[j] * bl * bl * fabs(om[j]);
+    double Fkleb = 1.0 / (1.0 + 5.5 * ipow(Ckleb * yy[j] / etamax, 6));
+    double visto = AK * Ccp * d[j] * Fwak * Fkleb;
+    if (fabs(visto) < fabs(visti)) iflag = 1;
+    Amu_t[j] = iflag == 0 ? visti : visto;
+  }
+}
+
+/* turbulence_model_BL (:6-205): vorticity at cell centres, then one line per
+ * wall cell, walls in subface order; a cell reached by several walls keeps
+ * the smallest mu_t; each wall then sets its ghost rows/columns, over the
+ * subface's node range (so a wall ending inside a face also copies its last
+ * column into the neighbouring interior one, :146-149). */
+static void turbulence_bl(OBlock *B, const Flow *f) {
+  const int nx = B->nx, ny = B->ny;
+  int wall = 0;
+  for (int k = 0; k < B->nsub; k++)
+    if (B->bc[9 * k + 6] == -10) wall = 1;
+  if (!wall) return;
+  const size_t gsz = (size_t)(nx + 3) * (ny + 3);
+  for (size_t q = 0; q < gsz; q++) B->Amu_t[q] = 0.0;
+  double *om = calloc(gsz, sizeof(double));
+  int *flag1 = calloc(gsz, sizeof(int));
+  for (int j = 1; j <= ny - 1; j++)
+    for (int i = 1; i <= nx - 1; i++) {
+      double xi = B->x1[GI(B, i + 1, j)] - B->x1[GI(B, i - 1, j)];
+      double yi = B->y1[GI(B, i + 1, j)] - B->y1[GI(B, i - 1, j)];
+      double xj = B->x1[GI(B, i, j + 1)] - B->x1[GI(B, i, j - 1)];
+      double yj = B->y1[GI(B, i, j + 1)] - B->y1[GI(B, i, j - 1)];
+      double Jac = 1.0 / (xi * yj - xj * yi);
+      double ix = Jac * yj, iy = -Jac * xj, jx = -Jac * yi, jy = Jac * xi;
+      double ui = f->uu[GI(B, i + 1, j)] - f->uu[GI(B, i - 1, j)];
+      double vi = f->v[GI(B, i + 1, j)] - f->v[GI(B, i - 1, j)];
+      double uj = f->uu[GI(B, i, j + 1)] - f->uu[GI(B, i, j - 1)];
+      double vj = f->v[GI(B, i, j + 1)] - f->v[GI(B, i, j - 1)];
+      double vx = vi * ix + vj * jx;
+      double uy = ui * iy + uj * jy;
+      om[GI(B, i, j)] = fabs(vx - uy);
+    }
+  const int nmax = (nx > ny ? nx : ny) + 1;
+  double *yy = calloc(nmax, sizeof(double)), *a1 = calloc(nmax, sizeof(double)), *t1 = calloc(nmax, sizeof(double));
+  double *d1 = calloc(nmax, sizeof(double)), *u1 = calloc(nmax, sizeof(double)), *v1 = calloc(nmax, sizeof(double));
+  double *o1 = calloc(nmax, sizeof(double));
+  double *At = B->Amu_t;
+  for (int k = 0; k < B->nsub; k++) {
+    const int *bc = &B->bc[9 * k];
+    if (bc[6] != -10) continue;
+    const int face = bc[1], ist = bc[2], iend = bc[3], jst = bc[4], jend = bc[5];
+    const int along_i = face == 1 || face == 3; /* lines run along i */
+    const int n1 = along_i ? nx : ny;           /* Ny of BL_model_1d */
+    const int lo = along_i ? jst : ist, hi = along_i ? jend - 1 : iend - 1;
+    for (int m = 0; m < nmax; m++) a1[m] = t1[m] = 0.0;
+    for (int L = lo; L <= hi; L++) {
+      double x0, y0;
+      if (face == 1) { x0 = (B->x1[GI(B, 1, L)] + B->x1[GI(B, 0, L)]) * 0.5; y0 = (B->y1[GI(B, 1, L)] + B->y1[GI(B, 0, L)]) * 0.5; }
+      else if (face == 3) { x0 = (B->x1[GI(B, nx - 1, L)] + B->x1[GI(B, nx, L)]) * 0.5; y0 = (B->y1[GI(B, nx - 1, L)] + B->y1[GI(B, nx, L)]) * 0.5; }
+      else if (face == 2) { x0 = (B->x1[GI(B, L, 1)] + B->x1[GI(B, L, 0)]) * 0.5; y0 = (B->y1[GI(B, L, 1)] + B->y1[GI(B, L, 0)]) * 0.5; }
+      else { x0 = (B->x1[GI(B, L, ny - 1)] + B->x1[GI(B, L, ny)]) * 0.5; y0 = (B->y1[GI(B, L, ny - 1)] + B->y1[GI(B, L, ny)]) * 0.5; }
+      for (int p = 1; p <= n1 - 1; p++) {
+        /* point p of the line, counted from the wall */
+        int i, j;
+        if (face == 1) { i = p; j = L; }
+        else if (face == 3) { i = nx - p; j = L; }
+        else if (face == 2) { i = L; j = p; }
+        else { i = L; j = ny - p; }
+        size_t q = GI(B, i, j);
+        double dx = B->x1[q] - x0, dy = B->y1[q] - y0;
+        yy[p] = sqrt(dx * dx + dy * dy);
+        u1[p] = f->uu[q]; v1[p] = f->v[q]; d1[p] = f->d[q]; o1[p] = om[q];
+        a1[p] = B->Amu[q];
+      }
+      bl_model_1d(n1, yy, a1, t1, d1, u1, v1, o1);
+      for (int p = 1; p <= n1 - 1; p++) {
+        int i, j;
+        if (face == 1) { i = p; j = L; }
+        else if (face == 3) { i = nx - p; j = L; }
+        else if (face == 2) { i = L; j = p; }
+        else { i = L; j = ny - p; }
+        size_t q = GI(B, i, j);
+        if (flag1[q] == 0) {
+          flag1[q] = 1;
+          At[q] = t1[p];
+        } else {
+          At[q] = fmin(At[q], t1[p]);
+        }
+      }
+    }
+    /* the wall's ghost rows / columns (:81-84, 111-114, 146-149, 180-183) */
+    if (face == 1 || face == 3) {
+      for (int i = -1; i <= nx + 1; i++) At[GI(B, i, jst - 1)] = At[GI(B, i, jst)];
+      for (int i = -1; i <= nx + 1; i++) At[GI(B, i, jend)] = At[GI(B, i, jend - 1)];
+      if (face == 1) {
+        for (int j = jst; j <= jend; j++) At[GI(B, nx, j)] = At[GI(B, nx - 1, j)];
+        for (int j = jst; j <= jend; j++) At[GI(B, 0, j)] = -At[GI(B, 1, j)];
+      } else {
+        for (int j = jst; j <= jend; j++) At[GI(B, 0, j)] = At[GI(B, 1, j)];
+        for (int j = jst; j <= jend; j++) At[GI(B, nx, j)] = -At[GI(B, nx - 1, j)];
+      }
+    } else {
+      if (face == 2) {
+        for (int i = ist; i <= iend; i++) At[GI(B, i, 0)] = -At[GI(B, i, 1)];
+        for (int i = ist; i <= iend; i++) At[GI(B, i, ny)] = At[GI(B, i, ny - 1)];
+      } else {
+        for (int i = ist; i <= iend; i++) At[GI(B, i, ny)] = -At[GI(B, i, ny - 1)];
+        for (int i = ist; i <= iend; i++) At[GI(B, i, 0)] = At[GI(B, i, 1)];
+      }
+      for (int j = -1; j <= ny + 1; j++) At[GI(B, ist - 1, j)] = At[GI(B, ist, j)];
+      for (int j = -1; j <= ny + 1; j++) At[GI(B, iend, j)] = At[GI(B, iend - 1, j)];
+    }
+  }
+  free(yy); free(a1); free(t1); free(d1); free(u1); free(v1); free(o1);
+  free(om); free(flag1);
+}
+
 /* sub_turbulence_SA.f90:4-175 */
 static void turbulence_sa(const OMesh *M, const OParams *P, OBlock *B, const Flow *f) {
   const double SA_sigma = 2.0 / 3.0, Cv1 = 7.1, Cb1 = 0.1355, Cb2 = 0.622, SA_k = 0.41;
@@ -1036,7 +1183,7 @@ int o_residual(OMesh *M, const OParams *P, OError *err) {
         memset(B->Amu_t, 0, sizeof(double) * (size_t)(B->nx + 3) * (B->ny + 3));
         if (M->turb == 2) turbulence_sa(M, P, B, &f);
         else if (M->turb == 3) turbulence_sst(M, P, B, &f);
-        else if (M->turb == 1) { flow_free(&f); return 3; } /* Baldwin-Lomax is not on this path */
+        else if (M->turb == 1) turbulence_bl(B, &f);
       }
     }
     residual_fvm(M, P, B, &f);

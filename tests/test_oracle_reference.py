@@ -70,6 +70,30 @@ def test_sst_isothermal():
     run_both(naca_case("r_sst", c, ds1=1e-4), 3, ("U", "Res", "dt", "Amu_t"))
 
 
+def test_baldwin_lomax():
+    """turbulence_model_BL (sub_turbulence_BL.f90): the C-grid's wall ends
+    inside face 2, so its ghost copies also reach the first wake columns."""
+    c = cases.control_sa_naca(iters=10, Re=1e5, Ma=0.5)
+    c.Iflag_turbulence_model = F.TURB_BL
+    run_both(naca_case("r_bl", c, ds1=1e-4), 3, ("U", "Res", "dt", "Amu_t"))
+
+
+def _walls(bcs):
+    for subs in bcs:
+        for s in subs:
+            if s[6] == F.BC_FARFIELD and s[1] in (1, 2):
+                s[6] = F.BC_WALL
+
+
+def test_baldwin_lomax_two_walls():
+    """A block with walls on two faces: each cell keeps the smaller mu_t of
+    the two wall lines through it, and the second wall's ghost copies follow
+    the first's."""
+    c = cases.control_vortex(iters=10)
+    c.If_viscous, c.Re, c.Iflag_turbulence_model = 1, 1e4, F.TURB_BL
+    run_both(vortex_case("r_bl_mb", c, bc_edit=_walls), 3, ("U", "Res", "dt", "Amu_t"))
+
+
 def _symm(bcs):
     for subs in bcs:
         for s in subs:
