@@ -59,6 +59,9 @@ def specs():
     out.append(("euler_rk3_mg3", naca(c), 2, ("U", "Res")))
     c = cases.control_sa_naca(iters=10, Re=1e5, Ma=0.5)
     out.append(("sa_fdm_weno5", naca(c, ds1=1e-4, fdm=(1,)), 2, ("U", "Res", "dt")))
+    c = cases.control_sa_naca(iters=10, Re=1e5, Ma=0.5)
+    c.Iflag_turbulence_model = F.TURB_BL
+    out.append(("baldwin_lomax_lusgs", naca(c, ds1=1e-4), 3, ("U", "Res", "dt", "Amu_t")))
     return out
 
 
