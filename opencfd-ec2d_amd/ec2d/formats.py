@@ -263,6 +263,90 @@ def write_flow2d(path: str, x1s, y1s, prims) -> None:
                     f.write(" ".join(_ftok(v) for v in (x1[j, i], y1[j, i], *q[j, i, :4])) + "\n")
 
 
+# ---- the reference's result files (output / output_Res, opencfd_ec2d_v1.5.4.f90:266-371) ----
+
+def fortran_f(x: float, w: int = 20, d: int = 10) -> str:
+    """Fortran Fw.d editing: correctly rounded, right-justified, asterisks when
+    the value does not fit."""
+    t = f"{float(x):{w}.{d}f}"
+    return "*" * w if len(t) > w else t
+
+
+def fortran_e(x: float, w: int, d: int = 10) -> str:
+    """Fortran Ew.d editing: 0.dddd...E+ee with d correctly rounded digits
+    (a three-digit exponent drops the E, as the standard prescribes)."""
+    x = float(x)
+    if x == 0.0 or x != x or x in (float("inf"), float("-inf")):
+        if x != x or x in (float("inf"), float("-inf")):
+            return f"{'NaN' if x != x else ('-Infinity' if x < 0 else 'Infinity'):>{w}}"
+        mant, e10 = "0" * d, 0
+    else:
+        m, e = f"{abs(x):.{d - 1}e}".split("e")
+        mant, e10 = m.replace(".", ""), int(e) + 1
+    sign = "-" if (x < 0 or (x == 0.0 and str(x).startswith("-"))) else ""
+    ex = f"E{'-' if e10 < 0 else '+'}{abs(e10):02d}" if abs(e10) <= 99 else f"{'-' if e10 < 0 else '+'}{abs(e10):03d}"
+    t = f"{sign}0.{mant}{ex}"
+    return "*" * w if len(t) > w else f"{t:>{w}}"
+
+
+def write_output(dirpath: str, nmesh: int, kstep: int, blocks, nvar: int, Cv: float, gamma: float, Ma: float,
+                 Re: float) -> List[str]:
+    """Result files of `output(nMesh)` (opencfd_ec2d_v1.5.4.f90:286-371), byte
+    for byte: flow2d.dat (nMesh 1; flow2d-<nMesh>.dat otherwise) with
+    x, y, d, u, v, T, p, Amut*Re; for nMesh 1 also flow2d-<Kstep>.dat with
+    x, y, d, u, v, T, p; SA2d.dat (5 equations) or SST2d.dat (6).  blocks: per
+    block a dict of U (ny+3, nx+3, nvar), Amu_t (ny+3, nx+3), x1, y1
+    (ny+3, nx+3), all over -1..n+1; rows j = 0..ny, i = 0..nx are written.
+    Returns the paths written."""
+    def prims(b):
+        U = b["U"][1:-1, 1:-1]
+        d1 = U[..., 0]
+        u1 = U[..., 1] / d1
+        v1 = U[..., 2] / d1
+        T1 = (U[..., 3] - 0.5 * d1 * (u1 * u1 + v1 * v1)) / (Cv * d1)
+        return d1, u1, v1, T1, d1 * T1 / (gamma * Ma * Ma)
+
+    def zone(b):
+        ny1, nx1 = b["U"].shape[0] - 2, b["U"].shape[1] - 2
+        return f"  zone i=  {nx1}  j=  {ny1}\n"
+
+    def write(path, header, cols_of):
+        with open(path, "w") as f:
+            f.write(f" {header}\n")
+            for b in blocks:
+                f.write(zone(b))
+                cols = [np.asarray(c).ravel() for c in cols_of(b)]
+                for row in zip(*cols):
+                    f.write("".join(fortran_f(v) for v in row) + "\n")
+        return path
+
+    out = []
+    xy = lambda b: (b["x1"][1:-1, 1:-1], b["y1"][1:-1, 1:-1])  # noqa: E731
+    if nmesh == 1:
+        out.append(write(os.path.join(dirpath, f"flow2d-{kstep:07d}.dat"), "variables=x,y,d,u,v,T,p",
+                         lambda b: (*xy(b), *prims(b))))
+    name = "flow2d.dat" if nmesh == 1 else f"flow2d-{nmesh:1d}.dat"
+    out.append(write(os.path.join(dirpath, name), "variables=x,y,d,u,v,T,p,Amut",
+                     lambda b: (*xy(b), *prims(b), b["Amu_t"][1:-1, 1:-1] * Re)))
+    if nvar == 5:
+        out.append(write(os.path.join(dirpath, "SA2d.dat"), "variables=x,y,vt",
+                         lambda b: (*xy(b), b["U"][1:-1, 1:-1, 4])))
+    if nvar == 6:
+        out.append(write(os.path.join(dirpath, "SST2d.dat"), "variables=x,y,Kt,Wt",
+                         lambda b: (*xy(b), b["U"][1:-1, 1:-1, 4], b["U"][1:-1, 1:-1, 5])))
+    return out
+
+
+def append_residual(dirpath: str, kstep: int, res_max, res_rms, nvar: int) -> str:
+    """One line of Residual.dat as output_Res writes it (opencfd_ec2d_v1.5.4.f90:266-281):
+    (I8,13E24.10) of Kstep, Res_max(1:6), Res_rms(1:NVAR)."""
+    vals = list(res_max[:6]) + list(res_rms[:nvar])
+    path = os.path.join(dirpath, "Residual.dat")
+    with open(path, "a") as f:
+        f.write(f"{kstep:8d}" + "".join(fortran_e(v, 24) for v in vals) + "\n")
+    return path
+
+
 def fortran_pow_int(x, n: int):
     """x**n for a constant integer n as flang evaluates it: ((x*x)*x)*... ."""
     r = x

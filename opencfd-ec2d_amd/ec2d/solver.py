@@ -80,8 +80,17 @@ class Case:
         return k
 
     def read_restart(self) -> List[np.ndarray]:
-        """U on the finest level from flow2d.dat (init_flow_read, sub_init.f90:509-523)."""
+        """U on the finest level from flow2d.dat (init_flow_read, sub_init.f90:509-523).
+
+        The reference defines no restart of turbulence fields: a 5-equation
+        case reads SA2d.dat and then, under the same NVAR = 5 test, SST2d.dat
+        into U(5:6) (:527-553), writing past the end of its U; a 6-equation
+        case reads neither, leaving k and omega as allocated.  Both raise."""
         c = self.ctl
+        if c.nvar >= 5:
+            raise ValueError("restart of turbulence fields: the reference's init_flow_read reads SST2d.dat into "
+                             "U(5:6) of 5-equation cases and no turbulence file for 6-equation ones "
+                             "(sub_init.f90:527-553); put U with ec2d_put_array instead")
         out = []
         with open(os.path.join(self.dir, "flow2d.dat")) as f:
             f.readline()
@@ -224,6 +233,27 @@ class Batch:
         rrms = np.zeros(6)
         capi.call("ec2d_res_norms", self.cases[case].id, nmesh, _ptr(rmax), _ptr(rrms))
         return rmax, rrms
+
+    def output(self, case: int = 0, nmesh: int = 1, dirpath: str = None) -> List[str]:
+        """output(nMesh) (opencfd_ec2d_v1.5.4.f90:286-371): the reference's
+        result files of one case, byte for byte, from the device state
+        (flow2d.dat / flow2d-<nMesh>.dat, flow2d-<Kstep>.dat, SA2d.dat or
+        SST2d.dat) into dirpath (default: the case directory)."""
+        c = self.cases[case]
+        kstep = self.time(case, 1)[0]
+        blocks = []
+        for mb, g in enumerate(c.levels[nmesh - 1], start=1):
+            blocks.append({"U": self.get("U", case, nmesh, mb), "Amu_t": self.get("Amu_t", case, nmesh, mb),
+                           "x1": g.x1, "y1": g.y1})
+        k = c.ctl
+        return F.write_output(dirpath or c.dir, nmesh, kstep, blocks, self._nv(c, nmesh), k.Cv, k.gamma, k.Ma, k.Re)
+
+    def output_res(self, case: int = 0, nmesh: int = 1, dirpath: str = None) -> str:
+        """output_Res(nMesh) (opencfd_ec2d_v1.5.4.f90:266-281): appends the
+        line of Kstep, Res_max(1:6) and Res_rms(1:NVAR) to Residual.dat."""
+        c = self.cases[case]
+        rmax, rrms = self.res_norms(case, nmesh)
+        return F.append_residual(dirpath or c.dir, self.time(case, nmesh)[0], rmax, rrms, c.ctl.nvar)
 
     def dual_inner(self) -> int:
         """kt_in of the last dual-time step (see ec2d_dual_inner)."""

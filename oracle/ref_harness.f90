@@ -244,6 +244,8 @@ contains
       else
         call NS_3stge_multigrid
       end if
+    case (22); call output(a)
+    case (23); call output_Res(a)
     case (21)
       ! run a iterations of the main loop body (timing helper; no output)
       do i = 1, a

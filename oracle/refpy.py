@@ -28,7 +28,7 @@ OPS = {
     "residual": 1, "lusgs": 2, "update": 3, "bc": 4, "exchange": 5, "clip": 6, "advance": 7,
     "mg2": 8, "mg3": 9, "euler1": 10, "rk3": 11, "restrict": 12, "prolong": 13, "force": 14,
     "add_force": 15, "delt_u": 16, "set_un": 17, "new_u": 18, "advance_lusgs": 19, "main_step": 20,
-    "main_steps": 21,
+    "main_steps": 21, "output": 22, "output_res": 23,
 }
 
 _lib = None

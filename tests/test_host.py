@@ -90,3 +90,29 @@ def test_write_case_files(tmp_path):
 def test_fortran_pow_is_left_to_right_chain():
     x = 1.0000000000000002
     assert F.fortran_pow_int(x, 6) == ((((x * x) * x) * x) * x) * x
+
+
+def test_fortran_edit_descriptors():
+    """Fw.d and Ew.d as the reference's result files use them (the reference's
+    own files are matched byte for byte in tests/test_oracle_reference.py)."""
+    assert F.fortran_f(25.87895096612) == "       25.8789509661"
+    assert F.fortran_f(-1e-12) == "       -0.0000000000"
+    assert F.fortran_f(1e12) == "*" * 20
+    assert F.fortran_e(0.006318817554, 20) == "    0.6318817554E-02"
+    assert F.fortran_e(0.0, 24) == " " * 8 + "0.0000000000E+00"
+    assert F.fortran_e(-123.456, 20) == "   -0.1234560000E+03"
+    assert F.fortran_e(9.99999999996, 20) == "    0.1000000000E+02"  # rounding carries into the exponent
+    assert F.fortran_e(1e-120, 20) == "    0.1000000000-119"
+
+
+def test_turbulence_restart_is_refused(tmp_path):
+    """The reference defines no restart of turbulence fields
+    (sub_init.f90:527-553): the host side refuses it instead of inventing one."""
+    from ec2d import solver
+
+    x, y, bc, _ = cases.naca_cgrid(33, 9, ds1=1e-3)
+    c = cases.control_sa_naca(iters=1, Re=1e5, Ma=0.5)
+    c.Iflag_init = 1
+    d = cases.write_case(str(tmp_path / "sa"), c, [x], [y], [bc])
+    with pytest.raises(ValueError, match="SST2d.dat"):
+        solver.Case.from_dir(d).read_restart()

@@ -94,6 +94,48 @@ def test_baldwin_lomax_two_walls():
     run_both(vortex_case("r_bl_mb", c, bc_edit=_walls), 3, ("U", "Res", "dt", "Amu_t"))
 
 
+def _oracle_blocks(oc, nmesh):
+    return [{"U": b.U, "Amu_t": b.Amu_t, "x1": b.geo["x1"], "y1": b.geo["y1"]} for b in oc.meshes[nmesh - 1]["blocks"]]
+
+
+@pytest.mark.parametrize("kind", ["sa", "sst", "euler_mg3"])
+def test_output_files_match_reference(kind, tmp_path):
+    """output / output_Res (opencfd_ec2d_v1.5.4.f90:266-371): the files the
+    reference writes from its state after two passes equal, byte for byte,
+    the ones formats.write_output / append_residual write from the oracle's."""
+    c = cases.control_sa_naca(iters=10, Re=1e5, Ma=0.5) if kind != "euler_mg3" else cases.control_euler_naca(iters=10)
+    if kind == "sst":
+        c.Iflag_turbulence_model, c.Twall = F.TURB_SST, 300.0
+    if kind == "euler_mg3":
+        c.Time_Method, c.Num_Mesh, c.CFL = F.TIME_RK3, 3, 0.3
+    d = naca_case(f"r_out_{kind}", c, ds1=1e-4 if kind != "euler_mg3" else 1e-3)
+    if os.path.exists(os.path.join(d, "Residual.dat")):
+        os.remove(os.path.join(d, "Residual.dat"))  # output_Res appends; case directories persist between runs
+    R = refpy.RefCase(d)
+    oc = O.OracleCase(d)
+    oc.init_flow()
+    for _ in range(2):
+        R.op("main_step")
+        oc.main_step()
+    meshes = (1, 2) if kind == "euler_mg3" else (1,)
+    mine = tmp_path / "mine"
+    mine.mkdir()
+    written = []
+    for nm in meshes:
+        R.op("output", nm)
+        nv = oc.meshes[nm - 1]["M"].nvar
+        written += F.write_output(str(mine), nm, oc.kstep[0], _oracle_blocks(oc, nm), nv, c.Cv, c.gamma, c.Ma, c.Re)
+    for _ in range(2):
+        R.op("output_res", 1)
+        mx, rms = oc.norms(1)
+        written.append(F.append_residual(str(mine), oc.kstep[0], mx, rms, c.nvar))
+    for path in sorted(set(written)):
+        name = os.path.basename(path)
+        want = open(os.path.join(d, name)).read()
+        got = open(path).read()
+        assert got == want, f"{name}: first difference at {next(i for i, (x, y) in enumerate(zip(got, want)) if x != y) if got[:len(want)] != want[:len(got)] else min(len(got), len(want))}"
+
+
 def _symm(bcs):
     for subs in bcs:
         for s in subs:
