@@ -21,7 +21,7 @@ def naca_case(name, ctl: F.Control, nx=65, ny=17, ds1=1e-3, fdm=(), **kw):
     return cases.write_case(case_dir(name), ctl, [x], [y], [bc], fdm_blocks=fdm)
 
 
-def dual_case(name, sa, limit, n_inner):
+def dual_case(name, sa, limit, n_inner, fdm=()):
     """A dual-time LU-SGS case (Time_Method -1) the reference keeps finite:
     Euler at CFL 5 with physical step 0.05, SA at CFL 0.5 with 0.1, local
     pseudo-time steps."""
@@ -29,7 +29,7 @@ def dual_case(name, sa, limit, n_inner):
     c.Time_Method = F.TIME_DUAL_LU_SGS
     c.Nstep_Inner_Limit, c.Res_Inner_Limit = n_inner, limit
     c.dt_global, c.CFL, c.Iflag_local_dt = (0.1, 0.5, 1) if sa else (0.05, 5.0, 1)
-    return naca_case(name, c, ds1=1e-4 if sa else 1e-3)
+    return naca_case(name, c, ds1=1e-4 if sa else 1e-3, fdm=fdm)
 
 
 def vortex_case(name, ctl: F.Control, seed=3, nbx=3, nby=2, wmin=9, wmax=21, hmin=9, hmax=17, h=0.25, bc_edit=None):

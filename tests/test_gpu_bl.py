@@ -29,6 +29,10 @@ def _walls(bcs):
 
 
 def _case(kind):
+    if kind == "mg2":
+        c = cases.control_sa_naca(iters=10, Re=1e5, Ma=0.5)
+        c.Iflag_turbulence_model, c.Time_Method, c.Num_Mesh, c.CFL = F.TURB_BL, F.TIME_RK3, 2, 0.3
+        return naca_case("g_bl_mg", c, ds1=1e-4)
     if kind == "naca":
         c = cases.control_sa_naca(iters=10, Re=1e5, Ma=0.5)
         c.Iflag_turbulence_model = F.TURB_BL
@@ -47,7 +51,7 @@ def _check(b, ref, k):
                                        err_msg=f"step {k} block {mb} {a}")
 
 
-@pytest.mark.parametrize("kind", ["naca", "two_walls"])
+@pytest.mark.parametrize("kind", ["naca", "two_walls", "mg2"])
 def test_baldwin_lomax_steps(kind):
     d = _case(kind)
     ref = O.OracleCase(d)

@@ -37,10 +37,12 @@ def _check(b, ref, k, exact):
     np.testing.assert_allclose(grms[:nv], rrms[:nv], rtol=1e-12 if exact else 1e-11, atol=0)
 
 
-@pytest.mark.parametrize("sa,limit", [(False, 1e-30), (False, 3.5e-3), (True, 1e-30)])
-def test_dual_time_steps(sa, limit):
+@pytest.mark.parametrize("sa,limit,fdm", [(False, 1e-30, ()), (False, 3.5e-3, ()), (True, 1e-30, ()), (True, 1e-30, (1,))])
+def test_dual_time_steps(sa, limit, fdm):
+    """The FD-WENO5 variant applies the source after the finite-difference
+    interior of the residual, as the reference does."""
     n_inner = 8 if limit > 1e-20 else 6
-    d = dual_case(f"g_dual_{int(sa)}_{int(limit > 1e-20)}", sa, limit, n_inner)
+    d = dual_case(f"g_dual_{int(sa)}_{int(limit > 1e-20)}_{len(fdm)}", sa, limit, n_inner, fdm)
     ref = O.OracleCase(d)
     ref.init_flow()
     with solver.Batch([solver.Case.from_dir(d)]) as b:

@@ -85,6 +85,14 @@ def _walls(bcs):
                 s[6] = F.BC_WALL
 
 
+def test_baldwin_lomax_multigrid():
+    """Baldwin-Lomax on the fine level of a 2-level RK3 V-cycle (the coarse
+    levels carry no turbulence model, opencfd_ec2d_v1.5.4.f90:252-259)."""
+    c = cases.control_sa_naca(iters=10, Re=1e5, Ma=0.5)
+    c.Iflag_turbulence_model, c.Time_Method, c.Num_Mesh, c.CFL = F.TURB_BL, F.TIME_RK3, 2, 0.3
+    run_both(naca_case("r_bl_mg", c, ds1=1e-4), 2, ("U", "Res", "dt", "Amu_t"))
+
+
 def test_baldwin_lomax_two_walls():
     """A block with walls on two faces: each cell keeps the smaller mu_t of
     the two wall lines through it, and the second wall's ghost copies follow
@@ -209,8 +217,9 @@ def ref_dual(d, nsteps, tmp_path):
     return np.load(out), inner
 
 
-@pytest.mark.parametrize("sa,limit", [(False, 1e-30), (False, 3.5e-3), (True, 1e-30)])
-def test_dual_time_lusgs(sa, limit, tmp_path):
+@pytest.mark.parametrize("sa,limit,fdm", [(False, 1e-30, ()), (False, 3.5e-3, ()), (True, 1e-30, ()),
+                                         (True, 1e-30, (1,))])
+def test_dual_time_lusgs(sa, limit, fdm, tmp_path):
     """Dual_time_LU_SGS (sub_NS_singlegrid.f90:231-310) and its source term
     (sub_Residual.f90:28-34, 55-64), with both first-call flags: Sfac =
     1/(3 dt) on the process's first residual only, alfa1 = 1/dt on the first
@@ -218,7 +227,7 @@ def test_dual_time_lusgs(sa, limit, tmp_path):
     (the recorded max_m Res_rms values lie at least 7% from it); Un and Un1
     are compared on the interior, the cells the reference sets."""
     n_inner = 8 if limit > 1e-20 else 6
-    d = dual_case(f"r_dual_{int(sa)}_{int(limit > 1e-20)}", sa, limit, n_inner)
+    d = dual_case(f"r_dual_{int(sa)}_{int(limit > 1e-20)}_{len(fdm)}", sa, limit, n_inner, fdm)
     z, inner = ref_dual(d, 3, tmp_path)
     oc = O.OracleCase(d)
     oc.init_flow()
