@@ -42,6 +42,7 @@ enum {
   EC2D_DU = 4,     /* dU(NV,1:nx,1:ny) */
   EC2D_LCI = 5, EC2D_LCJ = 6, EC2D_LVI = 7, EC2D_LVJ = 8, /* (1:nx,1:ny) */
   EC2D_AMU = 9, EC2D_AMU_T = 10,                          /* (-1:nx+1,-1:ny+1), interior filled */
+  EC2D_DW = 22,    /* dw(0:nx,0:ny) wall distance (SA / SST / model 4) */
   EC2D_UN = 23,    /* Un(NVun,0:nx,0:ny) */
   EC2D_QF = 24,    /* QF(4,1:nx-1,1:ny-1) */
   EC2D_DELTU = 25, /* deltU(4,-1:nx+1,-1:ny+1) */
@@ -71,7 +72,8 @@ int ec2d_case_create(const ec2d_control *ctl);
  *   x, y    (-1:nx+2,-1:ny+2) nodes incl. ghosts
  *   x1, y1  (-1:nx+1,-1:ny+1) cell centres
  *   vol, si, sj, ni1, ni2, nj1, nj2  (1:nx,1:ny)
- *   dw      (0:nx,0:ny) wall distance, NULL unless SA / SST
+ *   dw      (0:nx,0:ny) wall distance, NULL unless SA / SST / model 4; NULL for one of those: ec2d_commit
+ *           computes it on the device (comput_dw, sub_turbulence_SST.f90:286-337)
  *   fd      5 arrays Akx,Aky,Aix,Aiy,Ajac of (1:nx-1,1:ny-1) back to back, NULL unless fvm_fdm = 1 */
 int ec2d_define_block(int case_id, int nMesh, int mBlock, int nx, int ny, int nsub, const int *bc_msg, int fvm_fdm,
                       const double *x, const double *y, const double *x1, const double *y1, const double *vol,

@@ -21,6 +21,7 @@
 #include "ec2d_pointwise.hip"
 #include "ec2d_residual.hip"
 #include "ec2d_bl.hip"
+#include "ec2d_dw.hip"
 
 using namespace ec2d;
 
@@ -968,6 +969,7 @@ int array_map(const DLevel& L, int which, int* slot, int* ncomp, int* kind) {
     case EC2D_UN: *slot = p.Un; *ncomp = L.o.nvar_un; *kind = 3; return 0;
     case EC2D_QF: *slot = p.QF; *ncomp = 4; *kind = 2; return 0;
     case EC2D_DELTU: *slot = p.deltU; *ncomp = 4; *kind = 0; return 0;
+    case EC2D_DW: *slot = p.dw; *ncomp = 1; *kind = 3; return 0;
     case EC2D_UN1: if (p.Un1 < 0) return 1; *slot = p.Un1; *ncomp = L.o.nvar; *kind = 3; return 0;
     case EC2D_CLOSURE_MUT: if (p.clo < 0) return 1; *slot = p.clo; *ncomp = 1; *kind = 5; return 0;
     case EC2D_CLOSURE_MASK: if (p.clo < 0) return 1; *slot = p.clo + 1; *ncomp = 1; *kind = 5; return 0;
@@ -1199,6 +1201,74 @@ int ec2d_define_block(int case_id, int nMesh, int mBlock, int nx, int ny, int ns
   return 0;
 }
 
+bool wall_distance_needed(const ec2d_control& c) {
+  const int t = c.Iflag_turbulence_model;
+  return t == 2 || t == 3 || t == 4;
+}
+
+// comput_dw on the device (ec2d_dw.hip) for the finest-level blocks whose
+// caller gave no dw: node coordinates and the wall nodes of each case go up
+// once, k_dw_nodes / k_dw_cells write the dw plane.
+int device_wall_distance(DLevel& L) {
+  std::vector<int> blk, wall0;
+  std::vector<int64_t> off;
+  std::vector<double> xy, wx, wy;
+  for (size_t b = 0; b < L.hb.size(); b++) {
+    const HCase& cs = R.cases[L.blk_case[b]];
+    const HBlock& hb = cs.levels[0][L.blk_local[b]];
+    if (!hb.dw.empty() || !wall_distance_needed(cs.ctl)) continue;
+    // the wall nodes of the block's case (every block, every wall subface)
+    const int w0 = (int)wx.size();
+    for (const HBlock& b1 : cs.levels[0])
+      for (int k = 0; k < b1.nsub; k++) {
+        const int* s = &b1.bc[9 * k];
+        if (s[6] != -10) continue;
+        for (int j1 = s[4]; j1 <= s[5]; j1++)
+          for (int i1 = s[2]; i1 <= s[3]; i1++) {
+            wx.push_back(b1.x[ni_(b1.nx, i1, j1)]);
+            wy.push_back(b1.y[ni_(b1.nx, i1, j1)]);
+          }
+      }
+    blk.push_back((int)b);
+    wall0.push_back(w0);
+    wall0.push_back((int)wx.size());
+    off.push_back((int64_t)xy.size());
+    const int nn = hb.nx * hb.ny;
+    xy.resize(xy.size() + 2 * (size_t)nn);
+    double* px = xy.data() + off.back();
+    for (int j = 1; j <= hb.ny; j++)
+      for (int i = 1; i <= hb.nx; i++) {
+        px[(j - 1) * hb.nx + (i - 1)] = hb.x[ni_(hb.nx, i, j)];
+        px[nn + (j - 1) * hb.nx + (i - 1)] = hb.y[ni_(hb.nx, i, j)];
+      }
+  }
+  if (blk.empty()) return 0;
+  int maxn = 0;
+  for (int b : blk) maxn = std::max(maxn, L.hb[b].nx * L.hb[b].ny);
+  DwArgs a;
+  double *d_xy = nullptr, *d_wx = nullptr, *d_wy = nullptr, *d_node = nullptr;
+  int64_t* d_off = nullptr;
+  int *d_blk = nullptr, *d_w0 = nullptr;
+  auto up = [&](auto** dst, const auto& v) -> int {
+    using T = std::remove_reference_t<decltype(v[0])>;
+    HIPCHK(hipMalloc(dst, sizeof(T) * std::max<size_t>(v.size(), 1)));
+    if (!v.empty()) HIPCHK(hipMemcpy(*dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+    return 0;
+  };
+  int rc = up(&d_xy, xy) || up(&d_wx, wx) || up(&d_wy, wy) || up(&d_off, off) || up(&d_blk, blk) || up(&d_w0, wall0);
+  if (!rc) rc = hipMalloc(&d_node, sizeof(double) * std::max<size_t>(xy.size() / 2, 1)) == hipSuccess ? 0 : fail("hipMalloc failed");
+  if (!rc) {
+    a.nx_y = d_xy; a.off = d_off; a.blk = d_blk; a.wall0 = d_w0; a.wx = d_wx; a.wy = d_wy;
+    a.blocks = L.db; a.arena = L.arena; a.dw = L.pm.dw; a.node_dw = d_node;
+    const dim3 g((unsigned)((maxn + 255) / 256), (unsigned)blk.size());
+    hipLaunchKernelGGL(k_dw_nodes, g, dim3(256), 0, R.s, a);
+    hipLaunchKernelGGL(k_dw_cells, g, dim3(256), 0, R.s, a);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(R.s) != hipSuccess) rc = fail("wall distance kernels failed");
+  }
+  hipFree(d_xy); hipFree(d_wx); hipFree(d_wy); hipFree(d_off); hipFree(d_blk); hipFree(d_w0); hipFree(d_node);
+  return rc;
+}
+
 int ec2d_commit(void) {
   if (!R.inited) return fail("ec2d_init has not been called");
   if (R.cases.empty()) return fail("no case defined");
@@ -1365,6 +1435,11 @@ int ec2d_commit(void) {
       to_plane(bd, plane(L.pm.x1), hb.x1.data(), 5, 1, 0);
       to_plane(bd, plane(L.pm.y1), hb.y1.data(), 5, 1, 0);
       if (!hb.dw.empty()) to_plane(bd, plane(L.pm.dw), hb.dw.data(), 3, 1, 0);
+      else if (l == 0 && wall_distance_needed(R.cases[L.blk_case[b]].ctl)) {
+        // computed on the device below; B%dw(:,:) = 1 around it (sub_turbulence_SST.f90:288-293)
+        std::vector<double> one((size_t)(hb.nx + 1) * (hb.ny + 1), 1.0);
+        to_plane(bd, plane(L.pm.dw), one.data(), 3, 1, 0);
+      }
       if (!hb.fd.empty() && L.pm.Akx >= 0) {
         size_t n = (size_t)(hb.nx - 1) * (hb.ny - 1);
         int slots[5] = {L.pm.Akx, L.pm.Aky, L.pm.Aix, L.pm.Aiy, L.pm.Ajac};
@@ -1386,6 +1461,8 @@ int ec2d_commit(void) {
     HIPCHK(hipMemcpy(L.arena, img.data(), sizeof(double) * (size_t)off, hipMemcpyHostToDevice));
     HIPCHK(hipMalloc(&L.db, sizeof(BlockDesc) * L.hb.size()));
     HIPCHK(hipMemcpy(L.db, L.hb.data(), sizeof(BlockDesc) * L.hb.size(), hipMemcpyHostToDevice));
+    // after the block table: the wall-distance kernels read it
+    if (l == 0 && device_wall_distance(L)) return 1;
     std::vector<int> units(std::max<size_t>(subs.size(), 1), 0);
     for (auto& b : L.hb) ghost_phases(b, subs.data() + b.bc_off, units.data() + b.bc_off);
     HIPCHK(hipMalloc(&L.dunits, sizeof(int) * units.size()));

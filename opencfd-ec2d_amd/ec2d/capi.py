@@ -17,6 +17,7 @@ EC2D_LCI, EC2D_LCJ, EC2D_LVI, EC2D_LVJ = 5, 6, 7, 8
 EC2D_AMU, EC2D_AMU_T = 9, 10
 EC2D_UN, EC2D_QF, EC2D_DELTU = 23, 24, 25
 EC2D_UN1 = 31
+EC2D_DW = 22
 EC2D_CLOSURE_MUT, EC2D_CLOSURE_MASK = 40, 41
 CLOSURE_NPARAM = 6 * 64 + 64 + 64 * 64 + 64 + 64 * 64 + 64 + 64 + 1  # W1 b1 W2 b2 W3 b3 W4 b4
 CLOSURE_NNORM = 24  # mean, inv_std, lo, hi of the 6 features

@@ -45,12 +45,14 @@ class Case:
         self.id = -1
 
     @classmethod
-    def from_dir(cls, case_dir: str) -> "Case":
+    def from_dir(cls, case_dir: str, device_dw: bool = False) -> "Case":
+        """device_dw: leave the wall distance to the device (ec2d_commit runs
+        comput_dw there) instead of computing it on the host."""
         ctl = F.read_control(os.path.join(case_dir, "control.in"))
         xs, ys = F.read_mesh2d(os.path.join(case_dir, "Mesh2d.dat"))
         bcs = F.read_bcin(os.path.join(case_dir, "bc2d_1.in"))
         fdm = F.read_fdm_in(os.path.join(case_dir, "FDM.in"))
-        need_dw = ctl.Iflag_turbulence_model in (F.TURB_SA, F.TURB_SST, F.TURB_CLOSURE)
+        need_dw = ctl.Iflag_turbulence_model in (F.TURB_SA, F.TURB_SST, F.TURB_CLOSURE) and not device_dw
         levels = G.build_levels(xs, ys, bcs, ctl.Num_Mesh, fdm, need_dw)
         return cls(ctl, levels, case_dir)
 
@@ -155,6 +157,7 @@ class Batch:
         capi.EC2D_AMU_T: lambda B, nv: (B.ny + 3, B.nx + 3),
         capi.EC2D_UN: lambda B, nv: (B.ny + 1, B.nx + 1, nv),
         capi.EC2D_UN1: lambda B, nv: (B.ny + 1, B.nx + 1, nv),
+        capi.EC2D_DW: lambda B, nv: (B.ny + 1, B.nx + 1),
         capi.EC2D_QF: lambda B, nv: (B.ny - 1, B.nx - 1, 4),
         capi.EC2D_DELTU: lambda B, nv: (B.ny + 3, B.nx + 3, 4),
         capi.EC2D_CLOSURE_MUT: lambda B, nv: (B.ny + 3, B.nx + 3),
@@ -162,7 +165,7 @@ class Batch:
     }
     NAMES = {"U": capi.EC2D_U, "Res": capi.EC2D_RES, "dt": capi.EC2D_DT, "dU": capi.EC2D_DU, "Lci": capi.EC2D_LCI,
              "Lcj": capi.EC2D_LCJ, "Lvi": capi.EC2D_LVI, "Lvj": capi.EC2D_LVJ, "Amu": capi.EC2D_AMU,
-             "Amu_t": capi.EC2D_AMU_T, "Un": capi.EC2D_UN, "Un1": capi.EC2D_UN1, "QF": capi.EC2D_QF,
+             "Amu_t": capi.EC2D_AMU_T, "Un": capi.EC2D_UN, "Un1": capi.EC2D_UN1, "dw": capi.EC2D_DW, "QF": capi.EC2D_QF,
              "deltU": capi.EC2D_DELTU,
              "closure_mut": capi.EC2D_CLOSURE_MUT, "closure_mask": capi.EC2D_CLOSURE_MASK}
 
