@@ -53,6 +53,8 @@ struct DLevel {
   BlockDesc* db = nullptr;
   SubfaceDesc* ds = nullptr;
   int* dunits = nullptr;  // ghost fill order per block: subface indices, boundary conditions first
+  int bl_walls = 0, bl_lines = 0;  // Baldwin-Lomax: most wall subfaces of a block, most lines of one
+  int64_t max_plane = 0;           // largest block plane (doubles)
   double* dnorm = nullptr;
   std::vector<TileDesc> ht;
   TileDesc* dt = nullptr;
@@ -467,7 +469,12 @@ int launch_residual(int level, int clip) {
     ba.cps = R.dcps;
     ba.arena = L.arena;
     ba.pm = L.pm;
-    hipLaunchKernelGGL(k_bl, dim3((unsigned)L.hb.size()), dim3(256), 0, R.s, ba);
+    const unsigned nb = (unsigned)L.hb.size();
+    hipLaunchKernelGGL(k_bl_init, dim3((unsigned)((L.max_plane + 255) / 256), nb), dim3(256), 0, R.s, ba);
+    for (int w = 0; w < L.bl_walls; w++) {
+      hipLaunchKernelGGL(k_bl_lines, dim3((unsigned)std::max(1, (L.bl_lines + 3) / 4), nb), dim3(256), 0, R.s, ba, w);
+      hipLaunchKernelGGL(k_bl_ghost, dim3(nb), dim3(256), 0, R.s, ba, w);
+    }
     HIPCHK(hipGetLastError());
   }
   if ((a.closure == 1 || a.closure == 2) && closure_infer()) return 1;
@@ -1463,6 +1470,19 @@ int ec2d_commit(void) {
     HIPCHK(hipMemcpy(L.db, L.hb.data(), sizeof(BlockDesc) * L.hb.size(), hipMemcpyHostToDevice));
     // after the block table: the wall-distance kernels read it
     if (l == 0 && device_wall_distance(L)) return 1;
+    L.bl_walls = L.bl_lines = 0;
+    L.max_plane = 0;
+    for (auto& b : L.hb) {
+      L.max_plane = std::max(L.max_plane, b.plane);
+      int nw = 0;
+      for (int k = 0; k < b.nsub; k++) {
+        const SubfaceDesc& sf = subs[b.bc_off + k];
+        if (sf.neighb != -10) continue;
+        nw++;
+        L.bl_lines = std::max(L.bl_lines, (sf.face == 1 || sf.face == 3) ? sf.jend - sf.jst : sf.iend - sf.ist);
+      }
+      L.bl_walls = std::max(L.bl_walls, nw);
+    }
     std::vector<int> units(std::max<size_t>(subs.size(), 1), 0);
     for (auto& b : L.hb) ghost_phases(b, subs.data() + b.bc_off, units.data() + b.bc_off);
     HIPCHK(hipMalloc(&L.dunits, sizeof(int) * units.size()));
