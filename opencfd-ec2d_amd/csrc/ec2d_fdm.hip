@@ -6,7 +6,7 @@
 // WENO-JS along grid lines (:207-275), plus central viscous fluxes (:86-134).
 // The fused FVM residual kernel skips these faces and cells; three launches
 // here produce the interior residual:
-//   k_fdm_lines  one workgroup per grid line (i-lines, then j-lines): the split
+//   k_fdm_lines  one workgroup per grid line, i-lines then j-lines in one launch: the split
 //                fluxes of the whole line in LDS, WENO5 at every interface,
 //                interface fluxes F(i+1/2) to scratch planes;
 //   k_fdm_visc   viscous Ev1 / Ev2 at cell centres (scratch planes);
@@ -23,7 +23,7 @@ struct FdmArgs {
   const CaseParams* cps;
   double* arena;
   PlaneMap pm;
-  int dir;             // 0: i-lines, 1: j-lines
+  int n_i;             // k_fdm_lines: lines[0, n_i) are i-lines, the rest j-lines
   int fx, fy;          // first scratch plane of the i- and j-direction interface fluxes (4 each)
   int ev;              // first scratch plane of Ev1 (3) and Ev2 (3)
   int viscous;
@@ -97,17 +97,18 @@ __device__ __forceinline__ double weno_m(double vm2, double vm1, double v0, doub
 __global__ __launch_bounds__(256) void k_fdm_lines(FdmArgs a) {
   __shared__ double sfp[4][FDM_MAXN + 1], sfm[4][FDM_MAXN + 1];
   const int2 bl = a.lines[blockIdx.x];
+  const int dir = (int)blockIdx.x < a.n_i ? 0 : 1;
   const BlockDesc B = a.blocks[bl.x];
   const CaseParams cp = a.cps[B.case_id];
   const PlaneMap pm = a.pm;
   double* base = a.arena + B.base;
   auto PL = [&](int slot) { return base + (int64_t)slot * B.plane; };
   const int nxp = B.nx - 1, nyp = B.ny - 1;
-  const int n = a.dir == 0 ? nxp : nyp;
+  const int n = dir == 0 ? nxp : nyp;
   const int line = bl.y;
   // split fluxes at every point k = 1..n of the line (sub_Finite_Difference.f90:37-46, 59-68)
   for (int k = 1 + threadIdx.x; k <= n; k += blockDim.x) {
-    int i = a.dir == 0 ? k : line, j = a.dir == 0 ? line : k;
+    int i = dir == 0 ? k : line, j = dir == 0 ? line : k;
     int64_t q = pidx(B, i, j);
     double d = PL(pm.U)[q];
     double uu = PL(pm.U + 1)[q] / d;
@@ -115,8 +116,8 @@ __global__ __launch_bounds__(256) void k_fdm_lines(FdmArgs a) {
     double T = (PL(pm.U + 3)[q] - 0.5 * d * (uu * uu + vv * vv)) / (cp.Cv * d);
     double cc = sqrt(T) / cp.Ma;
     double jac = PL(pm.Ajac)[q];
-    double A1 = a.dir == 0 ? PL(pm.Akx)[q] / jac : PL(pm.Aix)[q] / jac;
-    double A2 = a.dir == 0 ? PL(pm.Aky)[q] / jac : PL(pm.Aiy)[q] / jac;
+    double A1 = dir == 0 ? PL(pm.Akx)[q] / jac : PL(pm.Aix)[q] / jac;
+    double A2 = dir == 0 ? PL(pm.Aky)[q] / jac : PL(pm.Aiy)[q] / jac;
     double fp[4], fm[4];
     split_sw(d, uu, vv, cc, fp, fm, A1, A2, cp.gamma);
     for (int m = 0; m < 4; m++) {
@@ -126,9 +127,9 @@ __global__ __launch_bounds__(256) void k_fdm_lines(FdmArgs a) {
   }
   __syncthreads();
   // interface fluxes F(k+1/2) = -(h+(k) + h-(k)) for k = LFDM .. n-LFDM
-  double* F = PL(a.dir == 0 ? a.fx : a.fy);
+  double* F = PL(dir == 0 ? a.fx : a.fy);
   for (int k = LFDM + threadIdx.x; k <= n - LFDM; k += blockDim.x) {
-    int i = a.dir == 0 ? k : line, j = a.dir == 0 ? line : k;
+    int i = dir == 0 ? k : line, j = dir == 0 ? line : k;
     int64_t q = pidx(B, i, j);
     for (int m = 0; m < 4; m++) {
       const double* p = sfp[m];

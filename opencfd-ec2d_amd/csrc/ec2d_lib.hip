@@ -85,7 +85,7 @@ struct DLevel {
   int prep_tiles = 0;         // LU-SGS prep workgroups per block: 32 x 8 cell tiles of the largest block
   std::vector<long long> case_cells;
   bool fdm = false;
-  int2 *d_ilines = nullptr, *d_jlines = nullptr, *d_frows = nullptr;
+  int2 *d_ilines = nullptr, *d_frows = nullptr;  // d_ilines: i-lines then j-lines
   int n_ilines = 0, n_jlines = 0, n_frows = 0;
 };
 
@@ -531,12 +531,9 @@ int launch_residual(int level, int clip) {
     f.fy = L.pm.fy;
     f.ev = L.pm.ev;
     f.viscous = visc ? 1 : 0;
-    f.dir = 0;
+    f.n_i = L.n_ilines;
     f.lines = L.d_ilines;
-    hipLaunchKernelGGL(k_fdm_lines, dim3(L.n_ilines), dim3(256), 0, R.s, f);
-    f.dir = 1;
-    f.lines = L.d_jlines;
-    hipLaunchKernelGGL(k_fdm_lines, dim3(L.n_jlines), dim3(256), 0, R.s, f);
+    hipLaunchKernelGGL(k_fdm_lines, dim3(L.n_ilines + L.n_jlines), dim3(256), 0, R.s, f);
     f.lines = L.d_frows;
     if (visc) hipLaunchKernelGGL(k_fdm_visc, dim3(L.n_frows), dim3(256), 0, R.s, f);
     hipLaunchKernelGGL(k_fdm_res, dim3(L.n_frows), dim3(256), 0, R.s, f);
@@ -1115,7 +1112,7 @@ int ec2d_finalize(void) {
     hipFree(L.dcase_tile0); hipFree(L.dncell); hipFree(L.dpartials); hipFree(L.dnorms); hipFree(L.drows);
     hipFree(L.dblocks_all); hipFree(L.sk); hipFree(L.dwall); hipFree(L.dcase_wall0); hipFree(L.dforces);
     hipFree(L.dgring);
-    hipFree(L.d_ilines); hipFree(L.d_jlines); hipFree(L.d_frows);
+    hipFree(L.d_ilines); hipFree(L.d_frows);
   }
   hipFree(R.clo.dnet); hipFree(R.clo.theta); hipFree(R.clo.m1); hipFree(R.clo.m2); hipFree(R.clo.part);
   hipFree(R.clo.losses); hipFree(R.clo.data); hipFree(R.clo.cells); hipFree(R.clo.idx);
@@ -1522,10 +1519,9 @@ int ec2d_commit(void) {
       L.n_ilines = (int)il.size();
       L.n_jlines = (int)jl.size();
       L.n_frows = (int)fr.size();
+      il.insert(il.end(), jl.begin(), jl.end());  // i-lines, then j-lines: one launch covers both
       HIPCHK(hipMalloc(&L.d_ilines, sizeof(int2) * il.size()));
       HIPCHK(hipMemcpy(L.d_ilines, il.data(), sizeof(int2) * il.size(), hipMemcpyHostToDevice));
-      HIPCHK(hipMalloc(&L.d_jlines, sizeof(int2) * jl.size()));
-      HIPCHK(hipMemcpy(L.d_jlines, jl.data(), sizeof(int2) * jl.size(), hipMemcpyHostToDevice));
       HIPCHK(hipMalloc(&L.d_frows, sizeof(int2) * fr.size()));
       HIPCHK(hipMemcpy(L.d_frows, fr.data(), sizeof(int2) * fr.size(), hipMemcpyHostToDevice));
     }
