@@ -147,6 +147,26 @@ int ec2d_time(int case_id, int nMesh, int *kstep, double *tt);
  * counterpart as an interface: the reference only prints it. */
 int ec2d_dual_inner(int *kt_in);
 
+/* Full-precision checkpoint support: the run state kept outside the arrays.
+ * The arrays (U, Un, Un1, Amu_t of every level and block) travel through
+ * ec2d_get_array / ec2d_put_array; these restore the rest, so that a run
+ * resumed from a checkpoint continues bit for bit as the uninterrupted run.
+ *   ec2d_set_time          sets Kstep and tt of a case's level (the values
+ *                          ec2d_time reads, Mesh_TYPE, opencfd_ec2d_v1.5.4.f90:88-90);
+ *   ec2d_get/set_run_counters  state[0] residual evaluations so far (the first
+ *                          one takes Sfac = 1/(3 dt): the `save` flag Iflag1 of
+ *                          Comput_Residual_one_mesh, sub_Residual.f90:19,28-34),
+ *                          state[1] physical dual-time steps so far (alfa1 =
+ *                          1/dt on the first: the flag Iflag of Dual_time_LU_SGS,
+ *                          sub_NS_singlegrid.f90:239-245), state[2] kt_in of the
+ *                          last physical step.
+ * No reference counterpart as an interface: the reference restarts from
+ * flow2d.dat, written at F20.10 (sub_init.f90:499-560), with Kstep, tt and
+ * both flags starting afresh. */
+int ec2d_set_time(int case_id, int nMesh, int kstep, double tt);
+int ec2d_get_run_counters(long long *state);
+int ec2d_set_run_counters(const long long *state);
+
 int ec2d_synchronize(void);
 
 /* Cells of the batch, and of one case, on a mesh level (Num_Cell, sub_init.f90:134). */

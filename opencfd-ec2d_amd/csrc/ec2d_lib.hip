@@ -2230,6 +2230,31 @@ int ec2d_time(int case_id, int nMesh, int* kstep, double* tt) {
   return 0;
 }
 
+int ec2d_set_time(int case_id, int nMesh, int kstep, double tt) {
+  if (case_id < 0 || case_id >= (int)R.cases.size() || nMesh < 1 || nMesh > 3) return fail("bad case / mesh");
+  if (kstep < 0) return fail("Kstep must not be negative");
+  R.cases[case_id].kstep[nMesh - 1] = kstep;
+  R.cases[case_id].tt[nMesh - 1] = tt;
+  return 0;
+}
+
+int ec2d_get_run_counters(long long* state) {
+  if (!state) return fail("null state");
+  state[0] = R.residual_calls;
+  state[1] = R.dual_steps;
+  state[2] = R.dual_kt;
+  return 0;
+}
+
+int ec2d_set_run_counters(const long long* state) {
+  if (!state) return fail("null state");
+  if (state[0] < 0 || state[1] < 0 || state[2] < 0 || state[2] > INT32_MAX) return fail("bad run counters");
+  R.residual_calls = state[0];
+  R.dual_steps = state[1];
+  R.dual_kt = (int)state[2];
+  return 0;
+}
+
 long long ec2d_num_cells(int case_id, int nMesh) {
   if (!R.committed || nMesh < 1 || nMesh > R.num_mesh) return -1;
   DLevel& L = R.lv[nMesh - 1];

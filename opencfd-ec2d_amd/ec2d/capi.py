@@ -66,6 +66,9 @@ SIGNATURES = {
     "ec2d_res_norms": [_i, _i, _dp, _dp],
     "ec2d_time": [_i, _i, _ip, _dp],
     "ec2d_dual_inner": [_ip],
+    "ec2d_set_time": [_i, _i, _i, C.c_double],
+    "ec2d_get_run_counters": [C.POINTER(C.c_longlong)],
+    "ec2d_set_run_counters": [C.POINTER(C.c_longlong)],
     "ec2d_synchronize": [],
     "ec2d_num_cells": [_i, _i],
     "ec2d_timing": [_dp, _dp, _dp, _dp],

@@ -258,6 +258,86 @@ class Batch:
         rmax, rrms = self.res_norms(case, nmesh)
         return F.append_residual(dirpath or c.dir, self.time(case, nmesh)[0], rmax, rrms, c.ctl.nvar)
 
+    # ---- full-precision checkpoint ---------------------------------------
+    def _state_names(self, c: Case, nmesh: int) -> List[str]:
+        names = ["U", "Un"]
+        if nmesh == 1:
+            if c.ctl.Time_Method == F.TIME_DUAL_LU_SGS:
+                names.append("Un1")
+            if c.ctl.Iflag_turbulence_model != 0:
+                names.append("Amu_t")
+        return names
+
+    def _state_keys(self):
+        for ci, c in enumerate(self.cases):
+            for lev in range(1, c.num_mesh + 1):
+                for mb in range(1, len(c.levels[lev - 1]) + 1):
+                    for n in self._state_names(c, lev):
+                        yield f"c{ci}_m{lev}_b{mb}_{n}", (n, ci, lev, mb)
+
+    def save_checkpoint(self, path: str):
+        """Writes the run state at full precision (FP64, numpy .npz, no pickles):
+        U and Un of every level and block, Un1 (dual-time stepping) and Amu_t
+        (turbulence cases) of the fine level, Kstep and tt of every level, the
+        dual-time counters (ec2d_get_run_counters) and, when a learned closure
+        has parameters, those and its relaxed mu_t.  `load_checkpoint` into a
+        batch built from the same cases continues bit for bit as the
+        uninterrupted run.  The reference's only restart file is flow2d.dat
+        at F20.10 (`output`, opencfd_ec2d_v1.5.4.f90:286-371), which restarts
+        with Kstep = 0 and the dual-time first-step flags set afresh.
+        Switches held on the host (closure enable, relaxation factor) are the
+        caller's to set again."""
+        rec = {"format": np.array([1], np.int64),
+               "layout": np.array([[c.ctl.nvar, c.num_mesh, len(c.levels[0])] for c in self.cases], np.int64)}
+        for key, (n, ci, lev, mb) in self._state_keys():
+            rec[key] = self.get(n, ci, lev, mb)
+        kt = np.zeros((len(self.cases), 3), np.int64)
+        tt = np.zeros((len(self.cases), 3))
+        for ci, c in enumerate(self.cases):
+            for lev in range(1, c.num_mesh + 1):
+                kt[ci, lev - 1], tt[ci, lev - 1] = self.time(ci, lev)
+        counters = (C.c_longlong * 3)()
+        capi.call("ec2d_get_run_counters", counters)
+        rec.update(kstep=kt, tt=tt, counters=np.array(list(counters), np.int64))
+        if self.L.ec2d_closure_get_params(None, None) == 0:
+            rec["closure_theta"], rec["closure_norm"] = self.closure_params()
+            for ci, c in enumerate(self.cases):
+                for mb in range(1, len(c.levels[0]) + 1):
+                    rec[f"c{ci}_m1_b{mb}_closure_mut"] = self.get("closure_mut", ci, 1, mb)
+        with open(path, "wb") as f:
+            np.savez(f, **rec)
+
+    def load_checkpoint(self, path: str):
+        """Restores what `save_checkpoint` wrote; the batch must hold the same
+        cases (variables, levels, blocks and block sizes), else ValueError.
+        Residual norms are those of the next residual evaluation."""
+        with np.load(path, allow_pickle=False) as z:
+            rec = {k: z[k] for k in z.files}
+        if int(rec.get("format", [0])[0]) != 1:
+            raise ValueError(f"{path}: not an ec2d checkpoint")
+        layout = np.array([[c.ctl.nvar, c.num_mesh, len(c.levels[0])] for c in self.cases], np.int64)
+        if rec["layout"].shape != layout.shape or (rec["layout"] != layout).any():
+            raise ValueError(f"{path}: checkpoint of other cases (layout {rec['layout'].tolist()})")
+        for key, (n, ci, lev, mb) in self._state_keys():
+            if key not in rec:
+                raise ValueError(f"{path}: no {key}")
+            B = self.cases[ci].levels[lev - 1][mb - 1]
+            want = self._SHAPES[self.NAMES[n]](B, self._nv(self.cases[ci], lev))
+            if rec[key].shape != want:
+                raise ValueError(f"{path}: {key} has shape {rec[key].shape}, the batch {want}")
+        if "closure_theta" in rec:
+            self.closure_set_params(rec["closure_theta"], rec["closure_norm"])
+            for ci, c in enumerate(self.cases):
+                for mb in range(1, len(c.levels[0]) + 1):
+                    self.put("closure_mut", rec[f"c{ci}_m1_b{mb}_closure_mut"], ci, 1, mb)
+        for key, (n, ci, lev, mb) in self._state_keys():
+            self.put(n, rec[key], ci, lev, mb)
+        for ci, c in enumerate(self.cases):
+            for lev in range(1, c.num_mesh + 1):
+                capi.call("ec2d_set_time", c.id, lev, int(rec["kstep"][ci, lev - 1]), float(rec["tt"][ci, lev - 1]))
+        counters = (C.c_longlong * 3)(*[int(v) for v in rec["counters"]])
+        capi.call("ec2d_set_run_counters", counters)
+
     def dual_inner(self) -> int:
         """kt_in of the last dual-time step (see ec2d_dual_inner)."""
         k = C.c_int()
