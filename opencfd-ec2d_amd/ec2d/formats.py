@@ -347,6 +347,127 @@ def append_residual(dirpath: str, kstep: int, res_max, res_rms, nvar: int) -> st
     return path
 
 
+def _list_digits(x: float):
+    """(digits, e) with x = 0.<digits> * 10**e for a positive finite double,
+    as flang's list-directed output picks them: the fewest significant digits
+    n for which n-digit decimals lie strictly between x and its neighbours'
+    midpoints, and of those the middle one (the lower of two), which is not
+    always the one nearest to x (1.6954791700008828 for the double nearest
+    1.69547917000088288475)."""
+    import decimal
+    import math
+
+    with decimal.localcontext() as ctx:
+        ctx.prec = 1200  # a double's exact decimal expansion has at most 767 digits
+        D = decimal.Decimal
+        ex = D(x)
+        up = D(math.ulp(x))
+        # below a power of two the spacing halves (normal range only)
+        dn = up / 2 if (math.frexp(x)[0] == 0.5 and x >= 2.0 ** -1021) else up
+        lo, hi = ex - dn / 2, ex + up / 2
+        adj = ex.adjusted()
+        for n in range(1, 18):
+            q = D(1).scaleb(adj - n + 1)
+            a, b = lo / q, hi / q
+            cmin = a.to_integral_value(decimal.ROUND_CEILING)
+            cmax = b.to_integral_value(decimal.ROUND_FLOOR)
+            if cmin == a:
+                cmin += 1
+            if cmax == b:
+                cmax -= 1
+            if cmin <= cmax:
+                t = (((cmin + cmax) // 2) * q).normalize().as_tuple()
+                digits = "".join(map(str, t.digits))
+                return digits, len(digits) + t.exponent
+    raise ValueError(x)
+
+
+def fortran_list_real(x: float) -> str:
+    """One REAL(8) item of list-directed output as flang's runtime edits it.
+
+    The digits are those of `_list_digits`.  The form is F when x, rounded to
+    one significant digit, lies in [0.1, 1e15): no leading zero (".5"), a
+    trailing point on whole numbers ("12.", "0.").  Otherwise it is 1P E with
+    a two-digit minimum exponent ("1.E-05", "2.5E+300").  Zero keeps its sign
+    ("-0.")."""
+    import decimal
+
+    x = float(x)
+    if x != x:
+        return "NaN"
+    sign = "-" if np.signbit(x) else ""
+    if x in (float("inf"), float("-inf")):
+        return sign + "Inf"
+    if x == 0.0:
+        return sign + "0."
+    digits, e = _list_digits(abs(x))
+    with decimal.localcontext() as ctx:
+        ctx.prec = 1200
+        ax = decimal.Decimal(abs(x))
+        r = ax.quantize(decimal.Decimal(1).scaleb(ax.adjusted()), rounding=decimal.ROUND_HALF_EVEN)
+        f_form = decimal.Decimal("0.1") <= r < decimal.Decimal("1e15")
+    if f_form:
+        if e <= 0:
+            return sign + "." + "0" * (-e) + digits
+        return sign + digits[:e].ljust(e, "0") + "." + digits[e:]
+    return sign + digits[0] + "." + digits[1:] + "E" + ("-" if e - 1 < 0 else "+") + f"{abs(e - 1):02d}"
+
+
+def fortran_list_line(*items) -> str:
+    """A list-directed record: a leading blank, items separated by one blank;
+    strings as they are, integers in minimal width, reals as fortran_list_real."""
+    out = []
+    for v in items:
+        if isinstance(v, str):
+            out.append(v)
+        elif isinstance(v, (int, np.integer)):
+            out.append(str(int(v)))
+        else:
+            out.append(fortran_list_real(v))
+    return " " + " ".join(out)
+
+
+def write_wall_cp(dirpath: str, case_dir: str = None) -> str:
+    """Cp.dat of the reference's wall-pressure post-processor (read_flow-1.2.f90:24-53):
+    Ma from control.in, the blocks' sizes from Mesh2d.dat, the subfaces from
+    bc2d_1.in (all in case_dir, default dirpath) and the flow from dirpath's
+    flow2d.dat (the F20.10 file `output` writes).  For every wall subface
+    (bc -10) a zone of x1(i,j) and -(p(i,j)-p00)*2 over its node-index range
+    ibegin..iend, jbegin..jend, j outer; p00 = 1/(1.4 Ma^2) (gamma fixed at 1.4
+    in the tool).  The cell-centred values are read at node indices, as the
+    tool does."""
+    case_dir = case_dir or dirpath
+    with open(os.path.join(case_dir, "control.in")) as f:
+        f.readline()
+        ma = _fnum(f.readline().split()[0])
+    p00 = 1.0 / (1.4 * ma * ma)
+    tok = _tokens(os.path.join(case_dir, "Mesh2d.dat"))
+    dims = [(int(tok[1 + 2 * m]), int(tok[2 + 2 * m])) for m in range(int(tok[0]))]
+    bcs = read_bcin(os.path.join(case_dir, "bc2d_1.in"))
+    lines = [fortran_list_line("variable=x,Cp")]
+    with open(os.path.join(dirpath, "flow2d.dat")) as f:
+        f.readline()
+        flows = []
+        for nx, ny in dims:
+            f.readline()
+            # the tool reads the first 7 items of each record (x1, y1, d, u, v, T, p)
+            q = np.array([[_fnum(t) for t in f.readline().split()[:7]] for _ in range((nx + 1) * (ny + 1))])
+            flows.append(q.reshape(ny + 1, nx + 1, 7))
+    for subs, q in zip(bcs, flows):
+        for sb in subs:
+            if sb[6] != BC_WALL:
+                continue
+            ib, ie, jb, je = sb[2], sb[3], sb[4], sb[5]
+            lines.append(fortran_list_line("zone i=", (je - jb + 1) * (ie - ib + 1)))
+            for j in range(jb, je + 1):
+                for i in range(ib, ie + 1):
+                    lines.append(fortran_list_line(q[j, i, 0], -(q[j, i, 6] - p00) * 2.0))
+    path = os.path.join(dirpath, "Cp.dat")
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    return path
+
+
 def fortran_pow_int(x, n: int):
     """x**n for a constant integer n as flang evaluates it: ((x*x)*x)*... ."""
     r = x

@@ -251,6 +251,15 @@ class Batch:
         k = c.ctl
         return F.write_output(dirpath or c.dir, nmesh, kstep, blocks, self._nv(c, nmesh), k.Cv, k.gamma, k.Ma, k.Re)
 
+    def wall_cp(self, case: int = 0, dirpath: str = None) -> str:
+        """Cp.dat of the reference's wall-pressure tool (read_flow-1.2.f90),
+        from the device state: `output(case, 1, dirpath)` writes flow2d.dat,
+        formats.write_wall_cp reads it back as the tool does.  Returns the path."""
+        c = self.cases[case]
+        d = dirpath or c.dir
+        self.output(case, 1, d)
+        return F.write_wall_cp(d, c.dir)
+
     def output_res(self, case: int = 0, nmesh: int = 1, dirpath: str = None) -> str:
         """output_Res(nMesh) (opencfd_ec2d_v1.5.4.f90:266-281): appends the
         line of Kstep, Res_max(1:6) and Res_rms(1:NVAR) to Residual.dat."""

@@ -6,7 +6,9 @@ append_residual reproduce the reference's files byte for byte from the
 oracle's state.  Here Batch.output / output_res write them from the GPU's
 state: for Euler cases, whose state matches the oracle bit for bit, the flow
 files must be byte-identical to the oracle's; Residual.dat holds tree-summed
-norms (1e-12 relative), compared as numbers to 1e-9.
+norms (1e-12 relative), compared as numbers to 1e-9.  Cp.dat, the wall-
+pressure tool's file (read_flow-1.2.f90), follows from flow2d.dat and must be
+byte-identical too.
 """
 import os
 
@@ -49,7 +51,9 @@ def test_output_files_from_device(mg, tmp_path):
         b.output_res(0, 1, str(gdir))
         mx, rms = ref.norms(1)
         F.append_residual(str(odir), ref.kstep[0], mx, rms, c.nvar)
-    assert "flow2d.dat" in names and (not mg or "flow2d-2.dat" in names)
+        names.append(os.path.basename(b.wall_cp(0, str(gdir))))
+        F.write_wall_cp(str(odir), d)
+    assert "flow2d.dat" in names and (not mg or "flow2d-2.dat" in names) and "Cp.dat" in names
     for n in names:
         assert (gdir / n).read_text() == (odir / n).read_text(), n
     g = (gdir / "Residual.dat").read_text().split()

@@ -105,6 +105,30 @@ def test_fortran_edit_descriptors():
     assert F.fortran_e(1e-120, 20) == "    0.1000000000-119"
 
 
+def test_list_directed_reals_match_flang():
+    """formats.fortran_list_real against what flang's runtime printed for 3326
+    doubles (tests/golden/flang_list_directed.npz, tests/golden/make_list_directed.py):
+    random bit patterns over the whole range, subnormals, powers of two and
+    ten with their neighbours, the edges of the F/E switch.  About one value
+    in six of the random ones has digits other than the nearest shortest
+    decimal (Python's repr), so a repr-based formatter fails here."""
+    import struct
+
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "flang_list_directed.npz"))
+    bad = []
+    n_repr = 0
+    for b, t in zip(z["bits"], z["text"]):
+        x = struct.unpack("<d", struct.pack("<q", int(b)))[0]
+        want = t.decode()
+        if F.fortran_list_real(x) != want:
+            bad.append((repr(x), F.fortran_list_real(x), want))
+        n_repr += float(want) == x and repr(abs(x)).lstrip("0").replace("e", "E") not in want
+    assert not bad, bad[:5]
+    assert n_repr > 100  # the fixture holds values where nearest-digit output would differ
+    assert F.fortran_list_line("zone i=", 41) == " zone i= 41"
+    assert F.fortran_list_line(0.5, -0.0, 12.0) == " .5 -0. 12."
+
+
 def test_turbulence_restart_is_refused(tmp_path):
     """The reference defines no restart of turbulence fields
     (sub_init.f90:527-553): the host side refuses it instead of inventing one."""

@@ -144,6 +144,41 @@ def test_output_files_match_reference(kind, tmp_path):
         assert got == want, f"{name}: first difference at {next(i for i, (x, y) in enumerate(zip(got, want)) if x != y) if got[:len(want)] != want[:len(got)] else min(len(got), len(want))}"
 
 
+READ_FLOW = os.path.join(os.path.dirname(refpy.LIB_PATH), "read_flow")
+
+
+@pytest.mark.skipif(not os.path.exists(READ_FLOW), reason="reference wall-Cp tool (oracle/_ref/read_flow) not built")
+@pytest.mark.parametrize("kind", ["sa", "walls"])
+def test_wall_cp_matches_reference_tool(kind, tmp_path):
+    """formats.write_wall_cp against the reference's own post-processor
+    (read_flow-1.2.f90, built unmodified into oracle/_ref/read_flow) on the
+    flow2d.dat the reference writes: Cp.dat byte for byte.  "walls": a
+    multi-block layout with wall subfaces on two faces of several blocks
+    (several zones; the tool reads cell values at node indices)."""
+    import shutil
+
+    if kind == "sa":
+        d = naca_case("r_cp_sa", cases.control_sa_naca(iters=10, Re=1e5, Ma=0.5), ds1=1e-4)
+    else:
+        d = vortex_case("r_cp_walls", cases.control_vortex(iters=10), bc_edit=_walls)
+    R = refpy.RefCase(d)
+    for _ in range(2):
+        R.op("main_step")
+    R.op("output", 1)
+    ref, mine = tmp_path / "ref", tmp_path / "mine"
+    ref.mkdir()
+    mine.mkdir()
+    for n in ("control.in", "Mesh2d.dat", "bc2d_1.in", "flow2d.dat"):
+        shutil.copy(os.path.join(d, n), ref / n)
+    shutil.copy(os.path.join(d, "flow2d.dat"), mine / "flow2d.dat")
+    subprocess.run([READ_FLOW], cwd=ref, check=True, capture_output=True, timeout=120)
+    F.write_wall_cp(str(mine), d)
+    want = (ref / "Cp.dat").read_text()
+    got = (mine / "Cp.dat").read_text()
+    assert want.count("zone") >= (1 if kind == "sa" else 2)
+    assert got == want
+
+
 def _symm(bcs):
     for subs in bcs:
         for s in subs:
