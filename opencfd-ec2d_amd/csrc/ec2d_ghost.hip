@@ -33,75 +33,99 @@ struct GhostArgs {
   int do_bc, do_exchange;
 };
 
+// Every point reads only cells it does not write (interior or inner layers
+// on the source side, ghost layers on the other), but the compiler cannot
+// prove that the planes do not overlap, so a load placed after a store waits
+// for the store's operands: a point that interleaved them paid one memory
+// round trip per variable and layer.  Each branch therefore loads all it
+// reads first, then computes and stores.
 template <int NV>
 __device__ void bc_point(const GhostArgs& a, const BlockDesc& B, const CaseParams& cp, const SubfaceDesc& sf, int i,
                          int j, int pt, double* const* U) {
   const double g = cp.gamma;
   const int face = sf.face;
   const int type = sf.neighb;
-  if (type == -10) {  // wall, sub_boundary.f90:306-386
+  if (type == -10 || type == -30) {  // wall (sub_boundary.f90:306-386), symmetry / slip wall (:391-451)
+    int64_t q1[EC2D_LAP], q2[EC2D_LAP];
+    double v[EC2D_LAP][NV], dwv[EC2D_LAP];
+#pragma unroll
     for (int k = 1; k <= EC2D_LAP; k++) {
       int i1, j1, i2, j2;
       if (face == 1) { i1 = i + k - 1; j1 = j; i2 = i - k; j2 = j; }
       else if (face == 2) { i1 = i; j1 = j + k - 1; i2 = i; j2 = j - k; }
       else if (face == 3) { i1 = i - k; j1 = j; i2 = i + k - 1; j2 = j; }
       else { i1 = i; j1 = j - k; i2 = i; j2 = j + k - 1; }
-      int64_t q1 = pidx(B, i1, j1), q2 = pidx(B, i2, j2);
-      double d1 = 0.0;
-      if (cp.Twall <= 0) {
-        U[0][q2] = U[0][q1];
-        U[1][q2] = -U[1][q1];
-        U[2][q2] = -U[2][q1];
-        U[3][q2] = U[3][q1];
-      } else {
-        d1 = U[0][q1];
-        double u1 = U[1][q1] / d1, v1 = U[2][q1] / d1;
-        double p1 = (U[3][q1] - 0.5 * d1 * (u1 * u1 + v1 * v1)) * (g - 1.0);
-        double T1 = g * cp.Ma * cp.Ma * p1 / d1;
-        double p2 = p1, T2 = 2.0 * cp.Twall - T1, u2 = -u1, v2 = -v1;
-        double d2 = g * cp.Ma * cp.Ma * p2 / T2;
-        U[0][q2] = d2;
-        U[1][q2] = d2 * u2;
-        U[2][q2] = d2 * v2;
-        U[3][q2] = p2 / (g - 1.0) + 0.5 * d2 * (u2 * u2 + v2 * v2);
-      }
-      if constexpr (NV == 5) U[4][q2] = 0.0;
+      q1[k - 1] = pidx(B, i1, j1);
+      q2[k - 1] = pidx(B, i2, j2);
+#pragma unroll
+      for (int m = 0; m < NV; m++) v[k - 1][m] = U[m][q1[k - 1]];
       if constexpr (NV == 6) {
-        double d2 = U[0][q2];
-        double T2 = (U[3][q2] - 0.5 * (U[1][q2] * U[1][q2] + U[2][q2] * U[2][q2]) / d2) / (cp.Cv * d2);
-        double Amu2 = (1.0 + cp.Tsb) * sqrt(T2 * T2 * T2) / (cp.Tsb + T2);
-        double dwv = (a.arena + B.base + (int64_t)a.pm.dw * B.plane)[q1];
-        double wt = 60.0 * Amu2 / (d2 * 0.075 * (dwv * dwv) * cp.Re * cp.Re);
-        U[4][q2] = -U[4][q1];
-        U[5][q2] = (d1 + d2) * wt - U[5][q1];
+        if (type == -10) dwv[k - 1] = (a.arena + B.base + (int64_t)a.pm.dw * B.plane)[q1[k - 1]];
       }
     }
-  } else if (type == -30) {  // symmetry / slip wall, sub_boundary.f90:391-451
-    double n1 = a.normals[2 * (sf.norm_off + pt)], n2 = a.normals[2 * (sf.norm_off + pt) + 1];
-    for (int k = 1; k <= EC2D_LAP; k++) {
-      int i1, j1, i2, j2;
-      if (face == 1) { i1 = i + k - 1; j1 = j; i2 = i - k; j2 = j; }
-      else if (face == 2) { i1 = i; j1 = j + k - 1; i2 = i; j2 = j - k; }
-      else if (face == 3) { i1 = i - k; j1 = j; i2 = i + k - 1; j2 = j; }
-      else { i1 = i; j1 = j - k; i2 = i; j2 = j + k - 1; }
-      int64_t q1 = pidx(B, i1, j1), q2 = pidx(B, i2, j2);
-      double Vn = U[1][q1] * n1 + U[2][q1] * n2;
-      U[0][q2] = U[0][q1];
-      U[1][q2] = U[1][q1] - 2.0 * Vn * n1;
-      U[2][q2] = U[2][q1] - 2.0 * Vn * n2;
-      U[3][q2] = U[3][q1];
+    if (type == -10) {
 #pragma unroll
-      for (int m = 4; m < NV; m++) U[m][q2] = U[m][q1];
+      for (int k = 0; k < EC2D_LAP; k++) {
+        const double* w = v[k];
+        double d1 = 0.0, s0, s1, s2, s3;
+        if (cp.Twall <= 0) {
+          s0 = w[0];
+          s1 = -w[1];
+          s2 = -w[2];
+          s3 = w[3];
+        } else {
+          d1 = w[0];
+          double u1 = w[1] / d1, v1 = w[2] / d1;
+          double p1 = (w[3] - 0.5 * d1 * (u1 * u1 + v1 * v1)) * (g - 1.0);
+          double T1 = g * cp.Ma * cp.Ma * p1 / d1;
+          double p2 = p1, T2 = 2.0 * cp.Twall - T1, u2 = -u1, v2 = -v1;
+          double d2 = g * cp.Ma * cp.Ma * p2 / T2;
+          s0 = d2;
+          s1 = d2 * u2;
+          s2 = d2 * v2;
+          s3 = p2 / (g - 1.0) + 0.5 * d2 * (u2 * u2 + v2 * v2);
+        }
+        U[0][q2[k]] = s0;
+        U[1][q2[k]] = s1;
+        U[2][q2[k]] = s2;
+        U[3][q2[k]] = s3;
+        if constexpr (NV == 5) U[4][q2[k]] = 0.0;
+        if constexpr (NV == 6) {
+          // the ghost state just written, as the reference reads it back
+          double d2 = s0;
+          double T2 = (s3 - 0.5 * (s1 * s1 + s2 * s2) / d2) / (cp.Cv * d2);
+          double Amu2 = (1.0 + cp.Tsb) * sqrt(T2 * T2 * T2) / (cp.Tsb + T2);
+          double wt = 60.0 * Amu2 / (d2 * 0.075 * (dwv[k] * dwv[k]) * cp.Re * cp.Re);
+          U[4][q2[k]] = -w[4];
+          U[5][q2[k]] = (d1 + d2) * wt - w[5];
+        }
+      }
+    } else {
+      double n1 = a.normals[2 * (sf.norm_off + pt)], n2 = a.normals[2 * (sf.norm_off + pt) + 1];
+#pragma unroll
+      for (int k = 0; k < EC2D_LAP; k++) {
+        const double* w = v[k];
+        double Vn = w[1] * n1 + w[2] * n2;
+        U[0][q2[k]] = w[0];
+        U[1][q2[k]] = w[1] - 2.0 * Vn * n1;
+        U[2][q2[k]] = w[2] - 2.0 * Vn * n2;
+        U[3][q2[k]] = w[3];
+#pragma unroll
+        for (int m = 4; m < NV; m++) U[m][q2[k]] = w[m];
+      }
     }
   } else {  // far field / inlet / outlet, sub_boundary.f90:457-581
     int i1 = i, j1 = j;
     if (face == 3) i1 = i - 1;
     else if (face == 4) j1 = j - 1;
     int64_t q1 = pidx(B, i1, j1);
-    double d1 = U[0][q1], u1 = U[1][q1] / d1, v1 = U[2][q1] / d1;
-    double p1 = (U[3][q1] - 0.5 * d1 * (u1 * u1 + v1 * v1)) * (g - 1.0);
-    double c1 = sqrt(g * p1 / d1);
+    double w[NV];
+#pragma unroll
+    for (int m = 0; m < NV; m++) w[m] = U[m][q1];
     double n1 = a.normals[2 * (sf.norm_off + pt)], n2 = a.normals[2 * (sf.norm_off + pt) + 1];
+    double d1 = w[0], u1 = w[1] / d1, v1 = w[2] / d1;
+    double p1 = (w[3] - 0.5 * d1 * (u1 * u1 + v1 * v1)) * (g - 1.0);
+    double c1 = sqrt(g * p1 / d1);
     double Ma_n = (u1 * n1 + v1 * n2) / c1;
     const double d_inf = 1.0;
     for (int k = 1; k <= EC2D_LAP; k++) {
@@ -114,7 +138,7 @@ __device__ void bc_point(const GhostArgs& a, const BlockDesc& B, const CaseParam
       if (type == -50 || (type == -20 && Ma_n >= 0.0)) {
         if (Ma_n > 1.0 || cp.P_outlet < 0 || type == -20) {
 #pragma unroll
-          for (int m = 0; m < NV; m++) U[m][q2] = U[m][q1];
+          for (int m = 0; m < NV; m++) U[m][q2] = w[m];
         } else {
           double pb = cp.P_outlet;
           double db = d1 + (cp.P_outlet - p1) / (c1 * c1);
@@ -126,7 +150,7 @@ __device__ void bc_point(const GhostArgs& a, const BlockDesc& B, const CaseParam
           U[2][q2] = d2 * v2;
           U[3][q2] = p2 / (g - 1.0) + 0.5 * d2 * (u2 * u2 + v2 * v2);
 #pragma unroll
-          for (int m = 4; m < NV; m++) U[m][q2] = U[m][q1];
+          for (int m = 4; m < NV; m++) U[m][q2] = w[m];
         }
       } else {
         if (cp.P_outlet < 0 || Ma_n <= -1.0) {
@@ -161,12 +185,15 @@ __device__ __forceinline__ void corner(double* const* U, int64_t q1, int64_t q2,
   double d2 = U[0][q2], uu2 = U[1][q2] / d2, v2 = U[2][q2] / d2, T2 = (U[3][q2] - (uu2 * U[1][q2] + v2 * U[2][q2]) * 0.5) / (d2 * Cv);
   double d3 = U[0][q3], uu3 = U[1][q3] / d3, v3 = U[2][q3] / d3, T3 = (U[3][q3] - (uu3 * U[1][q3] + v3 * U[2][q3]) * 0.5) / (d3 * Cv);
   double d4 = d1 + d3 - d2, uu4 = uu1 + uu3 - uu2, v4 = v1 + v3 - v2, T4 = T1 + T3 - T2;
+  double t4[NV > 4 ? NV - 4 : 1];
+#pragma unroll
+  for (int m = 4; m < NV; m++) t4[m - 4] = U[m][q1] + U[m][q3] - U[m][q2];
   U[0][q4] = d4;
   U[1][q4] = d4 * uu4;
   U[2][q4] = d4 * v4;
   U[3][q4] = d4 * (Cv * T4 + (uu4 * uu4 + v4 * v4) * 0.5);
 #pragma unroll
-  for (int m = 4; m < NV; m++) U[m][q4] = U[m][q1] + U[m][q3] - U[m][q2];
+  for (int m = 4; m < NV; m++) U[m][q4] = t4[m - 4];
 }
 
 template <int NV>
@@ -226,8 +253,11 @@ __global__ __launch_bounds__(1024) void k_ghost(GhostArgs a) {
         else if (sf.face == 3) { di = sf.iend + l - 1; dj = sf.jst + kk - 1; }
         else { di = sf.ist + kk - 1; dj = sf.jend + l - 1; }
         const int64_t qs = pidx(B1, si_, sj_), qd = pidx(B, di, dj);
+        double w[NV];
 #pragma unroll
-        for (int m = 0; m < NV; m++) U[m][qd] = U1[m][qs];
+        for (int m = 0; m < NV; m++) w[m] = U1[m][qs];
+#pragma unroll
+        for (int m = 0; m < NV; m++) U[m][qd] = w[m];
       }
     }
     u0 = u1;
