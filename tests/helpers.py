@@ -57,3 +57,40 @@ def first_diff(a, b):
     with np.errstate(divide="ignore", invalid="ignore"):
         rel = np.nanmax(np.abs(a - b) / np.maximum(np.abs(b), 1e-300))
     return f"{len(bad)} differ, first at {k}: {a[k]!r} vs {b[k]!r}, max rel {rel:.3e}"
+
+
+BC_CHOICES = (F.BC_FARFIELD, F.BC_WALL, F.BC_SYMMETRY, F.BC_INLET, F.BC_OUTLET)
+
+
+def random_case(seed: int, prefix: str):
+    """A seeded random multi-block case: 1..3 x 1..3 blocks of 9..21 x 9..17
+    points, each outer subface a random boundary type (far field, wall,
+    symmetry, inlet, outlet), random scheme / flux / reconstruction, inviscid
+    or laminar with an adiabatic or isothermal wall, LU-SGS, RK3 or first-order
+    Euler, with or without a back pressure.  Returns (case directory, steps)."""
+    rng = np.random.default_rng(seed)
+    c = cases.control_vortex(iters=10)
+    c.Iflag_Scheme = int(rng.integers(0, 7))
+    c.Iflag_Flux = int(rng.integers(1, 7))
+    c.IFlag_Reconstruction = int(rng.integers(0, 3))
+    if rng.random() < 0.5:
+        c.If_viscous, c.Re = 1, float(10.0 ** rng.uniform(3, 5))
+        c.Twall = 300.0 if rng.random() < 0.5 else -1.0
+    tm = int(rng.choice([F.TIME_LU_SGS, F.TIME_RK3, F.TIME_EULER1]))
+    c.Time_Method = tm
+    c.CFL = {F.TIME_LU_SGS: 2.0, F.TIME_RK3: 0.5, F.TIME_EULER1: 0.3}[tm]
+    if rng.random() < 0.5:
+        c.P_outlet = 1.0 / (1.4 * 0.5 * 0.5) * float(rng.uniform(0.95, 1.0))
+    kinds = [int(v) for v in rng.integers(0, len(BC_CHOICES), size=64)]
+
+    def edit(bcs):
+        k = 0
+        for subs in bcs:
+            for s in subs:
+                if s[6] == F.BC_FARFIELD:
+                    s[6] = BC_CHOICES[kinds[k % len(kinds)]]
+                    k += 1
+
+    d = vortex_case(f"{prefix}_rand_{seed}", c, seed=seed, nbx=int(rng.integers(1, 4)), nby=int(rng.integers(1, 4)),
+                    bc_edit=edit)
+    return d, 3

@@ -311,3 +311,15 @@ def test_geometry_matches_reference(fdm):
                 got, want = got[:-1, :-1], want[:-1, :-1]
             diff = first_diff(got, want)
             assert diff is None, f"level {lev} {n}: {diff}"
+
+
+@pytest.mark.parametrize("seed", range(24))
+def test_random_cases(seed):
+    """Seeded random multi-block cases (layout, boundary type per subface,
+    scheme, flux, reconstruction, viscosity, wall temperature, time method,
+    back pressure; tests/helpers.py random_case): the oracle equals the
+    reference bit for bit after every pass."""
+    from helpers import random_case
+
+    d, n = random_case(seed, "r")
+    run_both(d, n, ("U", "Res", "dt"))
