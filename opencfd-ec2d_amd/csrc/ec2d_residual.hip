@@ -480,6 +480,25 @@ __global__ __launch_bounds__(TX* TY) void k_residual(ResArgs a) {
       }
     }
   }
+  if constexpr (VISC) {
+    // mu_t of the first ghost layer (i = 0 or nx, j = 0 or ny, corners
+    // included), as the reference's array holds it after Amut_boundary; the
+    // output files carry it.  A ring cell belongs to the tile whose interior
+    // holds the cell next to it (its indices clamped to 1..nx-1, 1..ny-1).
+    // SST and Baldwin-Lomax read these cells from the plane above: the store
+    // writes back the value read.
+    if (a.write_visc) {
+      for (int idx = tid; idx < RW * RH; idx += NT) {
+        int li = idx % RW, lj = idx / RW;
+        int i = i0 - 2 + li, j = j0 - 2 + lj;
+        if (i < 0 || i > nx || j < 0 || j > ny) continue;
+        if (i != 0 && i != nx && j != 0 && j != ny) continue;
+        int ci = min(max(i, 1), nx - 1), cj = min(max(j, 1), ny - 1);
+        if (ci < i0 || ci >= i0 + TX || cj < j0 || cj >= j0 + TY) continue;
+        PL(pm.Amu_t)[pidx(B, i, j)] = sAt[lj][li];
+      }
+    }
+  }
   // ---- tile reduction (fixed tree: wave butterfly, then waves in order) ----
 #pragma unroll
   for (int m = 0; m < NV; m++) {

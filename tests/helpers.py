@@ -94,3 +94,35 @@ def random_case(seed: int, prefix: str):
     d = vortex_case(f"{prefix}_rand_{seed}", c, seed=seed, nbx=int(rng.integers(1, 4)), nby=int(rng.integers(1, 4)),
                     bc_edit=edit)
     return d, 3
+
+
+def random_naca_case(seed: int, prefix: str):
+    """A seeded random single-block C-grid case: grid 49..73 x 13..21 points,
+    inviscid, laminar, Baldwin-Lomax, SA or SST (isothermal wall), random
+    scheme / flux / reconstruction, optionally an FD-WENO5 block, one to three
+    mesh levels with LU-SGS, RK3 or first-order Euler.  Returns (case
+    directory, steps, exact, reference) - exact: whether the GPU must match the
+    oracle bit for bit (no exp / pow / tanh of Baldwin-Lomax, SA or SST); reference: whether
+    the reference runs the case (it refuses LU-SGS on more than one level)."""
+    rng = np.random.default_rng(1000 + seed)
+    turb = int(rng.choice([F.TURB_NONE, F.TURB_NONE, F.TURB_BL, F.TURB_SA, F.TURB_SST]))
+    nm = int(rng.integers(1, 4))
+    tm = int(rng.choice([F.TIME_LU_SGS, F.TIME_RK3, F.TIME_EULER1]))
+    if turb == F.TURB_NONE and rng.random() < 0.5:
+        c = cases.control_euler_naca(iters=10)
+    else:
+        c = cases.control_sa_naca(iters=10, Re=float(10.0 ** rng.uniform(4.5, 5.5)), Ma=0.5)
+        c.Iflag_turbulence_model = turb
+        if turb == F.TURB_SST:
+            c.Twall = 300.0
+    c.Iflag_Scheme = int(rng.integers(0, 7))
+    c.Iflag_Flux = int(rng.integers(1, 7))
+    c.IFlag_Reconstruction = int(rng.integers(0, 3))
+    c.Time_Method, c.Num_Mesh = tm, nm
+    c.CFL = {F.TIME_LU_SGS: 2.0, F.TIME_RK3: 0.3, F.TIME_EULER1: 0.2}[tm]
+    nx = int(rng.choice([49, 57, 65, 73]))
+    ny = int(rng.choice([13, 17, 21]))
+    fdm = (1,) if rng.random() < 0.3 else ()
+    d = naca_case(f"{prefix}_rnaca_{seed}", c, nx=nx, ny=ny, ds1=1e-4 if c.If_viscous else 1e-3, fdm=fdm)
+    exact = turb == F.TURB_NONE
+    return d, 3, exact, not (tm == F.TIME_LU_SGS and nm > 1)

@@ -16,3 +16,25 @@ pytestmark = pytest.mark.gpu
 def test_random_case(seed):
     d, n = random_case(seed, "g")
     _compare_steps(d, n)
+
+
+@pytest.mark.parametrize("seed", range(24))
+def test_random_naca_case(seed):
+    """Seeded random C-grid cases (tests/helpers.py random_naca_case),
+    LU-SGS multigrid included.  Bit-exact without a turbulence model; with
+    Baldwin-Lomax, SA or SST U, Res and mu_t (Amu_t, its first ghost layer
+    included) within 1e-11 relative per pass and the norms within 1e-10 (the
+    device evaluates exp, pow and tanh with its own math library)."""
+    from helpers import random_naca_case
+    from ec2d import formats as F
+    import os
+
+    d, n, exact, _ = random_naca_case(seed, "g")
+    c = F.read_control(os.path.join(d, "control.in"))
+    arrays = ("U", "Res") if c.Num_Mesh > 1 else ("U", "Res", "dt", "Lci", "Lcj")
+    if c.Iflag_turbulence_model != F.TURB_NONE:
+        arrays += ("Amu_t",)
+    if exact:
+        _compare_steps(d, n, arrays)
+    else:
+        _compare_steps(d, n, arrays, exact=False, rtol=1e-11, norm_rtol=1e-10)

@@ -323,3 +323,20 @@ def test_random_cases(seed):
 
     d, n = random_case(seed, "r")
     run_both(d, n, ("U", "Res", "dt"))
+
+
+@pytest.mark.parametrize("seed", range(24))
+def test_random_naca_cases(seed):
+    """Seeded random C-grid cases (turbulence model, scheme, flux,
+    reconstruction, FD-WENO5 block, mesh levels, time method;
+    tests/helpers.py random_naca_case): the oracle equals the reference bit
+    for bit.  LU-SGS on several levels is this build's extension, which the
+    reference refuses: those seeds are skipped here."""
+    from helpers import random_naca_case
+
+    d, n, _, reference = random_naca_case(seed, "r")
+    if not reference:
+        pytest.skip("LU-SGS multigrid: the reference refuses it (opencfd_ec2d_v1.5.4.f90:216-221)")
+    c = F.read_control(os.path.join(d, "control.in"))
+    arrays = ("U", "Res", "dt") + (("Amu_t",) if c.Iflag_turbulence_model != F.TURB_NONE else ())
+    run_both(d, n, arrays)
