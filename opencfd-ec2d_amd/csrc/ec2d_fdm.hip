@@ -4,12 +4,13 @@
 // The interior (more than LFDM = 4 cells from the block edge) uses
 // Steger-Warming flux-vector splitting at cell centres (:154-202) and 5th-order
 // WENO-JS along grid lines (:207-275), plus central viscous fluxes (:86-134).
-// The fused FVM residual kernel skips these faces and cells; three launches
+// The fused FVM residual kernel skips these faces and cells; two launches
 // here produce the interior residual:
-//   k_fdm_lines  one workgroup per grid line, i-lines then j-lines in one launch: the split
+//   k_fdm_lines  one workgroup per grid line, i-lines then j-lines: the split
 //                fluxes of the whole line in LDS, WENO5 at every interface,
-//                interface fluxes F(i+1/2) to scratch planes;
-//   k_fdm_visc   viscous Ev1 / Ev2 at cell centres (scratch planes);
+//                interface fluxes F(i+1/2) to scratch planes; viscous cases
+//                add one workgroup per row for Ev1 / Ev2 at cell centres
+//                (scratch planes, fdm_visc_row);
 //   k_fdm_res    Res = F(i+1/2)-F(i-1/2) + G(j+1/2)-G(j-1/2) for the interior.
 // Line length is at most 992 cells (checked on the host).
 #include "ec2d_device.h"
@@ -20,10 +21,12 @@ namespace ec2d {
 struct FdmArgs {
   const BlockDesc* blocks;
   const int2* lines;   // (block, line index): j for i-lines, i for j-lines
+  const int2* rows;    // (block, j) of every interior row
   const CaseParams* cps;
   double* arena;
   PlaneMap pm;
-  int n_i;             // k_fdm_lines: lines[0, n_i) are i-lines, the rest j-lines
+  int n_i;             // k_fdm_lines: lines[0, n_i) are i-lines, lines[n_i, n_lines) j-lines,
+  int n_lines;         // workgroups from n_lines on take the viscous fluxes of rows[blockIdx.x - n_lines]
   int fx, fy;          // first scratch plane of the i- and j-direction interface fluxes (4 each)
   int ev;              // first scratch plane of Ev1 (3) and Ev2 (3)
   int viscous;
@@ -94,8 +97,14 @@ __device__ __forceinline__ double weno_m(double vm2, double vm1, double v0, doub
   return W0 * q03 + W1 * q13 + W2 * q23;
 }
 
+__device__ void fdm_visc_row(const FdmArgs& a, int2 bl);
+
 __global__ __launch_bounds__(256) void k_fdm_lines(FdmArgs a) {
   __shared__ double sfp[4][FDM_MAXN + 1], sfm[4][FDM_MAXN + 1];
+  if ((int)blockIdx.x >= a.n_lines) {
+    fdm_visc_row(a, a.rows[blockIdx.x - a.n_lines]);
+    return;
+  }
   const int2 bl = a.lines[blockIdx.x];
   const int dir = (int)blockIdx.x < a.n_i ? 0 : 1;
   const BlockDesc B = a.blocks[bl.x];
@@ -142,8 +151,7 @@ __global__ __launch_bounds__(256) void k_fdm_lines(FdmArgs a) {
 }
 
 // viscous fluxes at cell centres 2..n-1 (sub_Finite_Difference.f90:86-117), one workgroup per row j
-__global__ __launch_bounds__(256) void k_fdm_visc(FdmArgs a) {
-  const int2 bl = a.lines[blockIdx.x];
+__device__ void fdm_visc_row(const FdmArgs& a, int2 bl) {
   const BlockDesc B = a.blocks[bl.x];
   const CaseParams cp = a.cps[B.case_id];
   const PlaneMap pm = a.pm;
@@ -191,7 +199,7 @@ __global__ __launch_bounds__(256) void k_fdm_visc(FdmArgs a) {
 
 // interior residual (sub_Finite_Difference.f90:122-145), one workgroup per row j
 __global__ __launch_bounds__(256) void k_fdm_res(FdmArgs a) {
-  const int2 bl = a.lines[blockIdx.x];
+  const int2 bl = a.rows[blockIdx.x];
   const BlockDesc B = a.blocks[bl.x];
   const PlaneMap pm = a.pm;
   double* base = a.arena + B.base;

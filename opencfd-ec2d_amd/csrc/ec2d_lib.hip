@@ -532,10 +532,12 @@ int launch_residual(int level, int clip) {
     f.ev = L.pm.ev;
     f.viscous = visc ? 1 : 0;
     f.n_i = L.n_ilines;
+    f.n_lines = L.n_ilines + L.n_jlines;
     f.lines = L.d_ilines;
-    hipLaunchKernelGGL(k_fdm_lines, dim3(L.n_ilines + L.n_jlines), dim3(256), 0, R.s, f);
-    f.lines = L.d_frows;
-    if (visc) hipLaunchKernelGGL(k_fdm_visc, dim3(L.n_frows), dim3(256), 0, R.s, f);
+    f.rows = L.d_frows;
+    // line fluxes and, viscous, the cell-centre viscous fluxes (which read the Amu / Amu_t planes
+    // k_residual has just written) in one launch
+    hipLaunchKernelGGL(k_fdm_lines, dim3(f.n_lines + (visc ? L.n_frows : 0)), dim3(256), 0, R.s, f);
     hipLaunchKernelGGL(k_fdm_res, dim3(L.n_frows), dim3(256), 0, R.s, f);
     if (nv == 4) hipLaunchKernelGGL(k_tile_norms<4>, dim3(L.ntiles), dim3(256), 0, R.s, L.dt, L.db, L.arena, L.pm, L.dpartials, L.tile_rows);
     else if (nv == 5) hipLaunchKernelGGL(k_tile_norms<5>, dim3(L.ntiles), dim3(256), 0, R.s, L.dt, L.db, L.arena, L.pm, L.dpartials, L.tile_rows);
