@@ -441,7 +441,7 @@ int closure_infer() {
   return 0;
 }
 
-int launch_residual(int level, int clip) {
+int launch_residual(int level, int clip, int force = 0) {
   DLevel& L = R.lv[level];
   ResArgs a;
   a.tiles = L.dt;
@@ -455,6 +455,7 @@ int launch_residual(int level, int clip) {
   a.pm = L.pm;
   a.clip = clip;
   a.write_visc = 1;
+  a.force = force;
   // learned closure: 1 = SA case with mu_t overridden where in range,
   // 2 = the closure supplies mu_t of every cell (turbulence model 4)
   a.closure = (level == 0 && L.o.turb == 4) ? 2 : (level == 0 && R.clo.enabled && L.pm.clo >= 0) ? 1 : 0;
@@ -714,11 +715,13 @@ int flush_clip(int level) {
 }
 
 // ------------------------------------------------------------ composite steps
-int residual(int level) {
+// force: 1 = comput_force_function, 2 = Add_force_function fused into the
+// residual's store (coarse levels; see ResArgs::force)
+int residual(int level, int force = 0) {
   int clip = R.clip_pending[level] ? 1 : 0;
   R.clip_pending[level] = false;
   prof_mark(1 + 10 * level);
-  int rc = launch_residual(level, clip);
+  int rc = launch_residual(level, clip, force);
   if (!rc && R.cases[0].ctl.Time_Method == -1) {
     // dual-time source after the whole residual and before the norms:
     // Sfac = 1/(3 dt) on the first residual call, 1/(2 dt) afterwards
@@ -758,8 +761,7 @@ void tick(int level) {
 
 int advance_lusgs(int level, bool add_force) {
   DLevel& L = R.lv[level];
-  if (residual(level)) return 1;
-  if (add_force && launch_pw(level, PW_ADD_FORCE)) return 1;
+  if (residual(level, add_force ? 2 : 0)) return 1;
   prof_mark(2 + 10 * level);
   if (launch_lusgs(level, L.dblocks_all, (int)L.hb.size(), 0.0, 1)) return 1;
   prof_mark(-(2 + 10 * level));
@@ -771,8 +773,7 @@ int advance_lusgs(int level, bool add_force) {
 }
 
 int advance_euler1(int level) {
-  if (residual(level)) return 1;
-  if (level != 0 && launch_pw(level, PW_ADD_FORCE)) return 1;
+  if (residual(level, level != 0 ? 2 : 0)) return 1;
   if (launch_pw(level, PW_EULER1)) return 1;
   if (launch_ghost(level, 1, 1)) return 1;
   tick(level);
@@ -783,8 +784,7 @@ int advance_rk3(int level) {
   if (flush_clip(level)) return 1;
   if (launch_pw(level, PW_COPY_UN)) return 1;
   for (int k = 1; k <= 3; k++) {
-    if (residual(level)) return 1;
-    if (level != 0 && launch_pw(level, PW_ADD_FORCE)) return 1;
+    if (residual(level, level != 0 ? 2 : 0)) return 1;
     if (launch_pw(level, PW_RK3, k)) return 1;
     if (launch_ghost(level, 1, 1)) return 1;
   }
@@ -868,16 +868,13 @@ int multigrid() {
   if (residual(0)) return 1;
   if (launch_xfer(0, 1, 2, true) || launch_xfer(0, 1, 1, true)) return 1;
   if (launch_ghost(1, 1, 1)) return 1;
-  if (residual(1)) return 1;
-  if (launch_pw(1, PW_FORCE)) return 1;
+  if (residual(1, 1)) return 1;
   if (smooth(1)) return 1;
   if (nm >= 3) {
-    if (residual(1)) return 1;
-    if (launch_pw(1, PW_ADD_FORCE)) return 1;
+    if (residual(1, 2)) return 1;
     if (launch_xfer(1, 2, 2, true) || launch_xfer(1, 2, 1, true)) return 1;
     if (launch_ghost(2, 1, 1)) return 1;
-    if (residual(2)) return 1;
-    if (launch_pw(2, PW_FORCE)) return 1;
+    if (residual(2, 1)) return 1;
     if (smooth(2)) return 1;
     if (launch_pw(2, PW_DELT_U)) return 1;
     if (launch_xfer(1, 2, 2, false)) return 1;

@@ -33,6 +33,8 @@ struct ResArgs {
   PlaneMap pm;
   int clip;                   // apply force_vt_kw to interior nu_t on load
   int write_visc;             // store Amu / Amu_t planes
+  int force;                  // coarse multigrid levels, interior cells, variables 1..4, after the norms' values:
+                              // 1: QF = QF - Res (comput_force_function), 2: Res = Res + QF (Add_force_function)
   int closure;                // 1 (SA): take mu_t of interior cells from the closure planes where in range;
                               // 2 (4 equations): mu_t of every ring cell from the closure plane;
                               // 3 (4 equations, Baldwin-Lomax): mu_t of every ring cell from Amu_t as k_bl left it
@@ -417,7 +419,12 @@ __global__ __launch_bounds__(TX* TY) void k_residual(ResArgs a) {
       }
       if (!fdm_cell) {
 #pragma unroll
-        for (int m = 0; m < 4; m++) PL(pm.Res + m)[q] = R[m];
+        for (int m = 0; m < 4; m++) {
+          double r = R[m];
+          if (a.force == 1) PL(pm.QF + m)[q] = PL(pm.QF + m)[q] - r;
+          else if (a.force == 2) r = r + PL(pm.QF + m)[q];
+          PL(pm.Res + m)[q] = r;
+        }
       }
       if constexpr (SA) PL(pm.Res + 4)[q] = R[4];
       if constexpr (NV == 6) {
