@@ -441,7 +441,7 @@ int closure_infer() {
   return 0;
 }
 
-int launch_residual(int level, int clip, int force = 0) {
+int launch_residual(int level, int clip, int force = 0, int set_un = 0) {
   DLevel& L = R.lv[level];
   ResArgs a;
   a.tiles = L.dt;
@@ -456,6 +456,7 @@ int launch_residual(int level, int clip, int force = 0) {
   a.clip = clip;
   a.write_visc = 1;
   a.force = force;
+  a.set_un = set_un;
   // learned closure: 1 = SA case with mu_t overridden where in range,
   // 2 = the closure supplies mu_t of every cell (turbulence model 4)
   a.closure = (level == 0 && L.o.turb == 4) ? 2 : (level == 0 && R.clo.enabled && L.pm.clo >= 0) ? 1 : 0;
@@ -716,12 +717,12 @@ int flush_clip(int level) {
 
 // ------------------------------------------------------------ composite steps
 // force: 1 = comput_force_function, 2 = Add_force_function fused into the
-// residual's store (coarse levels; see ResArgs::force)
-int residual(int level, int force = 0) {
+// residual's store (coarse levels; see ResArgs::force); set_un: Set_Un too
+int residual(int level, int force = 0, int set_un = 0) {
   int clip = R.clip_pending[level] ? 1 : 0;
   R.clip_pending[level] = false;
   prof_mark(1 + 10 * level);
-  int rc = launch_residual(level, clip, force);
+  int rc = launch_residual(level, clip, force, set_un);
   if (!rc && R.cases[0].ctl.Time_Method == -1) {
     // dual-time source after the whole residual and before the norms:
     // Sfac = 1/(3 dt) on the first residual call, 1/(2 dt) afterwards
@@ -837,10 +838,12 @@ int time_advance(int level) {
   return 0;
 }
 
-int smooth(int level) {
+// un_set: Set_Un already ran (fused into the residual that formed the
+// forcing function; the state has not changed since)
+int smooth(int level, bool un_set) {
   int tm = R.cases[0].ctl.Time_Method;
   if (tm == 1) {
-    if (launch_pw(level, PW_SET_UN)) return 1;
+    if (!un_set && launch_pw(level, PW_SET_UN)) return 1;
     for (int m = 0; m < 3; m++)
       if (advance_euler1(level)) return 1;
     return 0;
@@ -848,7 +851,7 @@ int smooth(int level) {
   if (tm == 3) return advance_rk3(level);
   // LU-SGS smoother with the forcing function added to the residual: an
   // extension (the reference rejects LU-SGS with multigrid, :216-221)
-  if (launch_pw(level, PW_SET_UN)) return 1;
+  if (!un_set && launch_pw(level, PW_SET_UN)) return 1;
   return advance_lusgs(level, level != 0);
 }
 
@@ -866,23 +869,22 @@ int multigrid() {
     if (R.lv[0].o.nvar >= 5) R.clip_pending[0] = true;
   }
   if (residual(0)) return 1;
-  if (launch_xfer(0, 1, 2, true) || launch_xfer(0, 1, 1, true)) return 1;
+  if (launch_xfer(0, 1, 3, true)) return 1;  // Res -> QF and U, one launch
   if (launch_ghost(1, 1, 1)) return 1;
-  if (residual(1, 1)) return 1;
-  if (smooth(1)) return 1;
+  // Set_Un of the Euler and LU-SGS smoothers rides on the residual that
+  // forms the forcing function (RK3 copies Un itself)
+  const int un = (tm == 1 || tm == 0) ? 1 : 0;
+  if (residual(1, 1, un)) return 1;
+  if (smooth(1, un != 0)) return 1;
   if (nm >= 3) {
     if (residual(1, 2)) return 1;
-    if (launch_xfer(1, 2, 2, true) || launch_xfer(1, 2, 1, true)) return 1;
+    if (launch_xfer(1, 2, 3, true)) return 1;
     if (launch_ghost(2, 1, 1)) return 1;
-    if (residual(2, 1)) return 1;
-    if (smooth(2)) return 1;
-    if (launch_pw(2, PW_DELT_U)) return 1;
-    if (launch_xfer(1, 2, 2, false)) return 1;
-    if (launch_pw(1, PW_NEW_U)) return 1;
+    if (residual(2, 1, un)) return 1;
+    if (smooth(2, un != 0)) return 1;
+    if (launch_xfer(1, 2, 3, false)) return 1;  // comput_delt_U, prolong_U, comput_new_U
   }
-  if (launch_pw(1, PW_DELT_U)) return 1;
-  if (launch_xfer(0, 1, 2, false)) return 1;
-  if (launch_pw(0, PW_NEW_U)) return 1;
+  if (launch_xfer(0, 1, 3, false)) return 1;
   if (launch_ghost(0, 1, 1)) return 1;
   return 0;
 }
@@ -1719,12 +1721,14 @@ int ec2d_ns_multigrid(void) {
 
 int ec2d_interpolation2h(int m1, int m2, int flag) {
   if (check_ready() || m1 < 1 || m2 != m1 + 1 || m2 > R.num_mesh) return fail("bad mesh levels");
+  if (flag != 1 && flag != 2) return fail("flag must be 1 (U) or 2 (Res)");
   if (flush_clip(m1 - 1)) return 1;
   return launch_xfer(m1 - 1, m2 - 1, flag, true);
 }
 
 int ec2d_prolong_u(int m1, int m2, int flag) {
   if (check_ready() || m2 < 1 || m1 != m2 + 1 || m1 > R.num_mesh) return fail("bad mesh levels");
+  if (flag != 1 && flag != 2) return fail("flag must be 1 (U) or 2 (deltU)");
   return launch_xfer(m2 - 1, m1 - 1, flag, false);
 }
 

@@ -148,7 +148,8 @@ struct XferArgs {
   double* fine_arena;
   double* coarse_arena;
   PlaneMap fpm, cpm;
-  int flag;          // 1: U, 2: Res -> QF (restrict) / deltU (prolong)
+  int flag;          // restrict: 1 U, 2 Res -> QF, 3 both; prolong: 1 U, 2 deltU,
+                     // 3 the coarse-grid correction (see k_prolong)
 };
 
 // fine -> coarse (rows are coarse rows j = 1..ny2-1)
@@ -165,7 +166,7 @@ __global__ __launch_bounds__(256) void k_restrict(XferArgs a) {
     const int i1 = 2 * i - 1, i2 = 2 * i;
     int64_t q11 = pidx(B1, i1, j1), q12 = pidx(B1, i1, j2), q21 = pidx(B1, i2, j1), q22 = pidx(B1, i2, j2);
     int64_t qc = pidx(B2, i, j);
-    if (a.flag == 1) {
+    if (a.flag & 1) {
       const double* vol1 = f + (int64_t)a.fpm.vol * B1.plane;
       double v11 = vol1[q11], v12 = vol1[q12], v21 = vol1[q21], v22 = vol1[q22];
       double vc = c[(int64_t)a.cpm.vol * B2.plane + qc];
@@ -173,7 +174,8 @@ __global__ __launch_bounds__(256) void k_restrict(XferArgs a) {
         const double* P1 = f + (int64_t)(a.fpm.U + m) * B1.plane;
         c[(int64_t)(a.cpm.U + m) * B2.plane + qc] = (P1[q11] * v11 + P1[q12] * v12 + P1[q21] * v21 + P1[q22] * v22) / vc;
       }
-    } else {
+    }
+    if (a.flag & 2) {
       for (int m = 0; m < 4; m++) {
         const double* P1 = f + (int64_t)(a.fpm.Res + m) * B1.plane;
         c[(int64_t)(a.cpm.QF + m) * B2.plane + qc] = P1[q11] + P1[q12] + P1[q21] + P1[q22];
@@ -193,6 +195,39 @@ __global__ __launch_bounds__(256) void k_prolong(XferArgs a) {
   const int ja1 = (j % 2 == 0) ? j / 2 : j / 2 + 1, ja2 = (j % 2 == 0) ? j / 2 + 1 : j / 2;
   double* c = a.coarse_arena + B1.base;
   double* f = a.fine_arena + B2.base;
+  if (a.flag == 3) {
+    // coarse-grid correction, the sequence comput_delt_U (coarse), prolong_U
+    // (flag 2) and comput_new_U (fine) of sub_NS_multigrid.f90:214-260 in
+    // one pass: the coarse deltU = U - Un of the four parents is formed where
+    // it is read; the fine row stores the prolonged deltU (0..nx) and adds it
+    // to U on interior cells.  Every coarse cell's deltU is also stored, once:
+    // by the fine cell of even index, or of the last index for the last
+    // coarse column / row.
+    const bool jw = (j % 2 == 0) || j == B2.ny;
+    const int jc = (j % 2 == 0) ? j / 2 : j / 2 + 1;
+    for (int i = threadIdx.x; i <= B2.nx; i += blockDim.x) {
+      const int ia1 = (i % 2 == 0) ? i / 2 : i / 2 + 1, ia2 = (i % 2 == 0) ? i / 2 + 1 : i / 2;
+      int64_t q11 = pidx(B1, ia1, ja1), q21 = pidx(B1, ia2, ja1), q12 = pidx(B1, ia1, ja2), q22 = pidx(B1, ia2, ja2);
+      int64_t qf = pidx(B2, i, j);
+      const bool interior = i >= 1 && i <= B2.nx - 1 && j >= 1 && j <= B2.ny - 1;
+      const bool cw = jw && ((i % 2 == 0) || i == B2.nx);
+      const int64_t qc = pidx(B1, (i % 2 == 0) ? i / 2 : i / 2 + 1, jc);
+      for (int m = 0; m < 4; m++) {
+        const double* Uc = c + (int64_t)(a.cpm.U + m) * B1.plane;
+        const double* Unc = c + (int64_t)(a.cpm.Un + m) * B1.plane;
+        const double d11 = Uc[q11] - Unc[q11], d21 = Uc[q21] - Unc[q21];
+        const double d12 = Uc[q12] - Unc[q12], d22 = Uc[q22] - Unc[q22];
+        const double v = a1 * d11 + a2 * (d21 + d12) + a3 * d22;
+        f[(int64_t)(a.fpm.deltU + m) * B2.plane + qf] = v;
+        if (interior) {
+          double* Uf = f + (int64_t)(a.fpm.U + m) * B2.plane;
+          Uf[qf] = Uf[qf] + v;
+        }
+        if (cw) c[(int64_t)(a.cpm.deltU + m) * B1.plane + qc] = Uc[qc] - Unc[qc];
+      }
+    }
+    return;
+  }
   const int slot_c = (a.flag == 1) ? a.cpm.U : a.cpm.deltU;
   const int slot_f = (a.flag == 1) ? a.fpm.U : a.fpm.deltU;
   for (int i = threadIdx.x; i <= B2.nx; i += blockDim.x) {

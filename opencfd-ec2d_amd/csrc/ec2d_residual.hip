@@ -35,6 +35,7 @@ struct ResArgs {
   int write_visc;             // store Amu / Amu_t planes
   int force;                  // coarse multigrid levels, interior cells, variables 1..4, after the norms' values:
                               // 1: QF = QF - Res (comput_force_function), 2: Res = Res + QF (Add_force_function)
+  int set_un;                 // also Un = U, variables 1..4, cells 0..nx x 0..ny (Set_Un)
   int closure;                // 1 (SA): take mu_t of interior cells from the closure planes where in range;
                               // 2 (4 equations): mu_t of every ring cell from the closure plane;
                               // 3 (4 equations, Baldwin-Lomax): mu_t of every ring cell from Amu_t as k_bl left it
@@ -504,6 +505,20 @@ __global__ __launch_bounds__(TX* TY) void k_residual(ResArgs a) {
         if (ci < i0 || ci >= i0 + TX || cj < j0 || cj >= j0 + TY) continue;
         PL(pm.Amu_t)[pidx(B, i, j)] = sAt[lj][li];
       }
+    }
+  }
+  if (a.set_un) {
+    // Set_Un over the cells and the first ghost layer; a ghost cell belongs
+    // to the tile that holds the interior cell next to it
+    for (int idx = tid; idx < RW * RH; idx += NT) {
+      int li = idx % RW, lj = idx / RW;
+      int i = i0 - 2 + li, j = j0 - 2 + lj;
+      if (i < 0 || i > nx || j < 0 || j > ny) continue;
+      int ci = min(max(i, 1), nx - 1), cj = min(max(j, 1), ny - 1);
+      if (ci < i0 || ci >= i0 + TX || cj < j0 || cj >= j0 + TY) continue;
+      int64_t q = pidx(B, i, j);
+#pragma unroll
+      for (int m = 0; m < 4; m++) PL(pm.Un + m)[q] = U[m][q];
     }
   }
   // ---- tile reduction (fixed tree: wave butterfly, then waves in order) ----

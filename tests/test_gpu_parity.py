@@ -355,3 +355,29 @@ def test_split_sweep_uneven_multiblock(split, hmax):
                 diff = first_diff(b.get("U", 0, 1, mb), ref.block(1, mb).U)
                 assert diff is None, f"step {k} block {mb}: {diff}"
         b.res_norms()
+
+
+@pytest.mark.parametrize("tm", [F.TIME_RK3, F.TIME_EULER1, F.TIME_LU_SGS])
+def test_multigrid_scratch_arrays(tm):
+    """The V-cycle fuses comput_force_function, Add_force_function and Set_Un
+    into the coarse residual, both restrictions into one launch, and
+    comput_delt_U, prolong_U and comput_new_U into one pass; every array the
+    reference's cycle leaves behind (Un, QF, deltU on every level, U and Res)
+    still equals the oracle's bit for bit."""
+    c = cases.control_euler_naca(iters=10)
+    c.Time_Method, c.Num_Mesh, c.CFL = tm, 3, (2.0 if tm == F.TIME_LU_SGS else 0.3)
+    d = naca_case(f"g_mg_scratch_{tm}", c)
+    ref = O.OracleCase(d)
+    ref.init_flow()
+    with solver.Batch([solver.Case.from_dir(d)]) as b:
+        b.init_flow()
+        for k in range(2):
+            ref.main_step()
+            b.main_step()
+            for nm in (1, 2, 3):
+                rb = ref.block(nm, 1)
+                for a in ("U", "Un", "QF", "deltU") + (("Res",) if nm == 1 else ()):
+                    if nm == 1 and a == "QF":
+                        continue
+                    diff = first_diff(b.get(a, 0, nm, 1), getattr(rb, a))
+                    assert diff is None, f"pass {k} level {nm} {a}: {diff}"
