@@ -211,8 +211,8 @@ def main():
         calibrate_copy()
         return
     if args.gpus != 1:
-        print("bench.py measures one MI355X: this build has no multi-GPU path (blocks and ensemble cases would "
-              "shard one GPU each; see DESIGN.md).  Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one MI355X: there is no multi-GPU path (blocks and ensemble cases would "
+              "shard with one GPU per share).  Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     from ec2d import solver
