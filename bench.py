@@ -17,7 +17,11 @@ The JSON line also carries:
                 its mean launch time (HIP events on the library stream),
                 against the 8 TB/s HBM peak; "traffic" is the HBM bytes per
                 launch read from the rocprofv3 FETCH_SIZE / WRITE_SIZE
-                summary committed under profiles/ (null if absent)
+                summary committed under profiles/ (null if absent);
+                sweep_steps_per_launch / sweep_ns_per_step give the LU-SGS
+                sweeps' dependent chain (hyperplane steps of the longest fine
+                block, both sweeps) and the measured time per step, the bound
+                that holds the update far below the HBM roofline
   cpu_baseline  the reference Fortran (flang build in oracle/_ref) or, for
                 workloads the reference refuses (LU-SGS with multigrid), the C
                 restatement in oracle/, timed on one host core over a bounded
@@ -266,6 +270,11 @@ def main():
     bpc = kernel_bytes_per_cell(dom, nv, viscous, sa)
     per_launch_ms = tm[f"{dom}_ms"]
     achieved = bpc * cells / (per_launch_ms * 1e-3) / 1e9 if per_launch_ms > 0 else 0.0
+    # the LU-SGS sweeps are a dependent chain: a block of ncol x nrow cells
+    # takes ncol + nrow - 1 hyperplane steps per sweep, two sweeps per update;
+    # blocks run side by side, so the longest block sets the chain
+    steps = max(2 * ((g.nx - 1) + (g.ny - 1) - 1) for c in batch for g in c.levels[0])
+    sweep_ms = levels["level1"]["lusgs_sweep"]["ms"] if "lusgs_sweep" in levels["level1"] else 0.0
     out = {
         "metric": "cell-iters/s on 1 MI355X",
         "value": value,
@@ -287,7 +296,9 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": read_traffic(desc["workload"]) if dom == "lusgs" else None,
                      "bytes_per_cell": bpc, "cells_per_launch": cells,
-                     "ms_per_launch": per_launch_ms},
+                     "ms_per_launch": per_launch_ms,
+                     "sweep_steps_per_launch": steps,
+                     "sweep_ns_per_step": sweep_ms * 1e6 / steps if steps else None},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
