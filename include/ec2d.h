@@ -41,7 +41,9 @@ enum {
   EC2D_DT = 3,     /* dt(1:nx-1,1:ny-1) */
   EC2D_DU = 4,     /* dU(NV,1:nx,1:ny) */
   EC2D_LCI = 5, EC2D_LCJ = 6, EC2D_LVI = 7, EC2D_LVJ = 8, /* (1:nx,1:ny) */
-  EC2D_AMU = 9, EC2D_AMU_T = 10,                          /* (-1:nx+1,-1:ny+1), interior filled */
+  EC2D_AMU = 9, EC2D_AMU_T = 10,                          /* (-1:nx+1,-1:ny+1); after a residual Amu holds
+                                                             the interior, Amu_t also the layer 0:nx x 0:ny
+                                                             (Amut_boundary's ghost values, which output writes) */
   EC2D_DW = 22,    /* dw(0:nx,0:ny) wall distance (SA / SST / model 4) */
   EC2D_UN = 23,    /* Un(NVun,0:nx,0:ny) */
   EC2D_QF = 24,    /* QF(4,1:nx-1,1:ny-1) */
