@@ -62,14 +62,19 @@ def test_output_files_from_device(mg, tmp_path):
     np.testing.assert_allclose([float(v) for v in g[1:]], [float(v) for v in o[1:]], rtol=1e-9, atol=0)
 
 
-def test_output_files_from_device_sa(tmp_path):
-    """SA: flow2d.dat (with mu_t on the first ghost layer, which
-    Amut_boundary sets) and SA2d.dat from the device against the oracle's.
-    The device's SA path agrees to 1e-11 relative (exp and pow of its own
-    math library), so the F20.10 columns are compared as numbers: equal
-    within one unit of the last printed digit."""
+@pytest.mark.parametrize("turb", [F.TURB_SA, F.TURB_SST, F.TURB_BL])
+def test_output_files_from_device_sa(turb, tmp_path):
+    """SA, SST (isothermal wall) and Baldwin-Lomax: flow2d.dat (with mu_t on
+    the first ghost layer, as each model leaves it) and SA2d.dat / SST2d.dat
+    from the device against the oracle's.  The device's turbulence paths agree
+    to 1e-11 relative (exp, pow and tanh of its own math library), so the
+    F20.10 columns are compared as numbers: equal within one unit of the last
+    printed digit."""
     c = cases.control_sa_naca(iters=10, Re=1e5, Ma=0.5)
-    d = naca_case("g_out_sa", c, ds1=1e-4)
+    c.Iflag_turbulence_model = turb
+    if turb == F.TURB_SST:
+        c.Twall = 300.0
+    d = naca_case(f"g_out_t{turb}", c, ds1=1e-4)
     ref = O.OracleCase(d)
     ref.init_flow()
     gdir, odir = tmp_path / "gpu", tmp_path / "oracle"
@@ -83,7 +88,8 @@ def test_output_files_from_device_sa(tmp_path):
         names = [os.path.basename(p) for p in b.output(0, 1, str(gdir))]
         blocks = [{"U": x.U, "Amu_t": x.Amu_t, "x1": x.geo["x1"], "y1": x.geo["y1"]} for x in ref.meshes[0]["blocks"]]
         F.write_output(str(odir), 1, ref.kstep[0], blocks, c.nvar, c.Cv, c.gamma, c.Ma, c.Re)
-    assert "SA2d.dat" in names and "flow2d.dat" in names
+    assert "flow2d.dat" in names
+    assert ("SA2d.dat" in names) == (turb == F.TURB_SA) and ("SST2d.dat" in names) == (turb == F.TURB_SST)
     for n in names:
         g = (gdir / n).read_text().splitlines()
         o = (odir / n).read_text().splitlines()
@@ -94,7 +100,9 @@ def test_output_files_from_device_sa(tmp_path):
                 continue
             np.testing.assert_allclose([float(v) for v in lg.split()], [float(v) for v in lo.split()], rtol=1e-9,
                                        atol=1.5e-10, err_msg=n)
-    # the Amut column of the far-field ghost row j = ny (the last nx+1 lines) holds the free-stream mu_t
-    B = solver.Case.from_dir(d).levels[0][0]
-    amut = np.array([float(v.split()[7]) for v in (gdir / "flow2d.dat").read_text().splitlines()[-(B.nx + 1):]])
-    assert np.all(amut > 0)
+    if turb == F.TURB_SA:
+        # the Amut column of the far-field ghost row j = ny (the last nx+1 lines) holds the free-stream
+        # mu_t of SA (SST and Baldwin-Lomax leave zero there, as the oracle does)
+        B = solver.Case.from_dir(d).levels[0][0]
+        amut = np.array([float(v.split()[7]) for v in (gdir / "flow2d.dat").read_text().splitlines()[-(B.nx + 1):]])
+        assert np.all(amut > 0)
