@@ -126,3 +126,30 @@ def random_naca_case(seed: int, prefix: str):
     d = naca_case(f"{prefix}_rnaca_{seed}", c, nx=nx, ny=ny, ds1=1e-4 if c.If_viscous else 1e-3, fdm=fdm)
     exact = turb == F.TURB_NONE
     return d, 3, exact, not (tm == F.TIME_LU_SGS and nm > 1)
+
+
+def random_dual_case(seed: int, prefix: str):
+    """A seeded random dual-time LU-SGS case (Time_Method -1) on the C-grid:
+    Euler, laminar or SA, random scheme / flux / reconstruction, optionally an
+    FD-WENO5 block, 3..8 inner iterations, an inner limit that ends some inner
+    loops early or none, physical steps of 0.03..0.08 (Euler, laminar) or 0.1
+    (SA).  Returns (case directory, exact): exact without SA."""
+    rng = np.random.default_rng(2000 + seed)
+    kind = int(rng.integers(0, 3))  # 0 Euler, 1 laminar, 2 SA
+    c = cases.control_sa_naca(iters=10, Re=1e5, Ma=0.5) if kind else cases.control_euler_naca(iters=10)
+    if kind == 1:
+        c.Iflag_turbulence_model = F.TURB_NONE
+    c.Time_Method = F.TIME_DUAL_LU_SGS
+    c.Iflag_Scheme = int(rng.integers(0, 7))
+    c.Iflag_Flux = int(rng.integers(1, 7))
+    c.IFlag_Reconstruction = int(rng.integers(0, 3))
+    c.Nstep_Inner_Limit = int(rng.integers(3, 9))
+    c.Res_Inner_Limit = float(10.0 ** rng.uniform(-3.0, -1.5)) if rng.random() < 0.6 else 1e-30
+    c.Iflag_local_dt = 1
+    if kind == 2:
+        c.dt_global, c.CFL = 0.1, 0.5
+    else:
+        c.dt_global, c.CFL = float(rng.uniform(0.03, 0.08)), float(rng.uniform(1.0, 5.0))
+    fdm = (1,) if rng.random() < 0.3 else ()
+    d = naca_case(f"{prefix}_rdual_{seed}", c, ds1=1e-4 if kind else 1e-3, fdm=fdm)
+    return d, kind != 2

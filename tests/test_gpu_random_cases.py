@@ -38,3 +38,25 @@ def test_random_naca_case(seed):
         _compare_steps(d, n, arrays)
     else:
         _compare_steps(d, n, arrays, exact=False, rtol=1e-11, norm_rtol=1e-10)
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_random_dual_time_case(seed):
+    """Seeded random dual-time cases (tests/helpers.py random_dual_case),
+    early inner exits included: U, Res, dt, Un, Un1, the inner-loop index
+    and the norms after every physical step, bit-exact without SA, 1e-11
+    with it (as tests/test_gpu_dual.py)."""
+    import oracle as O
+    from ec2d import solver
+    from helpers import random_dual_case
+    from test_gpu_dual import _check
+
+    d, exact = random_dual_case(seed, "g")
+    ref = O.OracleCase(d)
+    ref.init_flow()
+    with solver.Batch([solver.Case.from_dir(d)]) as b:
+        b.init_flow()
+        for k in range(3):
+            ref.main_step()
+            b.main_step()
+            _check(b, ref, k, exact)

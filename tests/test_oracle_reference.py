@@ -340,3 +340,24 @@ def test_random_naca_cases(seed):
     c = F.read_control(os.path.join(d, "control.in"))
     arrays = ("U", "Res", "dt") + (("Amu_t",) if c.Iflag_turbulence_model != F.TURB_NONE else ())
     run_both(d, n, arrays)
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_random_dual_time_cases(seed, tmp_path):
+    """Seeded random dual-time cases (tests/helpers.py random_dual_case):
+    the oracle equals the reference, run in a fresh process, bit for bit,
+    inner-loop exits included."""
+    from helpers import random_dual_case
+
+    d, _ = random_dual_case(seed, "r")
+    z, inner = ref_dual(d, 3, tmp_path)
+    oc = O.OracleCase(d)
+    oc.init_flow()
+    for k in range(3):
+        oc.main_step()
+        b = oc.block()
+        for a in ("U", "Res", "dt"):
+            diff = first_diff(getattr(b, a), z[f"{a}_{k}"])
+            assert diff is None, f"step {k} {a}: {diff}"
+        assert np.array_equal(oc.norms(1)[1], z[f"rms_{k}"]), f"step {k} rms"
+    assert oc.inner_iterations == inner
