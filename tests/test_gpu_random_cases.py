@@ -60,3 +60,54 @@ def test_random_dual_time_case(seed):
             ref.main_step()
             b.main_step()
             _check(b, ref, k, exact)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_batch(seed):
+    """A batch of 2-4 seeded random C-grid cases that share the level options
+    (turbulence model, scheme, flux, reconstruction, mesh levels, time method)
+    and differ in grid size, Mach number, angle of attack, Reynolds number
+    and wall distance: one launch per stage for all of them must equal each
+    case run alone by the oracle (bit for bit without a turbulence model,
+    1e-11 with one)."""
+    import numpy as np
+    import oracle as O
+    from ec2d import cases, solver
+    from ec2d import formats as F
+    from helpers import first_diff, naca_case
+
+    rng = np.random.default_rng(3000 + seed)
+    turb = int(rng.choice([F.TURB_NONE, F.TURB_NONE, F.TURB_SA, F.TURB_SST, F.TURB_BL]))
+    scheme, flux, recon = int(rng.integers(0, 7)), int(rng.integers(1, 7)), int(rng.integers(0, 3))
+    nm = int(rng.integers(1, 4))
+    tm = int(rng.choice([F.TIME_LU_SGS, F.TIME_RK3, F.TIME_EULER1]))
+    ds = []
+    for k in range(int(rng.integers(2, 5))):
+        ma, aoa = float(rng.uniform(0.3, 0.7)), float(rng.uniform(-2.0, 4.0))
+        if turb == F.TURB_NONE:
+            c = cases.control_euler_naca(iters=10, Ma=ma, aoa=aoa)
+        else:
+            c = cases.control_sa_naca(iters=10, Re=float(10.0 ** rng.uniform(4.5, 5.5)), Ma=ma, aoa=aoa)
+            c.Iflag_turbulence_model = turb
+            if turb == F.TURB_SST:
+                c.Twall = 300.0
+        c.Iflag_Scheme, c.Iflag_Flux, c.IFlag_Reconstruction = scheme, flux, recon
+        c.Time_Method, c.Num_Mesh = tm, nm
+        c.CFL = {F.TIME_LU_SGS: 2.0, F.TIME_RK3: 0.3, F.TIME_EULER1: 0.2}[tm]
+        nx, ny = int(rng.choice([49, 57, 65, 73])), int(rng.choice([13, 17, 21]))
+        ds.append(naca_case(f"g_rbatch_{seed}_{k}", c, nx=nx, ny=ny, ds1=1e-4 if turb else 1e-3))
+    cs = [solver.Case.from_dir(d) for d in ds]
+    with solver.Batch(cs) as b:
+        b.init_flow()
+        for _ in range(3):
+            b.main_step()
+        Us = [b.get("U", k) for k in range(len(ds))]
+    for k, d in enumerate(ds):
+        ref = O.OracleCase(d)
+        ref.init_flow()
+        for _ in range(3):
+            ref.main_step()
+        if turb == F.TURB_NONE:
+            assert first_diff(Us[k], ref.block().U) is None, f"case {k}"
+        else:
+            np.testing.assert_allclose(Us[k], ref.block().U, rtol=1e-11, atol=1e-300, err_msg=f"case {k}")
