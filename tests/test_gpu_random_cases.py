@@ -111,3 +111,45 @@ def test_random_batch(seed):
             assert first_diff(Us[k], ref.block().U) is None, f"case {k}"
         else:
             np.testing.assert_allclose(Us[k], ref.block().U, rtol=1e-11, atol=1e-300, err_msg=f"case {k}")
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_tuned_case(seed):
+    """Seeded random C-grid cases tall enough for split sweeps (65..129
+    points across), run with the launch shapes the autotuner picks on the live
+    state (residual tile rows, lanes per cell, workgroups per block): the
+    tuning leaves the state untouched and the tuned run equals the oracle
+    (bit for bit without a turbulence model, 1e-11 with one)."""
+    import numpy as np
+    import oracle as O
+    from ec2d import cases, solver
+    from ec2d import formats as F
+    from helpers import first_diff, naca_case
+
+    rng = np.random.default_rng(4000 + seed)
+    turb = int(rng.choice([F.TURB_NONE, F.TURB_NONE, F.TURB_SA, F.TURB_BL]))
+    nm = int(rng.integers(1, 4))
+    if turb == F.TURB_NONE:
+        c = cases.control_euler_naca(iters=10)
+    else:
+        c = cases.control_sa_naca(iters=10, Re=1e5, Ma=0.5)
+        c.Iflag_turbulence_model = turb
+    c.Iflag_Scheme, c.Iflag_Flux, c.IFlag_Reconstruction = int(rng.integers(0, 7)), int(rng.integers(1, 7)), 0
+    c.Time_Method, c.Num_Mesh, c.CFL = F.TIME_LU_SGS, nm, 2.0
+    nx, ny = int(rng.choice([97, 129, 161])), int(rng.choice([65, 97, 129]))
+    d = naca_case(f"g_rtuned_{seed}", c, nx=nx, ny=ny, ds1=1e-4 if turb else 1e-3, fdm=(1,) if rng.random() < 0.3 else ())
+    ref = O.OracleCase(d)
+    ref.init_flow()
+    with solver.Batch([solver.Case.from_dir(d)]) as b:
+        b.init_flow()
+        report = b.autotune("", reps=2)
+        assert first_diff(b.get("U"), ref.block().U) is None, "state changed by tuning"
+        for k in range(3):
+            ref.main_step()
+            b.main_step()
+            if turb == F.TURB_NONE:
+                assert first_diff(b.get("U"), ref.block().U) is None, f"step {k} ({report})"
+            else:
+                np.testing.assert_allclose(b.get("U"), ref.block().U, rtol=1e-11, atol=1e-300,
+                                           err_msg=f"step {k} ({report})")
+        b.res_norms()  # reports a timed-out cross-workgroup wait, if any
