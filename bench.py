@@ -1,6 +1,6 @@
 """Benchmark: cell-iterations per second of the MI355X residual + LU-SGS path.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c4]
 
 One "step" is one pass of the reference main-loop body (for the multigrid
 workload one V-cycle) over every cell of the case; value = Num_Cell * K /
@@ -11,6 +11,11 @@ Workloads (see opencfd-ec2d_amd/ec2d/cases.py):
   c1  NACA 0012 C-grid 257x65, Euler, Roe + MUSCL3 (van Albada), LU-SGS CFL 5
   c2  NACA 0012 C-grid 513x129, SA-RANS Re 6.5e6 M 0.73 AoA 2, hybrid
       FD-WENO5 interior / Roe edge band, LU-SGS smoothed 3-level V-cycle
+  c3  64 sheared, perturbed blocks in a seeded 8x8 layout (33..257 x 33..129),
+      Euler, AUSM + MUSCL3, LU-SGS, isentropic vortex start
+  c4  (default) 256 seeded NACA mpxx SA-RANS cases, 129x49 .. 257x65, Roe +
+      MUSCL3, LU-SGS, all cases in one launch per stage: the largest
+      single-GPU configuration
 
 The JSON line also carries:
   roofline      the dominant kernel's algorithmic bytes per launch divided by
@@ -24,8 +29,9 @@ The JSON line also carries:
                 that holds the update far below the HBM roofline
   cpu_baseline  the reference Fortran (flang build in oracle/_ref) or, for
                 workloads the reference refuses (LU-SGS with multigrid), the C
-                restatement in oracle/, timed on one host core over a bounded
-                sample of the same workload
+                restatement in oracle/, timed on the host over a bounded sample
+                of the same workload: one core for a single case, one process
+                per case on up to 16 cores for an ensemble
 
     python bench.py --calibrate-copy
 
@@ -85,43 +91,57 @@ def make_workload(name, d):
     raise SystemExit(f"unknown workload {name}")
 
 
-# algorithmic bytes per cell of each kernel (FP64, 8 B), counting every datum
-# the step must move once.
-#   lusgs:    U read + written, Res read, forward dU written and read back,
-#             final dU written (6 NV); vol, dt, Lci, Lcj, si, sj, ni1, ni2,
-#             nj1, nj2 read (10); Lvi, Lvj when viscous (2)
-#   residual: U read, 7 metrics read, Res written, dt, Lci, Lcj written;
-#             viscous: Lvi, Lvj written, x1, y1 read; SA: dw read
-def kernel_bytes_per_cell(kernel, nv, viscous, sa):
-    if kernel == "lusgs":
-        return 8 * (6 * nv + 10 + (2 if viscous else 0))
-    if kernel == "residual":
-        return 8 * (nv + 7 + nv + 3 + (2 + 2 if viscous else 0) + (1 if sa else 0))
-    raise ValueError(kernel)
+# Algorithmic bytes per cell-iteration (FP64, 8 B): every datum one
+# iteration must move once, 8 * (2 NV (U read + written) + 2 NV (the forward
+# sweep's dU written, read back by the backward sweep) + 7 (vol, si, ni1, ni2,
+# sj, nj1, nj2) + 2 when viscous (x1, y1) + 1 with a turbulence model (dw)):
+# 184 B for Euler (NV 4), 240 B for SA-RANS (NV 5).  The dominant kernel's
+# roofline is priced at this whole-iteration figure per cell of its launch.
+def bytes_per_cell_iter(nv, viscous, turb):
+    return 8 * (2 * nv + 2 * nv + 7 + (2 if viscous else 0) + (1 if turb else 0))
 
 
-def cpu_baseline(case_dir, cells, budget_s=20.0, kind="reference"):
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or "unknown"."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_workers(limit=16):
+    """Worker processes for the ensemble baseline: the CPUs this process may
+    run on, at most `limit` (the host share of one GPU)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(limit, n))
+
+
+def cpu_baseline(case_dirs, cells, budget_s=20.0, kind="reference"):
     """The CPU baseline, measured in a child process (the reference's Fortran
     runtime writes to the process's stdout at exit, which must not follow this
-    script's JSON line); None if the checker is not built."""
+    script's JSON line); None if the checker is not built.  case_dirs: one
+    directory (single case, one core) or several (ensemble: one process per
+    case, run concurrently)."""
     import subprocess
 
-    out = os.path.join(case_dir, "cpu_baseline.json")
-    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-baseline", case_dir, kind, str(cells), str(budget_s), out]
-    r = subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=600)
+    out = os.path.join(case_dirs[0], "cpu_baseline.json")
+    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-baseline", kind, str(budget_s), out] + list(case_dirs)
+    r = subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=900)
     if r.returncode != 0 or not os.path.exists(out):
         return None
     with open(out) as f:
         return json.load(f)
 
 
-def cpu_baseline_child(case_dir, kind, cells, budget_s, out):
-    """CPU on one core: time n and 2n main-loop passes, take the difference.
-
-    kind "reference": the unmodified reference (flang build in oracle/_ref).
-    kind "port": the C oracle, for workloads the reference refuses to run
-    (it stops on LU-SGS combined with multigrid, opencfd_ec2d_v1.5.4.f90:216-221).
-    """
+def _cpu_stepper(case_dir, kind):
+    """(steps(n), label) for one case on this process: the reference build or the C oracle."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import contextlib
     import io
@@ -130,40 +150,123 @@ def cpu_baseline_child(case_dir, kind, cells, budget_s, out):
         import refpy
 
         if not refpy.available():
-            return
+            return None, None
         with contextlib.redirect_stdout(io.StringIO()):
             R = refpy.RefCase(case_dir)
 
         def steps(n):
             with contextlib.redirect_stdout(io.StringIO()):
                 R.op("main_steps", n)
-        what = "flang -O2 serial build of the unmodified reference"
-    else:
-        import oracle as O
+        return steps, "flang -O2 serial build of the unmodified reference"
+    import oracle as O
 
-        if not O.available():
-            return
-        oc = O.OracleCase(case_dir)
-        oc.init_flow()
+    if not O.available():
+        return None, None
+    oc = O.OracleCase(case_dir)
+    oc.init_flow()
 
-        def steps(n):
-            for _ in range(n):
-                oc.main_step()
-        what = "C restatement (oracle/, gcc -O2, serial), bit-identical to the reference on every piece"
+    def steps(n):
+        for _ in range(n):
+            oc.main_step()
+    return steps, "C restatement (oracle/, gcc -O2, serial), bit-identical to the reference on every piece"
+
+
+def _case_cells(case_dir):
+    sys.path.insert(0, os.path.join(ROOT, "opencfd-ec2d_amd"))
+    from ec2d import formats as F
+
+    xs, _ = F.read_mesh2d(os.path.join(case_dir, "Mesh2d.dat"))
+    return int(sum((x.shape[1] - 1) * (x.shape[0] - 1) for x in xs))
+
+
+def _calibrate_worker(case_dir, kind, q):
+    steps, what = _cpu_stepper(case_dir, kind)
+    if steps is None:
+        q.put((None, None))
+        return
+    steps(1)
     t0 = time.perf_counter()
     steps(1)
-    per = max(time.perf_counter() - t0, 1e-4)
-    n = max(2, int(budget_s / 3.0 / per))
+    q.put((max(time.perf_counter() - t0, 1e-4), what))
+
+
+def _ensemble_worker(case_dir, kind, n_iter, barrier, q):
+    steps, _ = _cpu_stepper(case_dir, kind)
+    if steps is None:
+        barrier.abort()
+        return
+    steps(1)
+    barrier.wait()
     t0 = time.perf_counter()
-    steps(n)
-    t1 = time.perf_counter()
-    steps(2 * n)
-    t2 = time.perf_counter()
-    dt = (t2 - t1) - (t1 - t0)
-    if dt <= 0:
-        dt = (t2 - t1) / 2.0
-    res = {"value": cells * n / dt, "unit": "cell-iters/s", "cores": 1, "kind": kind,
-           "sample": f"{n} main-loop passes of the same case (time of {2 * n} passes minus time of {n}), {what}"}
+    steps(n_iter)
+    q.put((case_dir, time.perf_counter() - t0))
+
+
+def cpu_baseline_child(kind, budget_s, out, case_dirs):
+    """kind "reference": the unmodified reference (flang build in oracle/_ref).
+    kind "port": the C oracle, for workloads the reference refuses to run
+    (it stops on LU-SGS combined with multigrid, opencfd_ec2d_v1.5.4.f90:216-221).
+
+    One case: one core; time n and 2n main-loop passes, take the difference.
+    Several cases (an ensemble): P = min(16, host CPUs) processes, one case
+    each (the first P case directories), set up and one warm-up pass each,
+    then all released together through a barrier for the same number of
+    passes; throughput = sum of cells x passes / wall time of the slowest."""
+    if len(case_dirs) == 1:
+        steps, what = _cpu_stepper(case_dirs[0], kind)
+        if steps is None:
+            return
+        cells = _case_cells(case_dirs[0])
+        t0 = time.perf_counter()
+        steps(1)
+        per = max(time.perf_counter() - t0, 1e-4)
+        n = max(2, int(budget_s / 3.0 / per))
+        t0 = time.perf_counter()
+        steps(n)
+        t1 = time.perf_counter()
+        steps(2 * n)
+        t2 = time.perf_counter()
+        dt = (t2 - t1) - (t1 - t0)
+        if dt <= 0:
+            dt = (t2 - t1) / 2.0
+        res = {"value": cells * n / dt, "unit": "cell-iters/s", "cores": 1, "kind": kind, "cpu_model": cpu_model(),
+               "sample": f"{n} main-loop passes of the same case (time of {2 * n} passes minus time of {n}), {what}"}
+    else:
+        import multiprocessing as mp
+
+        P = min(host_workers(), len(case_dirs))
+        dirs = case_dirs[:P]
+        cells = [_case_cells(d) for d in dirs]
+        ctx = mp.get_context("fork")
+        # passes per process from one pass of the largest case, timed alone in
+        # a process of its own (the reference keeps its case in module globals)
+        q = ctx.Queue()
+        cal = ctx.Process(target=_calibrate_worker, args=(dirs[cells.index(max(cells))], kind, q))
+        cal.start()
+        per, what = q.get(timeout=600)
+        cal.join()
+        if per is None:
+            return
+        # passes per case in inverse proportion to its cells, so that every
+        # process runs for about budget_s
+        ns = [max(2, int(budget_s / per * max(cells) / c)) for c in cells]
+        barrier = ctx.Barrier(P + 1)
+        procs = [ctx.Process(target=_ensemble_worker, args=(d, kind, n, barrier, q)) for d, n in zip(dirs, ns)]
+        for p in procs:
+            p.start()
+        barrier.wait(timeout=600)
+        t0 = time.perf_counter()
+        done = [q.get(timeout=900) for _ in procs]
+        wall = time.perf_counter() - t0
+        for p in procs:
+            p.join()
+        work = sum(c * n for c, n in zip(cells, ns))
+        res = {"value": work / wall, "unit": "cell-iters/s", "cores": P, "kind": kind,
+               "cpu_model": cpu_model(),
+               "sample": f"{P} concurrent processes, one case each (cases 0..{P - 1} of the batch, {sum(cells)} cells), "
+                         f"{min(ns)}..{max(ns)} main-loop passes each (fewer on larger cases) after one warm-up pass, "
+                         f"sum of cells x passes over the wall time of the slowest ({wall:.1f} s; per-process "
+                         f"{min(t for _, t in done):.1f}..{max(t for _, t in done):.1f} s), {what}"}
     with open(out, "w") as f:
         json.dump(res, f)
 
@@ -196,14 +299,14 @@ def calibrate_copy():
 
 def main():
     if len(sys.argv) > 1 and sys.argv[1] == "--cpu-baseline":
-        case_dir, kind, cells, budget, out = sys.argv[2:7]
-        cpu_baseline_child(case_dir, kind, int(cells), float(budget), out)
+        kind, budget, out = sys.argv[2:5]
+        cpu_baseline_child(kind, float(budget), out, sys.argv[5:])
         return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default=os.environ.get("EC2D_WORKLOAD", "c2"))
+    ap.add_argument("--workload", default=os.environ.get("EC2D_WORKLOAD", "c4"))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile-rows", type=int, default=4)
     ap.add_argument("--calibrate-copy", action="store_true")
@@ -256,10 +359,7 @@ def main():
                                        for k in b.KERNEL_KINDS}
             b.set_profile(False)
         kind = "port" if desc["multigrid_levels"] > 1 else "reference"
-        # one case of a batch on one core (the cases are independent)
-        cpu = None if args.no_cpu else cpu_baseline(case_dirs[0], cells0, kind=kind)
-        if cpu is not None and len(case_dirs) > 1:
-            cpu["sample"] += f"; case 0 of the {len(case_dirs)}-case batch ({cells0} cells)"
+        cpu = None if args.no_cpu else cpu_baseline(case_dirs, cells0, kind=kind)
 
     ms_per_step = wall * 1000.0 / args.steps
     value = cells * args.steps / wall
@@ -267,7 +367,7 @@ def main():
     nprof = min(20, args.steps)
     share = {k: tm[f"{k}_ms"] * tm["launches"][k] / nprof for k in ("lusgs", "residual")}
     dom = max(share, key=share.get)
-    bpc = kernel_bytes_per_cell(dom, nv, viscous, sa)
+    bpc = bytes_per_cell_iter(nv, viscous, sa)
     per_launch_ms = tm[f"{dom}_ms"]
     achieved = bpc * cells / (per_launch_ms * 1e-3) / 1e9 if per_launch_ms > 0 else 0.0
     # the LU-SGS sweeps are a dependent chain: a block of ncol x nrow cells

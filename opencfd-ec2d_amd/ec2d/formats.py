@@ -148,7 +148,7 @@ def write_control(path: str, c: Control) -> None:
 
 
 def read_control(path: str) -> Control:
-    with open(path) as f:
+    with open(path, encoding="latin-1") as f:
         raw = f.read().splitlines()
     c = Control()
     types = {f.name: f.type for f in fields(Control)}
@@ -178,7 +178,7 @@ def write_mesh2d(path: str, xs: Sequence[np.ndarray], ys: Sequence[np.ndarray]) 
 
 
 def _tokens(path: str) -> List[str]:
-    with open(path) as f:
+    with open(path, encoding="latin-1") as f:
         return f.read().split()
 
 
@@ -214,7 +214,7 @@ def write_bcin(path: str, bcs: Sequence[Sequence[Sequence[int]]], title: str = "
 
 
 def read_bcin(path: str) -> List[List[List[int]]]:
-    with open(path) as f:
+    with open(path, encoding="latin-1") as f:
         lines = f.read().splitlines()
     nb = int(lines[2].split()[0])
     pos = 3
@@ -241,7 +241,7 @@ def write_fdm_in(path: str, blocks: Sequence[int]) -> None:
 def read_fdm_in(path: str) -> List[int]:
     if not os.path.exists(path):
         return []
-    with open(path) as f:
+    with open(path, encoding="latin-1") as f:
         lines = f.read().splitlines()
     nb = int(lines[4].split()[0])
     return [int(lines[6 + k].split()[0]) for k in range(nb)]
@@ -437,7 +437,7 @@ def write_wall_cp(dirpath: str, case_dir: str = None) -> str:
     in the tool).  The cell-centred values are read at node indices, as the
     tool does."""
     case_dir = case_dir or dirpath
-    with open(os.path.join(case_dir, "control.in")) as f:
+    with open(os.path.join(case_dir, "control.in"), encoding="latin-1") as f:
         f.readline()
         ma = _fnum(f.readline().split()[0])
     p00 = 1.0 / (1.4 * ma * ma)

@@ -9,6 +9,7 @@ main-loop passes in full FP64 precision.  The reference itself writes its
 outputs with 10 significant digits only, so the harness (oracle/ref_harness.f90)
 reads its arrays directly.
 """
+import hashlib
 import os
 import sys
 import tempfile
@@ -62,7 +63,29 @@ def specs():
     c = cases.control_sa_naca(iters=10, Re=1e5, Ma=0.5)
     c.Iflag_turbulence_model = F.TURB_BL
     out.append(("baldwin_lomax_lusgs", naca(c, ds1=1e-4), 3, ("U", "Res", "dt", "Amu_t")))
+    out.append(("rae2822_sample_sst_isothermal", rae2822_sample, 2, ("U", "Res", "dt", "Amu_t")))
     return out
+
+
+REF = "/root/reference"
+DIGESTS = {"rae2822_sample_sst_isothermal"}
+
+
+def rae2822_sample(d):
+    """The reference's own sample case (RAE2822, 3 blocks with a reversed
+    wake-cut interface, SST, Van Leer flux, characteristic MUSCL3, LU-SGS at
+    CFL 10): its Mesh2d.dat as shipped, its bc2d_1.in and control.in as
+    read_parameter reads them, rewritten without the comment text, with one
+    change: an isothermal wall (Twall 288.15 K, the free-stream temperature)
+    in place of the adiabatic one, whose SST omega ghost reads an unset
+    density (sub_boundary.f90:379)."""
+    import shutil
+
+    shutil.copy(os.path.join(REF, "Mesh2d.dat"), os.path.join(d, "Mesh2d.dat"))
+    F.write_bcin(os.path.join(d, "bc2d_1.in"), F.read_bcin(os.path.join(REF, "bc2d_1.in")))
+    c = F.read_control(os.path.join(REF, "control.in"))
+    c.Twall = 288.15
+    F.write_control(os.path.join(d, "control.in"), c)
 
 
 def dual_fixture(name, steps=3):
@@ -116,13 +139,22 @@ def main():
                 rms.append(rm)
                 rmax.append(mx)
             data = {f"file:{k}": np.array(v) for k, v in files.items()}
+            if name in DIGESTS:
+                # large case: input files as bytes, arrays as SHA-256 of their
+                # FP64 bytes (bit-exact parity needs no more) plus their shapes
+                data = {f"file:{k}": np.frombuffer(v.encode(), np.uint8) for k, v in files.items()}
             data["steps"] = np.array(steps)
             data["nblock"] = np.array(nblock)
             data["res_rms"] = np.array(rms)
             data["res_max"] = np.array(rmax)
             for mb in range(1, nblock + 1):
                 for a in arrays:
-                    data[f"{a}:{mb}"] = R.get(a, 1, mb)
+                    arr = R.get(a, 1, mb)
+                    if name in DIGESTS:
+                        data[f"sha256:{a}:{mb}"] = np.array(hashlib.sha256(np.ascontiguousarray(arr).tobytes()).hexdigest())
+                        data[f"shape:{a}:{mb}"] = np.array(arr.shape)
+                    else:
+                        data[f"{a}:{mb}"] = arr
             np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **data)
             print(name, "steps", steps, "blocks", nblock, "rms", rms[-1][:4])
 
