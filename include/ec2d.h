@@ -135,6 +135,26 @@ int ec2d_comput_new_u(int nMesh);
  * HIP graph replayed n times; no host synchronisation inside. */
 int ec2d_run(int n_iter);
 
+/* Runs main-loop passes (as ec2d_run, graph replays of check_every passes)
+ * until every case of the batch has reached its own residual drop: after the
+ * first fine-level residual of each pass a device check reduces each case's
+ * density Res_rms(1) (sub_Residual.f90:104-106) and stops the case
+ * (status 1) once it is at most drop times its value at the run's first
+ * check; a case whose rms is not finite stops with status 2 (the NaN guard;
+ * the reference has none), and one checked max_iter times without reaching
+ * the drop with status 3.  A stopped case is frozen: every kernel skips its
+ * blocks, so its state stays the one whose residual met the test and the
+ * other cases run on at the same cost per pass for fewer cells.  Per case
+ * (arrays of the batch's case count, each may be NULL): iters = residual
+ * checks (the state has had iters - 1 updates in this call, and its Kstep and
+ * tt count those), status, ms = host wall time from the call's start to the
+ * check that saw it stop (resolution: check_every passes).  Returns 2 with
+ * the reference's message if a cell's T fell below 1e-5 (sub_Residual.f90:
+ * 143-146).  No reference counterpart: the reference runs a fixed
+ * t_end / dt_global passes (opencfd_ec2d_v1.5.4.f90:172) and prints the
+ * residual; this is its per-case stop for ensembles. */
+int ec2d_run_to_drop(double drop, int max_iter, int check_every, int *iters, int *status, double *ms);
+
 /* Res_max(1:6), Res_rms(1:6) of mesh nMesh of a case (after a residual);
  * synchronises the stream.  Returns the T < 1e-5 error of the reference
  * (sub_Residual.f90:143-146) as status 2 with the cell in ec2d_last_error. */

@@ -299,6 +299,7 @@ __global__ __launch_bounds__(256) void k_closure_infer(ClosureArgs a) {
       int b, ii, jj;
       unpack_cell(a.cells[c], b, ii, jj);
       const BlockDesc B = a.blocks[b];
+      if (case_frozen(B.case_id)) continue;
       const CaseParams cp = a.cps[B.case_id];
       double* base = a.arena + B.base;
       const int64_t qq = pidx(B, ii, jj);

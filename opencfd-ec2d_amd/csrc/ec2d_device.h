@@ -64,6 +64,18 @@ struct TileDesc {
   int block, i0, j0, pad;  // first interior cell (1-based) of the tile
 };
 
+// Per-case run mask of a run to a residual drop (ec2d_run_to_drop): null
+// outside such a run; case c is frozen (every kernel leaves its blocks
+// untouched) while g_case_active[c] == 0.  Kernels test it once per
+// workgroup, before any barrier: a workgroup only ever holds cells of one
+// block, hence of one case.
+__device__ const int* g_case_active = nullptr;
+
+__device__ inline bool case_frozen(int case_id) {
+  const int* p = g_case_active;
+  return p != nullptr && p[case_id] == 0;
+}
+
 __host__ __device__ inline int64_t pidx(const BlockDesc& b, int i, int j) {
   return (int64_t)(j + 1) * b.P + (i + 1);
 }

@@ -102,12 +102,14 @@ __device__ void fdm_visc_row(const FdmArgs& a, int2 bl);
 __global__ __launch_bounds__(256) void k_fdm_lines(FdmArgs a) {
   __shared__ double sfp[4][FDM_MAXN + 1], sfm[4][FDM_MAXN + 1];
   if ((int)blockIdx.x >= a.n_lines) {
-    fdm_visc_row(a, a.rows[blockIdx.x - a.n_lines]);
+    const int2 r = a.rows[blockIdx.x - a.n_lines];
+    if (!case_frozen(a.blocks[r.x].case_id)) fdm_visc_row(a, r);
     return;
   }
   const int2 bl = a.lines[blockIdx.x];
   const int dir = (int)blockIdx.x < a.n_i ? 0 : 1;
   const BlockDesc B = a.blocks[bl.x];
+  if (case_frozen(B.case_id)) return;
   const CaseParams cp = a.cps[B.case_id];
   const PlaneMap pm = a.pm;
   double* base = a.arena + B.base;
@@ -206,7 +208,7 @@ __global__ __launch_bounds__(256) void k_fdm_res(FdmArgs a) {
   auto PL = [&](int slot) { return base + (int64_t)slot * B.plane; };
   const int nxp = B.nx - 1, nyp = B.ny - 1;
   const int j = bl.y;
-  if (j < LFDM + 1 || j > nyp - LFDM) return;
+  if (j < LFDM + 1 || j > nyp - LFDM || case_frozen(B.case_id)) return;
   const double* F = PL(a.fx);
   const double* G = PL(a.fy);
   const double* ev = PL(a.ev);
@@ -240,6 +242,7 @@ __global__ __launch_bounds__(256) void k_tile_norms(const TileDesc* tiles, const
   const int lt = blockIdx.x;
   const TileDesc t = tiles[lt];
   const BlockDesc B = blocks[t.block];
+  if (case_frozen(B.case_id)) return;  // a frozen case keeps the partials of its last residual
   double* base = arena + B.base;
   __shared__ double sred[2 * NV][4];
   double mx[NV], sm[NV];

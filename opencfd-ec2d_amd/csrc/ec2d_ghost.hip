@@ -200,6 +200,7 @@ template <int NV>
 __global__ __launch_bounds__(1024) void k_ghost(GhostArgs a) {
   const int bidx = a.block_list[blockIdx.x];
   const BlockDesc B = a.blocks[bidx];
+  if (case_frozen(B.case_id)) return;
   const CaseParams cp = a.cps[B.case_id];
   double* U[NV];
 #pragma unroll

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -109,6 +111,14 @@ struct Runtime {
   int dual_kt = 0;
   hipGraphExec_t graph = nullptr;
   bool graph_ok = false;
+  bool graph_pend[3] = {false, false, false};  // clip flags the cached graph was captured in (and leaves)
+  int graph_ticks[3] = {0, 0, 0};              // tick() calls of one captured pass, per level
+  // run to a residual drop (ec2d_run_to_drop): per-case mask and state
+  int* dactive = nullptr;
+  ConvState* dconv = nullptr;
+  bool conv_on = false, conv_pending = false;
+  double conv_drop = 0.0;
+  int conv_max = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_run_ms = 0, res_ms = 0, lusgs_ms = 0, ghost_ms = 0;
   int profile = 0;
@@ -746,6 +756,15 @@ int residual(int level, int force = 0, int set_un = 0) {
     rc = hipGetLastError() == hipSuccess ? 0 : fail("dual-time source launch failed");
   }
   R.residual_calls++;
+  if (!rc && level == 0 && R.conv_pending) {
+    // run to a residual drop: the first fine residual of the pass decides
+    // which cases go on (their LU-SGS / update / ghost launches follow)
+    R.conv_pending = false;
+    DLevel& L = R.lv[0];
+    hipLaunchKernelGGL(k_converge, dim3((int)R.cases.size()), dim3(256), 0, R.s, L.dpartials, L.dcase_tile0,
+                       L.dncell, L.o.nvar, R.dactive, R.dconv, R.conv_drop, R.conv_max);
+    rc = hipGetLastError() == hipSuccess ? 0 : fail("convergence check launch failed");
+  }
   prof_mark(-(1 + 10 * level));
   return rc;
 }
@@ -890,8 +909,19 @@ int multigrid() {
 }
 
 int main_step() {
+  R.conv_pending = R.conv_on;
   if (R.num_mesh == 1) return time_advance(0);
   return multigrid();
+}
+
+// A captured run graph bakes in launch shapes and kernel arguments: any call
+// that changes them drops it (the next ec2d_run captures a new one).
+int drop_graph() {
+  if (R.graph) {
+    HIPCHK(hipGraphExecDestroy(R.graph));
+    R.graph = nullptr;
+  }
+  return 0;
 }
 
 // -------------------------------------------------- host <-> device layouts
@@ -1120,6 +1150,8 @@ int ec2d_finalize(void) {
   hipFree(R.dcps);
   hipFree(R.derr);
   hipFree(R.dsweep_err);
+  hipFree(R.dactive);
+  hipFree(R.dconv);
   for (auto e : R.pev) hipEventDestroy(e);
   hipEventDestroy(R.ev0);
   hipEventDestroy(R.ev1);
@@ -1765,6 +1797,12 @@ int ec2d_run(int n_iter) {
     R.pev_kind.clear();
     for (int k = 0; k < n_iter; k++)
       if (main_step()) return 1;
+  } else if (R.graph && std::memcmp(R.graph_pend, R.clip_pending, sizeof(R.graph_pend)) == 0) {
+    // the graph of an earlier call still holds this loop body: replay it
+    for (int k = 0; k < n_iter; k++) HIPCHK(hipGraphLaunch(R.graph, R.s));
+    for (int k = 0; k < n_iter; k++)
+      for (int l = 0; l < 3; l++)
+        for (int t = 0; t < R.graph_ticks[l]; t++) tick(l);
   } else {
     // capture one pass of the loop body; the clip flag must be in a steady
     // state (the body leaves it as it found it from the second pass on)
@@ -1795,6 +1833,8 @@ int ec2d_run(int n_iter) {
           for (int t = 0; t < pass_ticks[l]; t++) tick(l);
       if (R.graph) HIPCHK(hipGraphExecDestroy(R.graph));
       R.graph = ge;
+      std::memcpy(R.graph_pend, pend_before, sizeof(R.graph_pend));
+      std::memcpy(R.graph_ticks, pass_ticks, sizeof(R.graph_ticks));
     }
   }
   HIPCHK(hipEventRecord(R.ev1, R.s));
@@ -1835,6 +1875,78 @@ int ec2d_run(int n_iter) {
 int ec2d_set_profile(int on) {
   R.profile = on;
   return 0;
+}
+
+int ec2d_run_to_drop(double drop, int max_iter, int check_every, int* iters, int* status, double* ms) {
+  if (check_ready()) return 1;
+  if (!(drop > 0.0 && drop < 1.0) || max_iter < 1 || check_every < 1)
+    return fail("ec2d_run_to_drop: need 0 < drop < 1, max_iter >= 1 and check_every >= 1");
+  if (R.cases[0].ctl.Time_Method == -1) return fail("ec2d_run_to_drop: dual-time stepping runs a fixed physical time");
+  if (R.profile) return fail("ec2d_run_to_drop: switch profiling off first (ec2d_set_profile(0))");
+  const int nc = (int)R.cases.size();
+  if (!R.dactive) {
+    HIPCHK(hipMalloc(&R.dactive, sizeof(int) * nc));
+    HIPCHK(hipMalloc(&R.dconv, sizeof(ConvState) * nc));
+  }
+  std::vector<int> act(nc, 1);
+  std::vector<ConvState> st(nc);
+  std::memset(st.data(), 0, sizeof(ConvState) * nc);
+  HIPCHK(hipMemcpy(R.dactive, act.data(), sizeof(int) * nc, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(R.dconv, st.data(), sizeof(ConvState) * nc, hipMemcpyHostToDevice));
+  const int* mask = R.dactive;
+  HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(g_case_active), &mask, sizeof(mask)));
+  R.conv_on = true;
+  R.conv_drop = drop;
+  R.conv_max = max_iter;
+  if (drop_graph()) return 1;
+  std::vector<std::array<int, 3>> k0(nc);
+  std::vector<std::array<double, 3>> t0(nc);
+  for (int c = 0; c < nc; c++)
+    for (int l = 0; l < 3; l++) k0[c][l] = R.cases[c].kstep[l], t0[c][l] = R.cases[c].tt[l];
+  std::vector<double> t_done(nc, 0.0);
+  std::vector<char> seen(nc, 0);
+  int rc = 0, passes = 0, left = nc;
+  const auto start = std::chrono::steady_clock::now();
+  while (left > 0 && passes < max_iter + check_every) {
+    if ((rc = ec2d_run(check_every))) break;
+    passes += check_every;
+    if (hipMemcpy(st.data(), R.dconv, sizeof(ConvState) * nc, hipMemcpyDeviceToHost) != hipSuccess) {
+      rc = fail("ec2d_run_to_drop: reading the case states failed");
+      break;
+    }
+    const double el = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - start).count();
+    for (int c = 0; c < nc; c++)
+      if (!seen[c] && st[c].status != 0) seen[c] = 1, t_done[c] = el, left--;
+  }
+  // leave the mode: every case runs again in later calls
+  R.conv_on = false;
+  mask = nullptr;
+  if (hipMemcpyToSymbol(HIP_SYMBOL(g_case_active), &mask, sizeof(mask)) != hipSuccess && !rc)
+    rc = fail("ec2d_run_to_drop: clearing the case mask failed");
+  if (drop_graph() && !rc) rc = 1;
+  // Kstep and tt of each case count the passes that updated it (one fewer
+  // than its residual checks); the host counted every pass for every case
+  for (int c = 0; c < nc && passes > 0; c++) {
+    const int upd = std::max(st[c].iters - 1, 0);
+    for (int l = 0; l < R.num_mesh; l++) {
+      const int per = (R.cases[c].kstep[l] - k0[c][l]) / passes;
+      R.cases[c].kstep[l] = k0[c][l];
+      R.cases[c].tt[l] = t0[c][l];
+      for (int k = 0; k < upd * per; k++) {
+        R.cases[c].tt[l] = R.cases[c].tt[l] + R.cases[c].ctl.dt_global;
+        R.cases[c].kstep[l] += 1;
+      }
+    }
+  }
+  for (int c = 0; c < nc; c++) {
+    if (iters) iters[c] = st[c].iters;
+    if (status) status[c] = st[c].status;
+    if (ms) ms[c] = t_done[c];
+  }
+  if (rc) return rc;
+  // the reference's T < 1e-5 stop and a split sweep's timeout, as ec2d_res_norms reports them
+  double mx[6], rms[6];
+  return ec2d_res_norms(0, 1, mx, rms);
 }
 
 int ec2d_timing(double* run_ms, double* residual_ms, double* lusgs_ms, double* ghost_ms) {

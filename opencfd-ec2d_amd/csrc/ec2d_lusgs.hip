@@ -291,6 +291,7 @@ template <int NV, bool VISC>
 __global__ __launch_bounds__(256) void k_lusgs_prep(LusgsArgs a) {
   using F = RecF<NV, VISC>;
   const BlockDesc B = a.blocks[a.block_list[blockIdx.y]];
+  if (case_frozen(B.case_id)) return;
   const int64_t n = B.sk_n;
   const int ncol = B.nx - 1, nrow = B.ny - 1;
   // tiles of the skewed (step, row) space: a workgroup covers 32 steps x 8
@@ -652,6 +653,7 @@ __global__ __launch_bounds__(MAXT) void k_lusgs_sweep(LusgsArgs a) {
   __shared__ Chain<NV> ch[2];  // one per sweep
   const int G = a.split, bi = blockIdx.x / G, half = blockIdx.x % G;  // half: the part of the block
   const BlockDesc B = a.blocks[a.block_list[bi]];
+  if (case_frozen(B.case_id)) return;  // every part of the block returns: no part waits on another
   const double g = a.cps[B.case_id].gamma;
   const int ncol = B.nx - 1, nrow = B.ny - 1;
   const int S = B.sk_S, R = B.sk_R;
@@ -697,6 +699,7 @@ template <int NV, bool VISC>
 __global__ __launch_bounds__(256) void k_lusgs_scatter(LusgsArgs a) {
   using F = RecF<NV, VISC>;
   const BlockDesc B = a.blocks[a.block_list[blockIdx.y]];
+  if (case_frozen(B.case_id)) return;
   const int ncol = B.nx - 1, nrow = B.ny - 1;
   // the tiles of k_lusgs_prep in the backward set's (step, row) space: eight
   // lanes read 8 consecutive output positions, a wave writes 8 cells of 8 rows

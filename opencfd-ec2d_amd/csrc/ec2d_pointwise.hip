@@ -35,6 +35,7 @@ struct PwArgs {
 __global__ __launch_bounds__(256) void k_pointwise(PwArgs a) {
   const int2 rj = a.rows[blockIdx.x];
   const BlockDesc B = a.blocks[rj.x];
+  if (case_frozen(B.case_id)) return;
   const int j = rj.y;
   double* base = a.arena + B.base;
   auto PL = [&](int slot) { return base + (int64_t)slot * B.plane; };
@@ -158,7 +159,7 @@ __global__ __launch_bounds__(256) void k_restrict(XferArgs a) {
   const BlockDesc B2 = a.coarse_blocks[rj.x];
   const BlockDesc B1 = a.fine_blocks[rj.x];
   const int j = rj.y;
-  if (j < 1 || j > B2.ny - 1) return;
+  if (j < 1 || j > B2.ny - 1 || case_frozen(B1.case_id)) return;
   double* c = a.coarse_arena + B2.base;
   double* f = a.fine_arena + B1.base;
   const int j1 = 2 * j - 1, j2 = 2 * j;
@@ -190,7 +191,7 @@ __global__ __launch_bounds__(256) void k_prolong(XferArgs a) {
   const BlockDesc B2 = a.fine_blocks[rj.x];
   const BlockDesc B1 = a.coarse_blocks[rj.x];
   const int j = rj.y;
-  if (j < 0 || j > B2.ny) return;
+  if (j < 0 || j > B2.ny || case_frozen(B2.case_id)) return;
   const double a1 = 9.0 / 16.0, a2 = 3.0 / 16.0, a3 = 1.0 / 16.0;
   const int ja1 = (j % 2 == 0) ? j / 2 : j / 2 + 1, ja2 = (j % 2 == 0) ? j / 2 + 1 : j / 2;
   double* c = a.coarse_arena + B1.base;

@@ -57,6 +57,7 @@ __global__ __launch_bounds__(TX* TY) void k_residual(ResArgs a) {
   const int lt = a.tile_perm[blockIdx.x];
   const TileDesc t = a.tiles[lt];
   const BlockDesc B = a.blocks[t.block];
+  if (case_frozen(B.case_id)) return;
   const CaseParams cp = a.cps[B.case_id];
   const PlaneMap pm = a.pm;
   const int nx = B.nx, ny = B.ny;
@@ -574,3 +575,46 @@ __global__ void k_norms(const double* partials, const int* case_tile0, const dou
 }
 
 }  // namespace ec2d
+
+// Run to a residual drop (ec2d_run_to_drop): after the first fine-level
+// residual of every main-loop pass, one workgroup per case reduces the
+// density's per-tile sums of Res^2 as k_norms does (the same order, so the
+// value equals Res_rms(1) of ec2d_res_norms) and decides whether the case
+// goes on.  The first check of a run records the reference value r0.  A case
+// stops (active = 0, its blocks frozen by every later kernel) when its rms
+// falls to drop * r0 (status 1), is not finite (status 2: the NaN guard), or
+// has been checked max_iter times (status 3).  iters counts the checks, i.e.
+// the residual evaluations of the case's state; the state it stops in has had
+// iters - 1 updates in this run.
+struct ConvState {
+  double r0, last;
+  int iters, status;
+};
+
+__global__ __launch_bounds__(256) void k_converge(const double* partials, const int* case_tile0, const double* ncell,
+                                                  int nv, int* active, ConvState* st, double drop, int max_iter) {
+  const int c = blockIdx.x;
+  if (active[c] == 0) return;
+  const int t0 = case_tile0[c], t1 = case_tile0[c + 1];
+  __shared__ double sh[256];
+  double acc = 0.0;
+  for (int t = t0 + threadIdx.x; t < t1; t += blockDim.x) acc = acc + partials[(int64_t)t * 2 * nv + 1];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sh[threadIdx.x] = sh[threadIdx.x] + sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double rms = sqrt(sh[0] / ncell[c]);
+    ConvState s = st[c];
+    s.iters += 1;
+    if (s.iters == 1) s.r0 = rms;
+    s.last = rms;
+    if (!isfinite(rms)) s.status = 2;
+    else if (rms <= drop * s.r0) s.status = 1;
+    else if (s.iters >= max_iter) s.status = 3;
+    if (s.status != 0) active[c] = 0;
+    st[c] = s;
+  }
+}

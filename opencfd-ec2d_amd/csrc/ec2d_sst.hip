@@ -40,7 +40,7 @@ __global__ __launch_bounds__(256) void k_sst_cell(SstArgs a) {
   const BlockDesc B = a.blocks[rj.x];
   const int j = rj.y;
   const int nx = B.nx, ny = B.ny;
-  if (j < 1 || j > ny - 1) return;
+  if (j < 1 || j > ny - 1 || case_frozen(B.case_id)) return;
   const CaseParams cp = a.cps[B.case_id];
   const PlaneMap pm = a.pm;
   double* base = a.arena + B.base;
@@ -180,7 +180,7 @@ __global__ __launch_bounds__(256) void k_sst_res(SstArgs a) {
   const BlockDesc B = a.blocks[rj.x];
   const int j = rj.y;
   const int nx = B.nx, ny = B.ny;
-  if (j < 1 || j > ny - 1) return;
+  if (j < 1 || j > ny - 1 || case_frozen(B.case_id)) return;
   const CaseParams cp = a.cps[B.case_id];
   const PlaneMap pm = a.pm;
   double* base = a.arena + B.base;

@@ -63,6 +63,7 @@ SIGNATURES = {
     "ec2d_set_un": [_i],
     "ec2d_comput_new_u": [_i],
     "ec2d_run": [_i],
+    "ec2d_run_to_drop": [C.c_double, _i, _i, _ip, _ip, _dp],
     "ec2d_res_norms": [_i, _i, _dp, _dp],
     "ec2d_time": [_i, _i, _ip, _dp],
     "ec2d_dual_inner": [_ip],

@@ -94,7 +94,8 @@ def _layer_distances(ny: int, ds1: float, ratio: float, far: float) -> Tuple[np.
 
 
 def naca_cgrid(nx: int, ny: int, m: float = 0.0, p: float = 0.4, t: float = 0.12, ds1: float = 1e-3,
-               ratio: float = 1.15, far: float = 20.0, n_wake: Optional[int] = None, mg_align: int = 4):
+               ratio: float = 1.15, far: float = 20.0, n_wake: Optional[int] = None, mg_align: int = 4,
+               wake_ds_growth: float = 0.0):
     """Single-block C-grid around a NACA 4-digit section.
 
     i runs along the C (lower wake from the outflow, lower surface, leading
@@ -104,6 +105,12 @@ def naca_cgrid(nx: int, ny: int, m: float = 0.0, p: float = 0.4, t: float = 0.12
     clustering; the wake stretches geometrically from the trailing-edge
     spacing to x = 1 + far.  Subface limits satisfy (idx-1) % mg_align == 0 so
     the grid admits log2(mg_align)+1 multigrid levels.
+
+    wake_ds_growth > 0 lets the first wall-normal spacing of the wake columns
+    grow linearly with the distance x - 1 behind the trailing edge, ds1 +
+    wake_ds_growth * (x - 1) (at most half the uniform spacing far / (ny - 1)),
+    each column's layers redistributed to reach the far field (0: every column
+    uses the surface distribution).
 
     Returns (x, y, bc, info) with x, y of shape (ny, nx).
     """
@@ -178,13 +185,15 @@ def naca_cgrid(nx: int, ny: int, m: float = 0.0, p: float = 0.4, t: float = 0.12
     ox[k0], oy[k0] = ix[k0], -far
     ox[k1], oy[k1] = ix[k1], far
     d, dinfo = _layer_distances(ny, ds1, ratio, far)
-    f = d / far
-    bl = np.minimum(1.0, d / 0.25) ** 0.5   # 0 at the wall (surface-normal lines), 1 beyond 0.25 chord
-    x = np.empty((ny, nx))
-    y = np.empty((ny, nx))
-    for j in range(ny):
-        x[j] = ix + (1.0 - bl[j]) * d[j] * nxv + bl[j] * (ox - ix) * f[j]
-        y[j] = iy + (1.0 - bl[j]) * d[j] * nyv + bl[j] * (oy - iy) * f[j]
+    D = np.repeat(d[:, None], nx, axis=1)   # layer distances per column
+    if wake_ds_growth > 0.0:
+        for k in list(range(n_wake)) + list(range(nx - n_wake, nx)):
+            ds_k = min(ds1 + wake_ds_growth * (ix[k] - 1.0), 0.5 * far / (ny - 1))
+            D[:, k] = _layer_distances(ny, ds_k, ratio, far)[0]
+    F_ = D / far
+    BL = np.minimum(1.0, D / 0.25) ** 0.5   # 0 at the wall (surface-normal lines), 1 beyond 0.25 chord
+    x = ix + (1.0 - BL) * D * nxv + BL * (ox - ix) * F_
+    y = iy + (1.0 - BL) * D * nyv + BL * (oy - iy) * F_
     # C-grid wake cut is a 1:1 self-interface on face 2 (reversed index order)
     iw1 = n_wake + 1                    # last node of the lower wake (TE)
     iw2 = nx - n_wake                   # first node of the upper wake (TE)
@@ -371,10 +380,17 @@ def write_vortex_case(dirpath: str, control: F.Control, seed: int = 0, nbx: int 
     return info
 
 
-def write_ensemble_case(dirpath: str, prm: dict, iters: int = 1000, cfl: float = 5.0, ds1: float = 5e-6) -> str:
+def write_ensemble_case(dirpath: str, prm: dict, iters: int = 1000, cfl: float = 5.0, ds1: float = 5e-6,
+                        wake_ds_growth: float = 1e-3) -> str:
     """One SA-RANS airfoil case of the ensemble (a dict of ensemble_params):
-    NACA mpxx C-grid of nx x ny points, Roe + MUSCL3, LU-SGS on one level."""
-    x, y, bc, _ = naca_cgrid(prm["nx"], prm["ny"], m=prm["m"], p=prm["p"], t=prm["t"], ds1=ds1)
+    NACA mpxx C-grid of nx x ny points, Roe + MUSCL3, LU-SGS on one level.
+    The wake's first spacing grows by wake_ds_growth per chord behind the
+    trailing edge: with the surface's 5e-6 chord in the whole wake, the
+    cells beside the wake cut reach aspect ratios near 1e5, and the residual
+    of those cells, which LU-SGS couples across the cut only through the
+    ghost cells, then decays about ten times more slowly."""
+    x, y, bc, _ = naca_cgrid(prm["nx"], prm["ny"], m=prm["m"], p=prm["p"], t=prm["t"], ds1=ds1,
+                             wake_ds_growth=wake_ds_growth)
     ctl = control_sa_naca(iters=iters, Ma=prm["Ma"], aoa=prm["aoa"], Re=prm["Re"], cfl=cfl)
     return write_case(dirpath, ctl, [x], [y], [bc])
 

@@ -231,6 +231,21 @@ class Batch:
     def run(self, n_iter: int):
         capi.call("ec2d_run", int(n_iter))
 
+    CONVERGED, NOT_FINITE, LIMIT = 1, 2, 3
+
+    def run_to_drop(self, drop: float = 1e-8, max_iter: int = 100000, check_every: int = 100):
+        """Runs every case of the batch until its density Res_rms has dropped
+        by `drop` from its value at the first pass (see ec2d_run_to_drop).
+        Returns per case (iters, status, ms): residual checks (the state has
+        had iters - 1 updates), status CONVERGED / NOT_FINITE / LIMIT, host
+        wall time in ms from the call's start to the check that saw it stop."""
+        n = len(self.cases)
+        it = (C.c_int * n)()
+        st = (C.c_int * n)()
+        ms = (C.c_double * n)()
+        capi.call("ec2d_run_to_drop", float(drop), int(max_iter), int(check_every), it, st, ms)
+        return list(it), list(st), list(ms)
+
     def res_norms(self, case: int = 0, nmesh: int = 1):
         rmax = np.zeros(6)
         rrms = np.zeros(6)

@@ -361,3 +361,26 @@ def test_random_dual_time_cases(seed, tmp_path):
             assert diff is None, f"step {k} {a}: {diff}"
         assert np.array_equal(oc.norms(1)[1], z[f"rms_{k}"]), f"step {k} rms"
     assert oc.inner_iterations == inner
+
+
+def test_cold_cell_stop_matches_reference():
+    """The reference's T < 1e-5 stop (sub_Residual.f90:143-146) ends its
+    process, so it runs in a child; two cold cells in block 2, and the first
+    in its loop order (j outer, i inner) is the one both report."""
+    d = vortex_case("r_cold", cases.control_vortex(iters=10))
+    script = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ref_cold_cell.py")
+    r = subprocess.run([sys.executable, script, d, "2", "7", "4", "3", "5"], capture_output=True, text=True,
+                       timeout=120)
+    line = [ln for ln in r.stdout.splitlines() if "T <1.d-5" in ln]
+    assert line, r.stdout[-2000:]
+    i, j, mb = (int(t) for t in line[0].split("!")[1].split()[:3])
+    assert (i, j, mb) == (7, 4, 2)
+    oc = O.OracleCase(d)
+    oc.init_flow()
+    U = oc.block(1, 2).U
+    for ci, cj in ((7, 4), (3, 5)):
+        q = U[cj + 1, ci + 1]
+        q[3] = 0.5 * (q[1] ** 2 + q[2] ** 2) / q[0]
+    with pytest.raises(O.SolverError) as e:
+        oc.residual(1)
+    assert str(e.value).startswith(f"T <1.d-5 ! {i} {j} {mb}"), str(e.value)
