@@ -297,6 +297,50 @@ def calibrate_copy():
     print(json.dumps({"copy_bytes_read": 8 * n, "copy_bytes_written": 8 * n, "launches": 5}))
 
 
+def converge(args):
+    """Time to a residual drop: every case of the workload runs until its
+    density Res_rms has fallen to `drop` times its first value (per-case
+    masks on the device, ec2d_run_to_drop), from freestream, with the launch
+    shapes the autotuner picks.  Prints one JSON line: the pass and wall-time
+    distribution over the cases, and how many stopped for which reason."""
+    import numpy as np
+
+    from ec2d import solver
+
+    with tempfile.TemporaryDirectory() as d:
+        desc = make_workload(args.workload, d)
+        case_dirs = desc.pop("case_dirs", [d])
+        desc.pop("data", None)
+        batch = [solver.Case.from_dir(cd) for cd in case_dirs]
+        with solver.Batch(batch, tile_rows=args.tile_rows) as b:
+            cells = b.num_cells(-1, 1)
+            b.init_flow()
+            tuning = b.autotune(args.tune_cache, reps=5) if not args.no_tune else "off"
+            t0 = time.perf_counter()
+            iters, status, ms = b.run_to_drop(args.converge, max_iter=args.max_iter, check_every=200)
+            wall = time.perf_counter() - t0
+            coeffs = [b.force_coefficients(k)[:2] for k in range(len(batch))]
+    it = np.array(iters)
+    ms = np.array(ms)
+    st = np.array(status)
+    ok = st == solver.Batch.CONVERGED
+    out = {"metric": f"wall time to a {args.converge:g} density-residual drop, every case",
+           "workload": desc["workload"], "cases": len(batch), "cells": cells,
+           "converged": int(ok.sum()), "not_finite": int((st == solver.Batch.NOT_FINITE).sum()),
+           "pass_limit": int((st == solver.Batch.LIMIT).sum()), "max_iter": args.max_iter,
+           "wall_s": wall,
+           "passes_converged": ({"min": int(it[ok].min()), "median": float(np.median(it[ok])), "max": int(it[ok].max())}
+                                if ok.any() else None),
+           "seconds_converged": ({"min": float(ms[ok].min() / 1e3), "median": float(np.median(ms[ok]) / 1e3),
+                                  "max": float(ms[ok].max() / 1e3)} if ok.any() else None),
+           "autotune": [t for t in tuning.strip().split("\n") if t]}
+    print(json.dumps(out))
+    if args.converge_out:
+        with open(args.converge_out, "w") as f:
+            json.dump({"summary": out, "iters": iters, "status": status, "ms": ms.tolist(),
+                       "cl_cd": [list(c) for c in coeffs]}, f)
+
+
 def main():
     if len(sys.argv) > 1 and sys.argv[1] == "--cpu-baseline":
         kind, budget, out = sys.argv[2:5]
@@ -311,6 +355,11 @@ def main():
     ap.add_argument("--tile-rows", type=int, default=4)
     ap.add_argument("--calibrate-copy", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--converge", type=float, default=0.0, metavar="DROP",
+                    help="instead of the throughput line: run every case of the workload to a density-residual drop "
+                         "of DROP (e.g. 1e-8) and report the passes and wall time each case needed")
+    ap.add_argument("--max-iter", type=int, default=60000, help="pass limit of --converge")
+    ap.add_argument("--converge-out", default="", help="--converge: per-case results as JSON to this file")
     ap.add_argument("--tune-cache", default="", help="autotune cache file: choices found there are reused without "
                     "timing launches, new ones are appended (profiled runs see the same shapes as the timed run)")
     args = ap.parse_args()
@@ -323,6 +372,10 @@ def main():
         sys.exit(2)
 
     from ec2d import solver
+
+    if args.converge > 0.0:
+        converge(args)
+        return
 
     with tempfile.TemporaryDirectory() as d:
         desc = make_workload(args.workload, d)

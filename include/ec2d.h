@@ -152,7 +152,9 @@ int ec2d_run(int n_iter);
  * the reference's message if a cell's T fell below 1e-5 (sub_Residual.f90:
  * 143-146).  No reference counterpart: the reference runs a fixed
  * t_end / dt_global passes (opencfd_ec2d_v1.5.4.f90:172) and prints the
- * residual; this is its per-case stop for ensembles. */
+ * residual; this is its per-case stop for ensembles.  With the environment
+ * variable EC2D_PROGRESS=1 it prints a progress line to stderr about every
+ * 10 s. */
 int ec2d_run_to_drop(double drop, int max_iter, int check_every, int *iters, int *status, double *ms);
 
 /* Res_max(1:6), Res_rms(1:6) of mesh nMesh of a case (after a residual);
@@ -266,15 +268,16 @@ int ec2d_set_profile(int on);
 int ec2d_lusgs_shape(int nMesh, int lanes_per_cell, int workgroups_per_block);
 
 /* Autotune the launch shapes of every mesh level on the committed batch:
- * residual tile rows (1, 2, 4, 8 x 64 cells), LU-SGS lanes per cell (1 or 2)
- * and workgroups per block (1, 2 or 4, see ec2d_lusgs_shape), each timed with
- * HIP events over `reps` launches on the current state;
- * the fastest is kept.  Results are cached per level configuration (equations,
- * scheme, flux, reconstruction, turbulence model, viscosity, block shapes) in
- * the text file cache_path (NULL or "": no cache).  report receives one line
- * per level.  Changes no result bit: the state is recomputed by the next step
- * (the timed launches write only Res, dt, the spectral radii, Amu, Amu_t and
- * dU).  No reference counterpart. */
+ * residual tile rows (1, 2, 4, 8 x 64 cells), LU-SGS lanes per cell (1 or 2),
+ * workgroups per block (1, 2 or 4, see ec2d_lusgs_shape) and sweep steps per
+ * prefetched register set (2, 3 or 4 with two lanes per cell and at most four
+ * waves), each timed with HIP events over `reps` launches on the current
+ * state; the fastest is kept.  Results are cached per level configuration
+ * (equations, scheme, flux, reconstruction, turbulence model, viscosity,
+ * block shapes) in the text file cache_path (NULL or "": no cache).  report
+ * receives one line per level.  Changes no result bit: the level's planes are
+ * copied before the timed launches and put back after them (a learned
+ * closure's under-relaxed mu_t included).  No reference counterpart. */
 int ec2d_autotune(const char *cache_path, int reps, char *report, int report_len);
 
 /* Tile shape of the fused residual kernel (rows x 64 cells); no reference

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <array>
+#include <cstdlib>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -76,6 +77,7 @@ struct DLevel {
   int lusgs_threads = 64;
   int lusgs_lpc = 1;          // LU-SGS lanes per cell: 2 when every block has at most 32 rows
   int lusgs_split = 1;        // LU-SGS sweep workgroups per block (2 or 4: a block's rows split between CUs)
+  int lusgs_u = 2;            // LU-SGS sweep steps per register set (record prefetch depth), up to 4 waves
   unsigned long long* dgring = nullptr;  // split sweeps: one ring per boundary between workgroups and direction
   int64_t gstride = 0;                   // slots of one ring
   int4* dwall = nullptr;      // wall faces (block, face, i, j), cases in order
@@ -268,10 +270,10 @@ int lusgs_layout(DLevel& L, int lpc, int split, bool upload) {
   L.gstride = 0;
   if (split > 1) {
     // one ring per boundary between parts of a block and sweep direction, a
-    // slot per step (npad <= S + 3) and double, all sentinel
+    // slot per step (npad <= S + 7) and double, all sentinel
     int max_S = 1;
     for (auto& d : L.hb) max_S = std::max(max_S, d.sk_S);
-    L.gstride = ((int64_t)(max_S + 4) * 2 * L.o.nvar + 31) / 32 * 32;
+    L.gstride = ((int64_t)(max_S + 8) * 2 * L.o.nvar + 31) / 32 * 32;
     const int64_t n = (int64_t)L.hb.size() * (split - 1) * 2 * L.gstride;
     HIPCHK(hipMalloc(&L.dgring, sizeof(unsigned long long) * n));
     hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, R.s, L.dgring, n, GSENT);
@@ -613,7 +615,10 @@ int launch_lusgs(int level, const int* dlist, int nblk, double alfa1, int update
   for (int d = (a.split > 1 ? 1 : 0); d <= (a.split > 1 ? 2 : 0); d++) {                  \
   a.dir = d;                                                                                \
   if (pair) {                                                                                \
-    if (nth <= 64) hipLaunchKernelGGL((k_lusgs_sweep<NV, V, 2, 64, 2>), gsweep, blk, 0, R.s, a); \
+    if (nth <= 64 && L.lusgs_u == 4) hipLaunchKernelGGL((k_lusgs_sweep<NV, V, 4, 64, 2>), gsweep, blk, 0, R.s, a); \
+    else if (nth <= 64) hipLaunchKernelGGL((k_lusgs_sweep<NV, V, 2, 64, 2>), gsweep, blk, 0, R.s, a); \
+    else if (nth <= 256 && L.lusgs_u == 4) hipLaunchKernelGGL((k_lusgs_sweep<NV, V, 4, 256, 2>), gsweep, blk, 0, R.s, a); \
+    else if (nth <= 256 && L.lusgs_u == 3) hipLaunchKernelGGL((k_lusgs_sweep<NV, V, 3, 256, 2>), gsweep, blk, 0, R.s, a); \
     else if (nth <= 256) hipLaunchKernelGGL((k_lusgs_sweep<NV, V, 2, 256, 2>), gsweep, blk, 0, R.s, a); \
     else if (nth <= 512) hipLaunchKernelGGL((k_lusgs_sweep<NV, V, 1, 512, 2>), gsweep, blk, 0, R.s, a); \
     else hipLaunchKernelGGL((k_lusgs_sweep<NV, V, 1, 1024, 2>), gsweep, blk, 0, R.s, a);       \
@@ -1907,6 +1912,9 @@ int ec2d_run_to_drop(double drop, int max_iter, int check_every, int* iters, int
   std::vector<char> seen(nc, 0);
   int rc = 0, passes = 0, left = nc;
   const auto start = std::chrono::steady_clock::now();
+  const char* pe = std::getenv("EC2D_PROGRESS");
+  const bool progress = pe && pe[0] == '1';
+  double last_print = 0.0;
   while (left > 0 && passes < max_iter + check_every) {
     if ((rc = ec2d_run(check_every))) break;
     passes += check_every;
@@ -1917,6 +1925,11 @@ int ec2d_run_to_drop(double drop, int max_iter, int check_every, int* iters, int
     const double el = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - start).count();
     for (int c = 0; c < nc; c++)
       if (!seen[c] && st[c].status != 0) seen[c] = 1, t_done[c] = el, left--;
+    if (progress && (el - last_print > 10000.0 || left == 0)) {
+      std::fprintf(stderr, "ec2d_run_to_drop: %d passes, %d of %d cases stopped, %.1f s\n", passes, nc - left, nc,
+                   el / 1e3);
+      last_print = el;
+    }
   }
   // leave the mode: every case runs again in later calls
   R.conv_on = false;
@@ -1962,6 +1975,7 @@ int ec2d_lusgs_shape(int nMesh, int lanes_per_cell, int workgroups_per_block) {
   if (lanes_per_cell != 1 && lanes_per_cell != 2) return fail("lanes per cell must be 1 or 2");
   HIPCHK(hipStreamSynchronize(R.s));
   if (lusgs_layout(R.lv[nMesh - 1], lanes_per_cell, workgroups_per_block, true)) return 1;
+  R.lv[nMesh - 1].lusgs_u = 2;
   if (R.graph) {  // a captured graph holds the old launch shape
     HIPCHK(hipGraphExecDestroy(R.graph));
     R.graph = nullptr;
@@ -1974,14 +1988,14 @@ int ec2d_autotune(const char* cache_path, int reps, char* report, int report_len
   if (R.profile) return fail("autotune needs the profile mode off");
   if (reps < 1) reps = 5;
   HIPCHK(hipStreamSynchronize(R.s));  // the layouts below are rewritten with null-stream copies
-  // cache: one line per level configuration, "key tile_rows lpc split"
-  struct Choice { int tr, lp, sp; };
+  // cache: one line per level configuration, "key tile_rows lpc split steps"
+  struct Choice { int tr, lp, sp, u; };
   std::vector<std::pair<std::string, Choice>> cache;
   if (cache_path && cache_path[0]) {
     if (FILE* f = std::fopen(cache_path, "r")) {
       char key[512];
-      int tr, lp, sp;
-      while (std::fscanf(f, "%511s %d %d %d", key, &tr, &lp, &sp) == 4) cache.push_back({key, {tr, lp, sp}});
+      int tr, lp, sp, u;
+      while (std::fscanf(f, "%511s %d %d %d %d", key, &tr, &lp, &sp, &u) == 5) cache.push_back({key, {tr, lp, sp, u}});
       std::fclose(f);
     }
   }
@@ -1994,11 +2008,17 @@ int ec2d_autotune(const char* cache_path, int reps, char* report, int report_len
                       std::to_string(L.o.flux) + "_r" + std::to_string(L.o.recon) + "_t" + std::to_string(L.o.turb) +
                       "_v" + std::to_string(R.cases[0].cp.If_viscous) + (L.fdm ? "_fdm" : "");
     for (auto& b : L.hb) key += "_" + std::to_string(b.nx) + "x" + std::to_string(b.ny);
-    int best_tr = -1, best_lp = -1, best_sp = 1;
+    int best_tr = -1, best_lp = -1, best_sp = 1, best_u = 2;
     for (auto& e : cache)
-      if (e.first == key) { best_tr = e.second.tr; best_lp = e.second.lp; best_sp = e.second.sp; }
+      if (e.first == key) { best_tr = e.second.tr; best_lp = e.second.lp; best_sp = e.second.sp; best_u = e.second.u; }
     char line[256];
     if (best_tr < 0) {
+      // the timed launches write the level's state (with a learned closure,
+      // its under-relaxed mu_t carries over to the next residual): keep a copy
+      // of the level's planes and put it back after timing
+      double* keep = nullptr;
+      HIPCHK(hipMalloc(&keep, sizeof(double) * (size_t)L.arena_n));
+      HIPCHK(hipMemcpyAsync(keep, L.arena, sizeof(double) * (size_t)L.arena_n, hipMemcpyDeviceToDevice, R.s));
       // residual tile rows
       double best = 1e30;
       for (int tr : {1, 2, 4, 8}) {
@@ -2017,27 +2037,41 @@ int ec2d_autotune(const char* cache_path, int reps, char* report, int report_len
             R.err.clear();
             continue;  // layout not possible for these blocks
           }
-          double ms;
-          if (time_launches(reps, [&] { return launch_lusgs(l, L.dblocks_all, (int)L.hb.size(), 0.0, 0); }, &ms))
-            return 1;
-          if (ms < best) { best = ms; best_lp = lp; best_sp = sp; }
+          // record prefetch depth: steps per register set (deeper sets for
+          // up to four waves with two lanes per cell)
+          for (int u : {2, 3, 4}) {
+            if (u != 2 && !(lp == 2 && L.lusgs_threads <= 256 && (u == 4 || L.lusgs_threads > 64))) continue;
+            L.lusgs_u = u;
+            double ms;
+            if (time_launches(reps, [&] { return launch_lusgs(l, L.dblocks_all, (int)L.hb.size(), 0.0, 0); }, &ms))
+              return 1;
+            if (ms < best) { best = ms; best_lp = lp; best_sp = sp; best_u = u; }
+          }
+          L.lusgs_u = 2;
         }
+      HIPCHK(hipMemcpyAsync(L.arena, keep, sizeof(double) * (size_t)L.arena_n, hipMemcpyDeviceToDevice, R.s));
+      HIPCHK(hipStreamSynchronize(R.s));
+      HIPCHK(hipFree(keep));
       if (best_lp < 0) return fail("no LU-SGS layout fits this level");
-      added.push_back({key, {best_tr, best_lp, best_sp}});
-      std::snprintf(line, sizeof(line), "level %d: measured tile_rows %d, lanes per cell %d, workgroups per block %d\n",
-                    l + 1, best_tr, best_lp, best_sp);
+      added.push_back({key, {best_tr, best_lp, best_sp, best_u}});
+      std::snprintf(line, sizeof(line),
+                    "level %d: measured tile_rows %d, lanes per cell %d, workgroups per block %d, steps per set %d\n",
+                    l + 1, best_tr, best_lp, best_sp, best_u);
     } else {
-      std::snprintf(line, sizeof(line), "level %d: cached tile_rows %d, lanes per cell %d, workgroups per block %d\n",
-                    l + 1, best_tr, best_lp, best_sp);
+      std::snprintf(line, sizeof(line),
+                    "level %d: cached tile_rows %d, lanes per cell %d, workgroups per block %d, steps per set %d\n",
+                    l + 1, best_tr, best_lp, best_sp, best_u);
     }
     rep_txt += line;
     L.tile_rows = best_tr;
     if (build_tiles(L)) return 1;
     if (lusgs_layout(L, best_lp, best_sp, true)) return 1;
+    L.lusgs_u = best_u;
   }
   if (!added.empty() && cache_path && cache_path[0]) {
     if (FILE* f = std::fopen(cache_path, "a")) {
-      for (auto& e : added) std::fprintf(f, "%s %d %d %d\n", e.first.c_str(), e.second.tr, e.second.lp, e.second.sp);
+      for (auto& e : added)
+        std::fprintf(f, "%s %d %d %d %d\n", e.first.c_str(), e.second.tr, e.second.lp, e.second.sp, e.second.u);
       std::fclose(f);
     }
   }
