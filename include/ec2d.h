@@ -44,6 +44,9 @@ enum {
   EC2D_AMU = 9, EC2D_AMU_T = 10,                          /* (-1:nx+1,-1:ny+1); after a residual Amu holds
                                                              the interior, Amu_t also the layer 0:nx x 0:ny
                                                              (Amut_boundary's ghost values, which output writes) */
+  EC2D_VOL = 11, EC2D_SI = 12, EC2D_SJ = 13,                 /* (1:nx,1:ny), read only: cell volume, face areas */
+  EC2D_NI1 = 14, EC2D_NI2 = 15, EC2D_NJ1 = 16, EC2D_NJ2 = 17, /* and face normals (sub_init.f90:180-204) */
+  EC2D_X1 = 20, EC2D_Y1 = 21,                                /* (-1:nx+1,-1:ny+1), read only: cell centres */
   EC2D_DW = 22,    /* dw(0:nx,0:ny) wall distance (SA / SST / model 4) */
   EC2D_UN = 23,    /* Un(NVun,0:nx,0:ny) */
   EC2D_QF = 24,    /* QF(4,1:nx-1,1:ny-1) */
@@ -74,6 +77,11 @@ int ec2d_case_create(const ec2d_control *ctl);
  *   x, y    (-1:nx+2,-1:ny+2) nodes incl. ghosts
  *   x1, y1  (-1:nx+1,-1:ny+1) cell centres
  *   vol, si, sj, ni1, ni2, nj1, nj2  (1:nx,1:ny)
+ *           x1 .. nj2 all NULL: ec2d_commit makes them on the device from the
+ *           interior nodes x, y (1:nx,1:ny; the ghost entries of x, y are not
+ *           read): the ghost nodes of Update_coordinate_buffer
+ *           (sub_boundary.f90:170-281), the cell centres and Creat_Mesh's
+ *           metrics (sub_init.f90:180-204); not for fvm_fdm = 1
  *   dw      (0:nx,0:ny) wall distance, NULL unless SA / SST / model 4; NULL for one of those: ec2d_commit
  *           computes it on the device (comput_dw, sub_turbulence_SST.f90:286-337)
  *   fd      5 arrays Akx,Aky,Aix,Aiy,Ajac of (1:nx-1,1:ny-1) back to back, NULL unless fvm_fdm = 1 */
@@ -224,6 +232,12 @@ int ec2d_closure_collect(int *nsamples);  /* samples from the current SA state *
  * the last epoch.  batch: a multiple of 64. */
 int ec2d_closure_train(int epochs, int batch, double lr, unsigned seed, double *first_epoch_loss,
                        double *last_epoch_loss);
+/* The gradient of the training loss with respect to the CL_NPARAM parameters
+ * (layout of ec2d_closure_get_params) over the first n valid collected
+ * samples as one mini-batch (n a multiple of 64), exactly as a training step
+ * of ec2d_closure_train computes it before its Adam update, and that batch's
+ * loss; the parameters do not change. */
+int ec2d_closure_gradient(int n, float *grad, double *loss);
 int ec2d_closure_get_params(float *theta, float *norm);
 int ec2d_closure_set_params(const float *theta, const float *norm);
 int ec2d_closure_infer(void);             /* fills EC2D_CLOSURE_MUT / _MASK from the current state */
@@ -263,8 +277,10 @@ int ec2d_set_profile(int on);
 /* No reference counterpart: the LU-SGS launch shape of one level, chosen by
  * hand (ec2d_autotune chooses it by timing otherwise): lanes per cell 1 or 2,
  * workgroups per block 1, 2 or 4 (each block's rows split between that many
- * CUs, for levels of at most 256/split blocks taller than split/2 waves).
- * Results do not depend on it. */
+ * CUs, for levels of at most 256/split blocks taller than split/2 waves), or
+ * 0 with one lane per cell: the fused launch, whose workgroups make each
+ * block's records themselves (blocks of at most 64 rows of cells), so that
+ * they never go through device memory.  Results do not depend on it. */
 int ec2d_lusgs_shape(int nMesh, int lanes_per_cell, int workgroups_per_block);
 
 /* Autotune the launch shapes of every mesh level on the committed batch:

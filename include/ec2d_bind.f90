@@ -31,6 +31,8 @@ module ec2d_bind
   integer(c_int), parameter :: EC2D_LCI = 5, EC2D_LCJ = 6, EC2D_LVI = 7, EC2D_LVJ = 8
   integer(c_int), parameter :: EC2D_AMU = 9, EC2D_AMU_T = 10, EC2D_DW = 22, EC2D_UN = 23
   integer(c_int), parameter :: EC2D_QF = 24, EC2D_DELTU = 25, EC2D_UN1 = 31
+  integer(c_int), parameter :: EC2D_VOL = 11, EC2D_SI = 12, EC2D_SJ = 13, EC2D_NI1 = 14, EC2D_NI2 = 15
+  integer(c_int), parameter :: EC2D_NJ1 = 16, EC2D_NJ2 = 17, EC2D_X1 = 20, EC2D_Y1 = 21
   integer(c_int), parameter :: EC2D_CLOSURE_MUT = 40, EC2D_CLOSURE_MASK = 41
 
   interface
@@ -237,6 +239,12 @@ module ec2d_bind
       real(c_double), value :: lr
       integer(c_int), value :: seed
       real(c_double) :: first_epoch_loss, last_epoch_loss
+    end function
+    integer(c_int) function ec2d_closure_gradient(n, grad, loss) bind(C, name="ec2d_closure_gradient")
+      import
+      integer(c_int), value :: n
+      real(c_float) :: grad(*)
+      real(c_double) :: loss
     end function
     integer(c_int) function ec2d_closure_get_params(theta, norm) bind(C, name="ec2d_closure_get_params")
       import

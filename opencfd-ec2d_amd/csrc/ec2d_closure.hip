@@ -9,15 +9,15 @@
 //
 // Features of cell (i, j) (gradients by central differences in index space,
 // as the SA source term takes them, sub_turbulence_SA.f90:128-150):
-//   f0 = log(1 + dw^2 |omega| / nu)       vorticity Reynolds number
-//   f1 = log(1 + dw |u| / nu)             wall-distance Reynolds number
-//   f2 = |S| / (|S| + |omega| + 1e-30)    strain vs rotation
-//   f3 = |u| / c                          local Mach number
-//   f4 = u . grad p / (|u| |grad p| + 1e-30)   streamwise pressure gradient sign
-//   f5 = log(1 + dw |grad p| / (rho |u|^2 + 1e-30))
-// with nu = Amu / rho (Amu including 1/Re), omega = v_x - u_y,
-// |S| = sqrt(2 u_x^2 + 2 v_y^2 + (u_y + v_x)^2).  The network sees
-// (f - mean) * inv_std.
+//   f0 = log(1 + dw |u| / nu)             wall-distance Reynolds number
+//   f1 = log(1 + |S| dw / |u|)            strain magnitude over |u| / dw
+//   f2 = log(1 + |omega| dw / |u|)        vorticity magnitude over |u| / dw
+//   f3 = asinh(dw (u . grad p) / (rho |u|^3))   streamwise pressure-gradient parameter
+//   f4 = |u| / c                          local Mach number
+//   f5 = mu Re                            molecular viscosity over its free-stream value
+// with nu = mu / rho (mu by Sutherland's law including 1/Re), omega = v_x -
+// u_y, |S| = sqrt(2 u_x^2 + 2 v_y^2 + (u_y + v_x)^2).  The logarithms and
+// asinh only compress the heavy tails; the network sees (f - mean) * inv_std.
 // Network: 6 -> 64 -> 64 -> 64 -> 1, tanh hidden layers, softplus output.
 // Layout: weights transposed, Wt[n][k] (bf16), so that the MFMA B operand of
 // a lane is 4 consecutive k of one column; W1 is padded to K = 16, W4 to
@@ -125,13 +125,12 @@ __device__ __forceinline__ bool closure_features(float* f, const BlockDesc& B, c
   const double S = sqrt(2.0 * ux * ux + 2.0 * vy * vy + (uy + vx) * (uy + vx));
   const double um = sqrt(u * u + v * v);
   const double c = sqrt(g * p / d);
-  const double gp = sqrt(px * px + py * py);
-  f[0] = (float)log1p(dw * dw * om / nu);
-  f[1] = (float)log1p(dw * um / nu);
-  f[2] = (float)(S / (S + om + 1e-30));
-  f[3] = (float)(um / c);
-  f[4] = (float)((u * px + v * py) / (um * gp + 1e-30));
-  f[5] = (float)log1p(dw * gp / (d * um * um + 1e-30));
+  f[0] = (float)log1p(dw * um / nu);
+  f[1] = (float)log1p(S * dw / um);
+  f[2] = (float)log1p(om * dw / um);
+  f[3] = (float)asinh(dw * (u * px + v * py) / (d * um * um * um));
+  f[4] = (float)(um / c);
+  f[5] = (float)(mu * cp.Re);
   bool ok = true;
 #pragma unroll
   for (int k = 0; k < CL_NF; k++) ok = ok && isfinite(f[k]);
@@ -523,6 +522,16 @@ __global__ __launch_bounds__(256) void k_closure_train(TrainArgs a) {
   }
   float* out = a.part + (int64_t)blockIdx.x * (CL_NPARAM + 1);
   for (int k = threadIdx.x; k < CL_NPARAM + 1; k += 256) out[k] = sred[0][k];
+}
+
+// The workgroup partials of one mini-batch summed in order (the gradient
+// k_closure_adam steps with), into out[CL_NPARAM + 1] (the loss last).
+__global__ __launch_bounds__(256) void k_closure_gsum(const float* part, int nparts, float* out) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k > CL_NPARAM) return;
+  float g = 0.f;
+  for (int p = 0; p < nparts; p++) g += part[(int64_t)p * (CL_NPARAM + 1) + k];
+  out[k] = g;
 }
 
 // Sum the workgroup partials in order and take one Adam step on the FP32

@@ -25,6 +25,7 @@
 #include "ec2d_residual.hip"
 #include "ec2d_bl.hip"
 #include "ec2d_dw.hip"
+#include "ec2d_geom.hip"
 
 using namespace ec2d;
 
@@ -33,6 +34,7 @@ namespace {
 struct HBlock {
   int nx = 0, ny = 0, nsub = 0, fdm = 0;
   bool defined = false;
+  bool geom_dev = false;  // ghost nodes, centres and metrics made on the device at commit
   std::vector<int> bc;
   std::vector<double> x, y, x1, y1, vol, si, sj, ni1, ni2, nj1, nj2, dw, fd;
 };
@@ -78,6 +80,7 @@ struct DLevel {
   int lusgs_lpc = 1;          // LU-SGS lanes per cell: 2 when every block has at most 32 rows
   int lusgs_split = 1;        // LU-SGS sweep workgroups per block (2 or 4: a block's rows split between CUs)
   int lusgs_u = 2;            // LU-SGS sweep steps per register set (record prefetch depth), up to 4 waves
+  bool lusgs_fused = false;   // LU-SGS records made inside the sweep workgroup (k_lusgs_fused)
   unsigned long long* dgring = nullptr;  // split sweeps: one ring per boundary between workgroups and direction
   int64_t gstride = 0;                   // slots of one ring
   int4* dwall = nullptr;      // wall faces (block, face, i, j), cases in order
@@ -85,6 +88,7 @@ struct DLevel {
   double* dforces = nullptr;  // 4 per case
   double* sk = nullptr;       // LU-SGS records (skewed, see ec2d_lusgs.hip)
   int64_t sk_max_n = 0;       // largest record field of a block (doubles)
+  int64_t sk_total = 0;       // doubles of the level's record arena
   int max_cells = 0;          // largest block (cells)
   int prep_tiles = 0;         // LU-SGS prep workgroups per block: 32 x 8 cell tiles of the largest block
   std::vector<long long> case_cells;
@@ -260,6 +264,7 @@ int lusgs_layout(DLevel& L, int lpc, int split, bool upload) {
   }
   if (L.sk) HIPCHK(hipFree(L.sk));
   L.sk = nullptr;
+  L.sk_total = std::max<int64_t>(sk_off, 1);
   HIPCHK(hipMalloc(&L.sk, sizeof(double) * (size_t)std::max<int64_t>(sk_off, 1)));
   HIPCHK(hipMemset(L.sk, 0, sizeof(double) * (size_t)std::max<int64_t>(sk_off, 1)));
   L.lusgs_lpc = lpc;
@@ -578,20 +583,22 @@ int reduce_norms(DLevel& L) {
 
 void prof_mark(int kind);
 
-int launch_lusgs(int level, const int* dlist, int nblk, double alfa1, int update) {
+// sk, gring: the record arena and split-sweep rings to use (null: the level's own)
+int launch_lusgs(int level, const int* dlist, int nblk, double alfa1, int update, double* sk = nullptr,
+                 unsigned long long* gring = nullptr) {
   DLevel& L = R.lv[level];
   LusgsArgs a;
   a.blocks = L.db;
   a.block_list = dlist;
   a.cps = R.dcps;
   a.arena = L.arena;
-  a.sk = L.sk;
+  a.sk = sk ? sk : L.sk;
   a.pm = L.pm;
   a.alfa1 = alfa1;
   a.update = update;
   a.split = L.lusgs_split;
   a.dir = 0;
-  a.gring = L.dgring;
+  a.gring = gring ? gring : L.dgring;
   a.gstride = L.gstride;
   a.err = R.dsweep_err;
   bool visc = R.cases[0].cp.If_viscous == 1;
@@ -608,6 +615,11 @@ int launch_lusgs(int level, const int* dlist, int nblk, double alfa1, int update
   if (nth < 64 || nth > 1024 || nth % 64 != 0) return fail("LU-SGS launch shape out of range");
   const int pk = 10 * level;
 #define LLAUNCH(NV, V)                                                                      \
+  if (L.lusgs_fused) {                                                                      \
+    prof_mark(5 + pk);                                                                      \
+    hipLaunchKernelGGL((k_lusgs_fused<NV, V>), dim3(nblk), dim3(64 * (1 + FPROD)), 0, R.s, a); \
+    prof_mark(-(5 + pk));                                                                   \
+  } else {                                                                                  \
   prof_mark(4 + pk);                                                                        \
   hipLaunchKernelGGL((k_lusgs_prep<NV, V>), gprep, dim3(256), 0, R.s, a);                   \
   prof_mark(-(4 + pk));                                                                     \
@@ -629,6 +641,7 @@ int launch_lusgs(int level, const int* dlist, int nblk, double alfa1, int update
   }                                                                                         \
   }                                                                                         \
   prof_mark(-(5 + pk));                                                                     \
+  }                                                                                         \
   prof_mark(6 + pk);                                                                        \
   hipLaunchKernelGGL((k_lusgs_scatter<NV, V>), gscat, dim3(256), 0, R.s, a);                \
   prof_mark(-(6 + pk));
@@ -1013,6 +1026,15 @@ int array_map(const DLevel& L, int which, int* slot, int* ncomp, int* kind) {
     case EC2D_UN1: if (p.Un1 < 0) return 1; *slot = p.Un1; *ncomp = L.o.nvar; *kind = 3; return 0;
     case EC2D_CLOSURE_MUT: if (p.clo < 0) return 1; *slot = p.clo; *ncomp = 1; *kind = 5; return 0;
     case EC2D_CLOSURE_MASK: if (p.clo < 0) return 1; *slot = p.clo + 1; *ncomp = 1; *kind = 5; return 0;
+    case EC2D_VOL: *slot = p.vol; *ncomp = 1; *kind = 1; return 0;
+    case EC2D_SI: *slot = p.si; *ncomp = 1; *kind = 1; return 0;
+    case EC2D_SJ: *slot = p.sj; *ncomp = 1; *kind = 1; return 0;
+    case EC2D_NI1: *slot = p.ni1; *ncomp = 1; *kind = 1; return 0;
+    case EC2D_NI2: *slot = p.ni2; *ncomp = 1; *kind = 1; return 0;
+    case EC2D_NJ1: *slot = p.nj1; *ncomp = 1; *kind = 1; return 0;
+    case EC2D_NJ2: *slot = p.nj2; *ncomp = 1; *kind = 1; return 0;
+    case EC2D_X1: *slot = p.x1; *ncomp = 1; *kind = 5; return 0;
+    case EC2D_Y1: *slot = p.y1; *ncomp = 1; *kind = 5; return 0;
   }
   return 1;
 }
@@ -1124,6 +1146,11 @@ struct Lcg {
   double uniform() { return (next() + 0.5) / 2147483648.0; }  // (0, 1)
 };
 
+// autotune report text of an LU-SGS shape (0: the fused launch)
+std::string shape_text(int sp) {
+  return sp == 0 ? std::string("fused preparation and sweeps") : "workgroups per block " + std::to_string(sp);
+}
+
 extern "C" {
 
 int ec2d_init(void) {
@@ -1233,13 +1260,123 @@ int ec2d_define_block(int case_id, int nMesh, int mBlock, int nx, int ny, int ns
   b.nx = nx; b.ny = ny; b.nsub = nsub; b.fdm = fvm_fdm;
   b.bc.assign(bc_msg, bc_msg + 9 * nsub);
   size_t nn = (size_t)(nx + 4) * (ny + 4), ng = (size_t)(nx + 3) * (ny + 3), np = (size_t)nx * ny;
+  if (!x || !y) return fail("block nodes missing");
   b.x.assign(x, x + nn); b.y.assign(y, y + nn);
-  b.x1.assign(x1, x1 + ng); b.y1.assign(y1, y1 + ng);
-  b.vol.assign(vol, vol + np); b.si.assign(si, si + np); b.sj.assign(sj, sj + np);
-  b.ni1.assign(ni1, ni1 + np); b.ni2.assign(ni2, ni2 + np); b.nj1.assign(nj1, nj1 + np); b.nj2.assign(nj2, nj2 + np);
+  b.geom_dev = x1 == nullptr;
+  if (b.geom_dev) {
+    // the device makes the rest at commit from the interior nodes
+    if (y1 || vol || si || sj || ni1 || ni2 || nj1 || nj2)
+      return fail("give all of x1, y1, vol, si, sj, ni1, ni2, nj1, nj2, or none (made on the device)");
+    if (fvm_fdm) return fail("finite-difference blocks need their metrics from the caller");
+  } else {
+    if (!y1 || !vol || !si || !sj || !ni1 || !ni2 || !nj1 || !nj2) return fail("block geometry array missing");
+    b.x1.assign(x1, x1 + ng); b.y1.assign(y1, y1 + ng);
+    b.vol.assign(vol, vol + np); b.si.assign(si, si + np); b.sj.assign(sj, sj + np);
+    b.ni1.assign(ni1, ni1 + np); b.ni2.assign(ni2, ni2 + np); b.nj1.assign(nj1, nj1 + np); b.nj2.assign(nj2, nj2 + np);
+  }
   if (dw) b.dw.assign(dw, dw + (size_t)(nx + 1) * (ny + 1)); else b.dw.clear();
   if (fd && fvm_fdm) b.fd.assign(fd, fd + 5 * (size_t)(nx - 1) * (ny - 1)); else b.fd.clear();
   b.defined = true;
+  return 0;
+}
+
+// Blocks defined without their geometry: ghost nodes, cell centres and face
+// metrics from the interior nodes on the device (k_geometry), one launch per
+// mesh level over every such block of the batch; the results go to the host
+// copies the rest of ec2d_commit uploads and reads.
+int device_geometry() {
+  int nlev = 0;
+  for (auto& c : R.cases) nlev = std::max(nlev, c.num_mesh);
+  for (int l = 0; l < nlev; l++) {
+    std::vector<HBlock*> blks;
+    std::vector<int> first;  // launch index of each case's block 1 on this level (-1: none)
+    for (auto& c : R.cases) {
+      if (l >= c.num_mesh) { first.push_back(-1); continue; }
+      int ndev = 0;
+      for (auto& b : c.levels[l]) ndev += b.geom_dev ? 1 : 0;
+      if (ndev && ndev != (int)c.levels[l].size()) return fail("a case mixes blocks with and without geometry");
+      first.push_back(ndev ? (int)blks.size() : -1);
+      if (ndev)
+        for (auto& b : c.levels[l]) blks.push_back(&b);
+    }
+    if (blks.empty()) continue;
+    std::vector<int64_t> noff, coff, moff;
+    std::vector<int> dims, bc, sub0, nbblk;
+    std::vector<double> xs, ys;
+    int64_t nc = 0, nm = 0;
+    size_t k = 0;
+    for (size_t ci = 0; ci < R.cases.size(); ci++) {
+      if (first[ci] < 0) continue;
+      for (const HBlock& h : R.cases[ci].levels[l]) {
+        noff.push_back((int64_t)xs.size());
+        xs.insert(xs.end(), h.x.begin(), h.x.end());
+        ys.insert(ys.end(), h.y.begin(), h.y.end());
+        dims.push_back(h.nx);
+        dims.push_back(h.ny);
+        sub0.push_back((int)k);
+        for (int s = 0; s < h.nsub; s++, k++) {
+          const int* m = &h.bc[9 * s];
+          bc.insert(bc.end(), m, m + 9);
+          int nb = -1;
+          if (m[6] > 0) {
+            if (m[6] > (int)R.cases[ci].levels[l].size()) return fail("interface to a missing block");
+            const HBlock& o = R.cases[ci].levels[l][m[6] - 1];
+            if (m[7] < 1 || m[7] > o.nsub) return fail("interface to a missing subface");
+            nb = first[ci] + m[6] - 1;
+          }
+          nbblk.push_back(nb);
+        }
+        coff.push_back(nc);
+        nc += 2 * (int64_t)(h.nx + 3) * (h.ny + 3);
+        moff.push_back(nm);
+        nm += 7 * (int64_t)h.nx * h.ny;
+      }
+    }
+    sub0.push_back((int)k);
+    GeomArgs a;
+    double *dx = nullptr, *dy = nullptr, *dc = nullptr, *dm = nullptr;
+    int64_t *dno = nullptr, *dco = nullptr, *dmo = nullptr;
+    int *ddims = nullptr, *dbc = nullptr, *dsub0 = nullptr, *dnb = nullptr;
+    auto up = [&](auto** dst, const auto& v) -> int {
+      using T = std::remove_reference_t<decltype(v[0])>;
+      HIPCHK(hipMalloc(dst, sizeof(T) * std::max<size_t>(v.size(), 1)));
+      if (!v.empty()) HIPCHK(hipMemcpy(*dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+      return 0;
+    };
+    int rc = up(&dx, xs) || up(&dy, ys) || up(&dno, noff) || up(&dco, coff) || up(&dmo, moff) || up(&ddims, dims) ||
+             up(&dbc, bc) || up(&dsub0, sub0) || up(&dnb, nbblk);
+    if (!rc && (hipMalloc(&dc, sizeof(double) * std::max<int64_t>(nc, 1)) != hipSuccess ||
+                hipMalloc(&dm, sizeof(double) * std::max<int64_t>(nm, 1)) != hipSuccess))
+      rc = fail("hipMalloc failed");
+    std::vector<double> cen((size_t)nc), met((size_t)nm);
+    if (!rc) {
+      a.x = dx; a.y = dy; a.noff = dno; a.dims = ddims; a.bc = dbc; a.sub0 = dsub0; a.nbblk = dnb;
+      a.cen = dc; a.coff = dco; a.met = dm; a.moff = dmo;
+      hipLaunchKernelGGL(k_geometry, dim3((unsigned)blks.size()), dim3(256), 0, R.s, a);
+      if (hipGetLastError() != hipSuccess || hipStreamSynchronize(R.s) != hipSuccess ||
+          hipMemcpy(xs.data(), dx, sizeof(double) * xs.size(), hipMemcpyDeviceToHost) != hipSuccess ||
+          hipMemcpy(ys.data(), dy, sizeof(double) * ys.size(), hipMemcpyDeviceToHost) != hipSuccess ||
+          hipMemcpy(cen.data(), dc, sizeof(double) * cen.size(), hipMemcpyDeviceToHost) != hipSuccess ||
+          hipMemcpy(met.data(), dm, sizeof(double) * met.size(), hipMemcpyDeviceToHost) != hipSuccess)
+        rc = fail("geometry kernel failed");
+    }
+    hipFree(dx); hipFree(dy); hipFree(dc); hipFree(dm); hipFree(dno); hipFree(dco); hipFree(dmo);
+    hipFree(ddims); hipFree(dbc); hipFree(dsub0); hipFree(dnb);
+    if (rc) return rc;
+    for (size_t b = 0; b < blks.size(); b++) {
+      HBlock& h = *blks[b];
+      const size_t nn = h.x.size(), ng = (size_t)(h.nx + 3) * (h.ny + 3), np = (size_t)h.nx * h.ny;
+      std::copy(xs.begin() + noff[b], xs.begin() + noff[b] + nn, h.x.begin());
+      std::copy(ys.begin() + noff[b], ys.begin() + noff[b] + nn, h.y.begin());
+      const double* c0 = cen.data() + coff[b];
+      h.x1.assign(c0, c0 + ng);
+      h.y1.assign(c0 + ng, c0 + 2 * ng);
+      const double* m0 = met.data() + moff[b];
+      std::vector<double>* dst[7] = {&h.vol, &h.si, &h.sj, &h.ni1, &h.ni2, &h.nj1, &h.nj2};
+      for (int q = 0; q < 7; q++) dst[q]->assign(m0 + q * np, m0 + (q + 1) * np);
+      h.geom_dev = false;
+    }
+  }
   return 0;
 }
 
@@ -1331,6 +1468,8 @@ int ec2d_commit(void) {
       for (auto& b : c.levels[l])
         if (!b.defined) return fail("undefined block");
     }
+  // geometry the caller left to the device (before anything reads it)
+  if (device_geometry()) return 1;
   std::vector<CaseParams> cps;
   for (auto& c : R.cases) cps.push_back(c.cp);
   HIPCHK(hipMalloc(&R.dcps, sizeof(CaseParams) * cps.size()));
@@ -1603,6 +1742,8 @@ int ec2d_put_array(int case_id, int nMesh, int mBlock, int which, const double* 
   if (case_id < 0 || case_id >= (int)R.cases.size() || nMesh < 1 || nMesh > R.num_mesh) return fail("bad case / mesh");
   int gb = global_block(case_id, nMesh - 1, mBlock);
   if (gb < 0) return fail("bad block");
+  if ((which >= EC2D_VOL && which <= EC2D_NJ2) || which == EC2D_X1 || which == EC2D_Y1)
+    return fail("the geometry is fixed at ec2d_commit (read-only selector)");
   DLevel& L = R.lv[nMesh - 1];
   int slot, nc, kind;
   if (array_map(L, which, &slot, &nc, &kind)) return fail("bad array selector");
@@ -1974,8 +2115,16 @@ int ec2d_lusgs_shape(int nMesh, int lanes_per_cell, int workgroups_per_block) {
   if (check_ready() || nMesh < 1 || nMesh > R.num_mesh) return fail("bad mesh level");
   if (lanes_per_cell != 1 && lanes_per_cell != 2) return fail("lanes per cell must be 1 or 2");
   HIPCHK(hipStreamSynchronize(R.s));
-  if (lusgs_layout(R.lv[nMesh - 1], lanes_per_cell, workgroups_per_block, true)) return 1;
-  R.lv[nMesh - 1].lusgs_u = 2;
+  DLevel& L = R.lv[nMesh - 1];
+  const bool fused = workgroups_per_block == 0;
+  if (fused) {
+    if (lanes_per_cell != 1) return fail("the fused LU-SGS launch takes one lane per cell");
+    for (auto& b : L.hb)
+      if (b.ny - 1 > 64) return fail("the fused LU-SGS launch needs blocks of at most 64 rows of cells");
+  }
+  if (lusgs_layout(L, lanes_per_cell, fused ? 1 : workgroups_per_block, true)) return 1;
+  L.lusgs_u = 2;
+  L.lusgs_fused = fused;
   if (R.graph) {  // a captured graph holds the old launch shape
     HIPCHK(hipGraphExecDestroy(R.graph));
     R.graph = nullptr;
@@ -2031,6 +2180,7 @@ int ec2d_autotune(const char* cache_path, int reps, char* report, int report_len
       // LU-SGS lanes per cell and workgroups per block (the state after a
       // residual; dU only is written)
       best = 1e30;
+      L.lusgs_fused = false;
       for (int sp : {1, 2, 4})
         for (int lp : {1, 2}) {
           if (lusgs_layout(L, lp, sp, true)) {
@@ -2049,24 +2199,35 @@ int ec2d_autotune(const char* cache_path, int reps, char* report, int report_len
           }
           L.lusgs_u = 2;
         }
+      // the fused variant: records made in the sweep workgroup (rows <= 64)
+      bool fits = true;
+      for (auto& b : L.hb) fits = fits && b.ny - 1 <= 64;
+      if (fits && !lusgs_layout(L, 1, 1, true)) {
+        L.lusgs_fused = true;
+        double ms;
+        if (time_launches(reps, [&] { return launch_lusgs(l, L.dblocks_all, (int)L.hb.size(), 0.0, 0); }, &ms))
+          return 1;
+        if (ms < best) { best = ms; best_lp = 1; best_sp = 0; best_u = 2; }
+        L.lusgs_fused = false;
+      }
+      R.err.clear();
       HIPCHK(hipMemcpyAsync(L.arena, keep, sizeof(double) * (size_t)L.arena_n, hipMemcpyDeviceToDevice, R.s));
       HIPCHK(hipStreamSynchronize(R.s));
       HIPCHK(hipFree(keep));
       if (best_lp < 0) return fail("no LU-SGS layout fits this level");
       added.push_back({key, {best_tr, best_lp, best_sp, best_u}});
-      std::snprintf(line, sizeof(line),
-                    "level %d: measured tile_rows %d, lanes per cell %d, workgroups per block %d, steps per set %d\n",
-                    l + 1, best_tr, best_lp, best_sp, best_u);
+      std::snprintf(line, sizeof(line), "level %d: measured tile_rows %d, lanes per cell %d, %s, steps per set %d\n",
+                    l + 1, best_tr, best_lp, shape_text(best_sp).c_str(), best_u);
     } else {
-      std::snprintf(line, sizeof(line),
-                    "level %d: cached tile_rows %d, lanes per cell %d, workgroups per block %d, steps per set %d\n",
-                    l + 1, best_tr, best_lp, best_sp, best_u);
+      std::snprintf(line, sizeof(line), "level %d: cached tile_rows %d, lanes per cell %d, %s, steps per set %d\n",
+                    l + 1, best_tr, best_lp, shape_text(best_sp).c_str(), best_u);
     }
     rep_txt += line;
     L.tile_rows = best_tr;
     if (build_tiles(L)) return 1;
-    if (lusgs_layout(L, best_lp, best_sp, true)) return 1;
+    if (lusgs_layout(L, best_lp, best_sp == 0 ? 1 : best_sp, true)) return 1;
     L.lusgs_u = best_u;
+    L.lusgs_fused = best_sp == 0;
   }
   if (!added.empty() && cache_path && cache_path[0]) {
     if (FILE* f = std::fopen(cache_path, "a")) {
@@ -2202,6 +2363,45 @@ int ec2d_closure_train(int epochs, int batch, double lr, unsigned seed, double* 
   }
   if (first_epoch_loss) *first_epoch_loss = f / steps_per_epoch;
   if (last_epoch_loss) *last_epoch_loss = l / steps_per_epoch;
+  return 0;
+}
+
+int ec2d_closure_gradient(int n, float* grad, double* loss) {
+  if (check_ready()) return 1;
+  if (!R.clo.data) return fail("no training data (ec2d_closure_collect)");
+  if (!R.clo.ready) return fail("closure has no parameters (train or set them first)");
+  if (n < 64 || n % 64 != 0) return fail("the gradient batch must be a positive multiple of 64");
+  std::vector<float> d((size_t)8 * R.clo.ncell);
+  HIPCHK(hipMemcpyAsync(d.data(), R.clo.data, sizeof(float) * d.size(), hipMemcpyDeviceToHost, R.s));
+  HIPCHK(hipStreamSynchronize(R.s));
+  std::vector<int> idx;
+  for (int c = 0; c < R.clo.ncell && (int)idx.size() < n; c++)
+    if (d[(size_t)8 * c + 7] != 0.f) idx.push_back(c);
+  if ((int)idx.size() < n) return fail("fewer valid samples than the gradient batch");
+  int* didx = nullptr;
+  float *part = nullptr, *out = nullptr;
+  HIPCHK(hipMalloc(&didx, sizeof(int) * n));
+  HIPCHK(hipMalloc(&part, sizeof(float) * (size_t)(n / 64) * (CL_NPARAM + 1)));
+  HIPCHK(hipMalloc(&out, sizeof(float) * (CL_NPARAM + 1)));
+  HIPCHK(hipMemcpyAsync(didx, idx.data(), sizeof(int) * n, hipMemcpyHostToDevice, R.s));
+  TrainArgs ta;
+  ta.net = R.clo.dnet;
+  ta.data = R.clo.data;
+  ta.idx = didx;
+  ta.nbatch = n;
+  ta.part = part;
+  hipLaunchKernelGGL(k_closure_train, dim3(n / 64), dim3(256), 0, R.s, ta);
+  hipLaunchKernelGGL(k_closure_gsum, dim3((CL_NPARAM + 1 + 255) / 256), dim3(256), 0, R.s, part, n / 64, out);
+  std::vector<float> g(CL_NPARAM + 1);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(g.data(), out, sizeof(float) * g.size(), hipMemcpyDeviceToHost, R.s);
+  if (e == hipSuccess) e = hipStreamSynchronize(R.s);
+  hipFree(didx);
+  hipFree(part);
+  hipFree(out);
+  if (e != hipSuccess) return fail(std::string("closure gradient: ") + hipGetErrorString(e));
+  if (grad) std::memcpy(grad, g.data(), sizeof(float) * CL_NPARAM);
+  if (loss) *loss = g[CL_NPARAM];
   return 0;
 }
 
@@ -2354,7 +2554,7 @@ int ec2d_res_norms(int case_id, int nMesh, double* res_max, double* res_rms) {
   int se = 0;
   HIPCHK(hipMemcpyAsync(&se, R.dsweep_err, sizeof(se), hipMemcpyDeviceToHost, R.s));
   HIPCHK(hipStreamSynchronize(R.s));
-  if (se) return fail("a split LU-SGS sweep timed out waiting for its other half (results invalid)");
+  if (se) return fail("an LU-SGS sweep timed out waiting for another wave or workgroup (results invalid)");
   for (int m = 0; m < 6; m++) {
     res_max[m] = buf[m];
     res_rms[m] = buf[6 + m];

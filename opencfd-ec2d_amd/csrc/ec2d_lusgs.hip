@@ -218,17 +218,14 @@ __device__ __forceinline__ void load_rec(Rec<NV, NS>& p, __amdgpu_buffer_rsrc_t 
 // (i-1,j) and (i,j-1); the face areas are the cell's own (sub_LU_SGS.f90:43-56).
 // Backward: cell (ncol-c, nrow-r), neighbours (i+1,j) and (i,j+1), areas of the
 // neighbour, normals of the cell itself (sub_LU_SGS.f90:86-99).
-template <int NV, bool VISC, bool FWD>
-__device__ __forceinline__ void write_rec(double* __restrict__ rb, int64_t n, int64_t pos, const BlockDesc& B,
-                                          const double* base, const PlaneMap& pm, int c, int r, double g,
-                                          double alfa1) {
+// SH(d) and SD(k, e) give the destination of shared double d and of side
+// double e of side k (the global skewed records of k_lusgs_prep, or an LDS
+// slot of k_lusgs_fused).
+template <int NV, bool VISC, bool FWD, class SHF, class SDF>
+__device__ __forceinline__ void make_rec(SHF SH, SDF SD, const BlockDesc& B, const double* base, const PlaneMap& pm,
+                                         int c, int r, double g, double alfa1) {
   using F = RecF<NV, VISC>;
   const int ncol = B.nx - 1, nrow = B.ny - 1;
-  // shared double d, side double e of side k (layout of RecF)
-  auto SH = [&](int d) -> double& { return rb[(int64_t)(d / 2) * 2 * n + 2 * pos + (d & 1)]; };
-  auto SD = [&](int k, int e) -> double& {
-    return rb[(int64_t)F::NSH * n + (int64_t)(e / 2) * 4 * n + 2 * (2 * pos + k) + (e & 1)];
-  };
   auto PL = [&](int slot) { return base + (int64_t)slot * B.plane; };
   // A padding position is never active in a sweep, and a side without a
   // neighbour is never used (the sweep selects 0 for it): neither is
@@ -280,6 +277,19 @@ __device__ __forceinline__ void write_rec(double* __restrict__ rb, int64_t n, in
     SD(k, F::LC) = (k == 0 ? Lcj : Lci)[qn];
     if constexpr (VISC) SD(k, F::LV) = (k == 0 ? Lvj : Lvi)[qn];
   }
+}
+
+template <int NV, bool VISC, bool FWD>
+__device__ __forceinline__ void write_rec(double* __restrict__ rb, int64_t n, int64_t pos, const BlockDesc& B,
+                                          const double* base, const PlaneMap& pm, int c, int r, double g,
+                                          double alfa1) {
+  using F = RecF<NV, VISC>;
+  // shared double d, side double e of side k (layout of RecF)
+  auto SH = [&](int d) -> double& { return rb[(int64_t)(d / 2) * 2 * n + 2 * pos + (d & 1)]; };
+  auto SD = [&](int k, int e) -> double& {
+    return rb[(int64_t)F::NSH * n + (int64_t)(e / 2) * 4 * n + 2 * (2 * pos + k) + (e & 1)];
+  };
+  make_rec<NV, VISC, FWD>(SH, SD, B, base, pm, c, r, g, alfa1);
 }
 
 // One thread per cell, a wave per 8 x 8 tile of cells (a workgroup per 32
@@ -518,7 +528,10 @@ __device__ __forceinline__ void sweep(__amdgpu_buffer_rsrc_t rs, uint32_t rec, u
                                       double g, Chain<NV>& ch) {
   Ahead<NV> ah = {};  // prog = 0: step 0 needs nothing (its input is never used)
   const int npad = (S + 2 * U - 1) / (2 * U) * (2 * U);
-  const int tl = min(t, R - 1);  // loads of lanes beyond the block stay in its records
+  // lanes past the block's last row load that row's records: the same cache
+  // lines as its own lane, so padding rows cost no memory traffic (their
+  // results never reach a cell)
+  const int tl = min(t, nrow - 1);
   constexpr int NS = 3 - LPC;
   Out<NV> own = {};
   Rec<NV, NS> A[U], B[U];
@@ -691,6 +704,167 @@ __global__ __launch_bounds__(MAXT) void k_lusgs_sweep(LusgsArgs a) {
     if (wave < nw) sweep<NV, VISC, false, U, LPC>(rs, bw, out, n16, S, R, ncol, nrow, t, k, lane, wave, cin, cout, g, ch[1]);
     else if (rcv) comm_receive<NV>(ch[1], gr(half - 1, 1), npad, a.err);
     else if (snd) comm_forward<NV>(ch[1], nw, gr(half, 1), npad);
+  }
+}
+
+// ------------------------------------------------- preparation in the sweep
+// For levels whose blocks have at most 64 rows (one wave per block, one lane
+// per row): k_lusgs_fused does the work of k_lusgs_prep inside the sweep's
+// workgroup, so the records never go through HBM.  Wave 0 runs both sweeps
+// exactly as k_lusgs_sweep<..., LPC = 1> does (the same sweep_step); waves
+// 1..FPROD produce the records, each every FPROD-th step of the sequence
+// (forward steps 0..npad-1, then backward steps 0..npad-1), lane t the cell
+// of row t, straight from the block planes (make_rec, the arithmetic of
+// k_lusgs_prep) into a ring of FRING LDS slots.  The forward dU still goes to
+// the backward records' X pairs in global memory (the backward sweep reads it
+// there, after a fence), and the backward dU to the output pairs that
+// k_lusgs_scatter reads.
+//
+// Protocol: produced[p] counts the steps producer p has written, consumed
+// the steps the sweep wave has read.  The sweep wave waits, before step g,
+// for produced[g % FPROD] > g / FPROD; producer p waits, before writing
+// step g into slot g % FRING, for consumed >= g - FRING + 1.  A producer
+// waits only for the sweep wave and the sweep wave only for producers that
+// are never more than FRING steps ahead, so every wait ends; waits are also
+// bounded (SPIN_MAX polls, then err is set and the wave stops waiting).
+constexpr int FRING = 6;  // LDS record slots
+constexpr int FPROD = 3;  // producer waves
+
+template <int NV, bool VISC>
+struct FusedLds {
+  using F = RecF<NV, VISC>;
+  double sh[FRING][F::NSH][64];     // shared doubles of slot, lane
+  double sd[FRING][2][F::NSD][64];  // side doubles
+  int produced[FPROD];
+  int consumed;
+};
+
+__device__ __forceinline__ bool wait_bounded(const int* p, int v, int* err) {
+  int n = 0;
+  while (lds_get(p) < v) {
+    if (++n >= SPIN_MAX) {
+      atomicOr(err, 2);
+      return false;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  return true;
+}
+
+template <int NV, bool VISC>
+__global__ __launch_bounds__(64 * (1 + FPROD)) void k_lusgs_fused(LusgsArgs a) {
+  using F = RecF<NV, VISC>;
+  __shared__ FusedLds<NV, VISC> L;
+  const BlockDesc B = a.blocks[a.block_list[blockIdx.x]];
+  if (case_frozen(B.case_id)) return;
+  const double g = a.cps[B.case_id].gamma;
+  const int ncol = B.nx - 1, nrow = B.ny - 1;
+  const int S = B.sk_S, R = B.sk_R;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (tid <= FPROD) {
+    if (tid < FPROD) L.produced[tid] = 0;
+    else L.consumed = 0;
+  }
+  __syncthreads();
+  constexpr int U = 2;  // the sweep's step padding (npad) of k_lusgs_sweep<..., 2, ..., 1>
+  const int npad = (S + 2 * U - 1) / (2 * U) * (2 * U);
+  const int ntot = 2 * npad;
+  const uint32_t n8 = (uint32_t)B.sk_n * 8u, n16 = 2u * n8;
+  const uint32_t bytes = (uint32_t)((2 * F::NREC + 2 * F::XP) * B.sk_n * 8);
+  const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(a.sk + B.sk_base, 0, bytes, 0x00020000);
+  const uint32_t bw = F::NREC * n8, out = 2 * F::NREC * n8;
+  bool ok = true;
+  if (wave > 0) {
+    // ---- producer
+    const double* base = a.arena + B.base;
+    const int p = wave - 1;
+    for (int gs = p, k = 0; gs < ntot && ok; gs += FPROD, k++) {
+      ok = wait_bounded(&L.consumed, gs - FRING + 1, a.err);
+      const int slot = gs % FRING;
+      const bool fwd = gs < npad;
+      const int st = fwd ? gs : gs - npad;
+      auto SH = [&](int d) -> double& { return L.sh[slot][d][lane]; };
+      auto SD = [&](int kk, int e) -> double& { return L.sd[slot][kk][e][lane]; };
+      if (fwd) make_rec<NV, VISC, true>(SH, SD, B, base, a.pm, st - lane, lane, g, a.alfa1);
+      else make_rec<NV, VISC, false>(SH, SD, B, base, a.pm, st - lane, lane, g, a.alfa1);
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);
+      __builtin_amdgcn_s_waitcnt(0xc07f);  // the slot's LDS writes done (lgkmcnt 0) before publishing
+      lds_put(&L.produced[p], k + 1);
+    }
+    return;
+  }
+  // ---- the sweep wave: lane t = row t
+  const int t = lane;
+  const int tl = min(t, nrow - 1);
+  Chain<NV>& ch = *reinterpret_cast<Chain<NV>*>(&L);  // no other compute wave: never touched
+  auto load_lds = [&](Rec<NV, 2>& r, int slot, bool fwd) {
+#pragma unroll
+    for (int m = 0; m < NV; m++) {
+      if (fwd) r.X[m] = L.sh[slot][m][t];
+      r.U[m] = L.sh[slot][F::U + m][t];
+    }
+    r.al0 = L.sh[slot][F::AL0][t];
+    r.y0 = L.sh[slot][F::Y0][t];
+    if constexpr (NV == 5) {
+      r.al4 = L.sh[slot][F::AL4][t];
+      r.y4 = L.sh[slot][F::Y4][t];
+    } else {
+      r.al4 = r.al0;
+      r.y4 = r.y0;
+    }
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+#pragma unroll
+      for (int m = 0; m < NV; m++) r.F[h][m] = L.sd[slot][h][F::F + m][t];
+      r.n1[h] = L.sd[slot][h][F::N1][t];
+      r.n2[h] = L.sd[slot][h][F::N2][t];
+      r.a[h] = L.sd[slot][h][F::A][t];
+      r.Lc[h] = L.sd[slot][h][F::LC][t];
+      if constexpr (VISC) r.Lv[h] = L.sd[slot][h][F::LV][t];
+      else r.Lv[h] = 0.0;
+    }
+  };
+  // the backward sweep's X (the forward dU) from the backward records' X pairs
+  auto load_x = [&](double* x, int st) {
+    const uint32_t v16 = (uint32_t)(min(st, S - 1) * R + tl) * 16u;
+#pragma unroll
+    for (int j = 0; j < F::XP; j++) {
+      const D2 v = bload2(rs, v16, bw + (uint32_t)j * n16);
+      x[2 * j] = v.x;
+      if (2 * j + 1 < NV) x[2 * j + 1] = v.y;
+    }
+  };
+  for (int dir = 0; dir < 2; dir++) {
+    const bool fwd = dir == 0;
+    Ahead<NV> ah = {};
+    Out<NV> own = {};
+    double xa[2 * F::XP], xb[2 * F::XP];
+    if (!fwd) {
+      // the forward sweep's stores to the X pairs before these loads
+      __threadfence();
+      load_x(xa, 0);
+    }
+    for (int st = 0; st < npad; st++) {
+      const int gs = dir * npad + st;
+      if (!fwd && st + 1 < npad) load_x((st & 1) ? xa : xb, st + 1);
+      if (ok) ok = wait_bounded(&L.produced[gs % FPROD], gs / FPROD + 1, a.err);
+      Rec<NV, 2> r;
+      load_lds(r, gs % FRING, fwd);
+      if (!fwd) {
+        const double* x = (st & 1) ? xb : xa;
+#pragma unroll
+        for (int m = 0; m < NV; m++) r.X[m] = x[m];
+      }
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);
+      __builtin_amdgcn_s_waitcnt(0xc07f);  // the slot's LDS reads done before it is released
+      lds_put(&L.consumed, gs + 1);
+      if (fwd)
+        sweep_step<NV, VISC, true, 1>(r, own, rs, bw, n16, st, S, R, ncol, nrow, t, 0, lane, 0, false, false, g, ch, ah);
+      else
+        sweep_step<NV, VISC, false, 1>(r, own, rs, out, n16, st, S, R, ncol, nrow, t, 0, lane, 0, false, false, g, ch,
+                                       ah);
+    }
   }
 }
 

@@ -18,6 +18,8 @@ EC2D_AMU, EC2D_AMU_T = 9, 10
 EC2D_UN, EC2D_QF, EC2D_DELTU = 23, 24, 25
 EC2D_UN1 = 31
 EC2D_DW = 22
+EC2D_VOL, EC2D_SI, EC2D_SJ, EC2D_NI1, EC2D_NI2, EC2D_NJ1, EC2D_NJ2 = 11, 12, 13, 14, 15, 16, 17
+EC2D_X1, EC2D_Y1 = 20, 21
 EC2D_CLOSURE_MUT, EC2D_CLOSURE_MASK = 40, 41
 CLOSURE_NPARAM = 6 * 64 + 64 + 64 * 64 + 64 + 64 * 64 + 64 + 64 + 1  # W1 b1 W2 b2 W3 b3 W4 b4
 CLOSURE_NNORM = 24  # mean, inv_std, lo, hi of the 6 features
@@ -80,6 +82,7 @@ SIGNATURES = {
     "ec2d_autotune": [C.c_char_p, _i, C.c_char_p, _i],
     "ec2d_closure_collect": [_ip],
     "ec2d_closure_train": [_i, _i, C.c_double, C.c_uint, _dp, _dp],
+    "ec2d_closure_gradient": [_i, C.POINTER(C.c_float), _dp],
     "ec2d_closure_get_params": [C.POINTER(C.c_float), C.POINTER(C.c_float)],
     "ec2d_closure_set_params": [C.POINTER(C.c_float), C.POINTER(C.c_float)],
     "ec2d_closure_infer": [],

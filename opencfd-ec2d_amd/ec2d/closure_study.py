@@ -1,19 +1,27 @@
 """Learned closure against SA on held-out airfoils, on one GPU.
 
-1. A training ensemble of seeded NACA mpxx cases (SA-RANS) runs in one batch
-   to a density-residual drop `drop` (or `max_iters`); the network trains on
-   the per-cell fields of the final states with the library's own BF16 MFMA
-   kernels.
-2. A held-out ensemble (another seed) runs twice, in one batch each: with SA
-   (five equations) and with the closure as its eddy-viscosity model (four
-   equations, turbulence model 4).  For every case: iterations and wall time
-   to the drop, and the final lift and drag.  A closure case whose residual
-   stalls (growth past `stall` x its best) is counted as falling back to SA.
+1. Training set: the SA-RANS ensemble of the 256-case workload (seeded NACA
+   mpxx sections, 129x49 .. 257x65, Roe + MUSCL3, LU-SGS) runs in one batch
+   until every case's density residual has dropped by `drop` (per-case
+   masks, ec2d_run_to_drop; a case that has not after `max_iter` passes
+   stops there).  The network trains on the per-cell fields of these final
+   states with the library's own BF16 MFMA forward / backward kernels.
+2. Held-out set: `n_test` parameter sets of another seed, each on a 513x129
+   C-grid, solved twice in one batch each, from freestream to the same drop:
+   with SA (five equations) and with the network as the eddy-viscosity model
+   (four equations, turbulence model 4).  SA judges the closure's lift and
+   drag.
+3. Per-case fallback: a closure case falls back to SA when its residual does
+   not reach the drop (it stalls: the pass limit, or a non-finite residual)
+   or when more than `range_frac` of its cells have features outside the
+   training range (the model then extrapolates); a fallen-back case costs its
+   closure time plus its SA time and reports SA's coefficients.
 
-The residual drop is the density Res_rms relative to iteration 1.  Prints
-progress lines and one JSON line with the results.
+Times are host wall times from the start of each batch's run to the check
+that saw the case stop (checks every 200 passes).  Prints progress lines and
+one JSON line.
 
-    python -m ec2d.closure_study --n-train 8 --n-test 4 --drop 1e-8
+    python -m ec2d.closure_study [--n-train 256] [--n-test 32] [--drop 1e-8]
 """
 import argparse
 import dataclasses
@@ -23,114 +31,130 @@ import sys
 import tempfile
 import time
 
-from . import capi, cases, solver
+import numpy as np
+
+from . import cases, solver
 from . import formats as F
 
 
-def _write(d, prm, scheme, model):
-    cases.write_ensemble_case(d, prm, iters=10 ** 7)
-    p = os.path.join(d, "control.in")
-    c = F.read_control(p)
-    c = dataclasses.replace(c, Iflag_Scheme=scheme, Iflag_turbulence_model=model)
-    F.write_control(p, c)
+def _write(d, prm, model, nx=None, ny=None):
+    p = dict(prm)
+    if nx:
+        p["nx"], p["ny"] = nx, ny
+    cases.write_ensemble_case(d, p, iters=10 ** 7)
+    path = os.path.join(d, "control.in")
+    c = dataclasses.replace(F.read_control(path), Iflag_turbulence_model=model)
+    F.write_control(path, c)
     return d
 
 
-def run_to_drop(b, ncase, drop, max_iters, chunk, label, stall=None):
-    """Run a batch until every case's density residual fell by `drop` (or
-    max_iters); per case: iterations to the drop (None if not reached), wall
-    seconds to the drop, final (Cl, Cd)."""
-    b.run(1)
-    b.synchronize()
-    r0 = [b.res_norms(c)[1][0] for c in range(ncase)]
-    it, t_run = 1, 0.0
-    hit = [None] * ncase
-    t_hit = [None] * ncase
-    stalled = False
-    while it < max_iters and any(h is None for h in hit):
+def _solve(dirs, drop, max_iter, label, theta=None, norm=None, relax=1.0):
+    """One batch to the drop; per case passes, status, seconds, (Cl, Cd) and,
+    with a closure, the fraction of cells whose features lay outside the
+    training range at the end."""
+    batch = [solver.Case.from_dir(x) for x in dirs]
+    with solver.Batch(batch) as b:
+        if theta is not None:
+            b.closure_set_params(theta, norm)
+            b.closure_relaxation(relax)
+        b.init_flow()
+        b.autotune(reps=3)
         t0 = time.perf_counter()
-        b.run(chunk)
-        b.synchronize()
-        t_run += time.perf_counter() - t0
-        it += chunk
-        try:
-            rel = [b.res_norms(c)[1][0] / r0[c] for c in range(ncase)]
-        except capi.Ec2dError as e:
-            # a cell's temperature fell below the reference's limit: with the
-            # closure this is the fallback to SA; with SA a failure
-            print(f"{label}: iteration {it}: {e}", flush=True)
-            return {"iters": hit, "seconds": t_hit, "coef": [(float("nan"), float("nan"))] * ncase,
-                    "ms_per_iter": 1e3 * t_run / max(it - 1, 1), "iterations_run": it, "stalled": True,
-                    "diverged": str(e)}
-        for c in range(ncase):
-            if hit[c] is None and rel[c] <= drop:
-                hit[c], t_hit[c] = it, t_run
-        if stall is not None and not b.closure_monitor(growth=stall):
-            stalled = True
-        print(f"{label}: iteration {it}, {t_run:.1f} s, worst drop {max(rel):.3e}, reached {sum(h is not None for h in hit)}/{ncase}",
+        iters, status, ms = b.run_to_drop(drop, max_iter=max_iter, check_every=200)
+        wall = time.perf_counter() - t0
+        coef = [b.force_coefficients(k)[:2] for k in range(len(batch))]
+        out_frac = None
+        if theta is not None:
+            out_frac = []
+            for k, c in enumerate(batch):
+                m = b.get("closure_mask", k)[2:-2, 2:-2]
+                out_frac.append(float(1.0 - m.mean()))
+        res = {"iters": iters, "status": status, "seconds": [v / 1e3 for v in ms], "coef": coef,
+               "out_of_range": out_frac, "wall_s": wall}
+        print(f"{label}: {sum(s == solver.Batch.CONVERGED for s in status)}/{len(batch)} converged in {wall:.1f} s",
               flush=True)
-        if stalled:
-            break
-    coef = [b.force_coefficients(c)[:2] for c in range(ncase)]
-    return {"iters": hit, "seconds": t_hit, "coef": coef, "ms_per_iter": 1e3 * t_run / max(it - 1, 1),
-            "iterations_run": it, "stalled": stalled, "diverged": None}
+        return res
 
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("--n-train", type=int, default=8)
-    ap.add_argument("--n-test", type=int, default=4)
+    ap.add_argument("--n-train", type=int, default=256)
+    ap.add_argument("--n-test", type=int, default=32)
     ap.add_argument("--seed-train", type=int, default=0)
     ap.add_argument("--seed-test", type=int, default=1)
+    ap.add_argument("--test-grid", default="513x129")
     ap.add_argument("--drop", type=float, default=1e-8)
-    ap.add_argument("--max-iters", type=int, default=120000)
-    ap.add_argument("--chunk", type=int, default=2000)
-    ap.add_argument("--scheme", type=int, default=F.SCHEME_UD1)
-    ap.add_argument("--epochs", type=int, default=30)
-    ap.add_argument("--stall", type=float, default=1e3)
+    ap.add_argument("--max-iter", type=int, default=60000)
+    ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--relax", type=float, default=1.0, help="under-relaxation of the closure's mu_t")
+    ap.add_argument("--range-frac", type=float, default=0.01,
+                    help="fall back to SA when more than this fraction of a case's cells leave the training range")
+    ap.add_argument("--out", default="", help="also write the JSON result to this file")
     args = ap.parse_args(argv)
+    nx, ny = (int(v) for v in args.test_grid.split("x"))
     train = cases.ensemble_params(seed=args.seed_train, n=args.n_train)
     test = cases.ensemble_params(seed=args.seed_test, n=args.n_test)
-    out = {"scheme": args.scheme, "drop": args.drop, "relax": args.relax, "train_cases": len(train), "test_cases": len(test)}
+    out = {"drop": args.drop, "max_iter": args.max_iter, "relax": args.relax, "train_cases": len(train),
+           "test_cases": len(test), "test_grid": args.test_grid}
     with tempfile.TemporaryDirectory() as d:
-        dirs = [_write(os.path.join(d, f"tr{k}"), p, args.scheme, F.TURB_SA) for k, p in enumerate(train)]
-        with solver.Batch([solver.Case.from_dir(x) for x in dirs]) as b:
+        # 1. training fields and training
+        dirs = [_write(os.path.join(d, f"tr{k}"), p, F.TURB_SA) for k, p in enumerate(train)]
+        batch = [solver.Case.from_dir(x) for x in dirs]
+        with solver.Batch(batch) as b:
             b.init_flow()
             b.autotune(reps=3)
-            tr = run_to_drop(b, len(dirs), args.drop, args.max_iters, args.chunk, "train SA")
+            t0 = time.perf_counter()
+            it, st, _ = b.run_to_drop(args.drop, max_iter=args.max_iter, check_every=200)
+            t_sa = time.perf_counter() - t0
+            print(f"training ensemble: {sum(s == solver.Batch.CONVERGED for s in st)}/{len(st)} converged "
+                  f"in {t_sa:.1f} s", flush=True)
             n = b.closure_collect()
             t0 = time.perf_counter()
-            first, last = b.closure_train(epochs=args.epochs, batch=1024, lr=2e-3, seed=1)
-            out["training"] = {"samples": n, "first_epoch_loss": first, "last_epoch_loss": last,
-                               "seconds": time.perf_counter() - t0, "sa_iterations_to_drop": tr["iters"]}
+            first, last = b.closure_train(epochs=args.epochs, batch=1024, lr=args.lr, seed=1)
+            t_tr = time.perf_counter() - t0
             theta, norm = b.closure_params()
-        print(f"trained on {n} cells: loss {first:.4g} -> {last:.4g}", flush=True)
-        res = {}
-        for label, model in (("sa", F.TURB_SA), ("closure", F.TURB_CLOSURE)):
-            dirs = [_write(os.path.join(d, f"{label}{k}"), p, args.scheme, model) for k, p in enumerate(test)]
-            with solver.Batch([solver.Case.from_dir(x) for x in dirs]) as b:
-                if model == F.TURB_CLOSURE:
-                    b.closure_set_params(theta, norm)
-                    b.closure_relaxation(args.relax)
-                b.init_flow()
-                b.autotune(reps=3)
-                res[label] = run_to_drop(b, len(dirs), args.drop, args.max_iters, args.chunk, f"held-out {label}",
-                                         stall=args.stall if model == F.TURB_CLOSURE else None)
-        per = []
-        for k in range(len(test)):
-            (cl_s, cd_s), (cl_c, cd_c) = res["sa"]["coef"][k], res["closure"]["coef"][k]
-            per.append({"case": k, "Ma": test[k]["Ma"], "aoa": test[k]["aoa"], "Re": test[k]["Re"],
-                        "grid": f"{test[k]['nx']}x{test[k]['ny']}",
-                        "sa_iters": res["sa"]["iters"][k], "closure_iters": res["closure"]["iters"][k],
-                        "sa_seconds": res["sa"]["seconds"][k], "closure_seconds": res["closure"]["seconds"][k],
-                        "Cl_sa": cl_s, "Cl_closure": cl_c, "Cd_sa": cd_s, "Cd_closure": cd_c,
-                        "Cl_rel_err": abs(cl_c - cl_s) / max(abs(cl_s), 1e-12), "Cd_err_counts": 1e4 * abs(cd_c - cd_s)})
-        out["held_out"] = per
-        out["ms_per_iteration"] = {k: v["ms_per_iter"] for k, v in res.items()}
-        out["closure_stalled"] = res["closure"]["stalled"]
-        out["closure_diverged"] = res["closure"]["diverged"]
-    print(json.dumps(out))
+        out["training"] = {"samples": n, "first_epoch_loss": first, "last_epoch_loss": last, "train_seconds": t_tr,
+                           "ensemble_seconds": t_sa, "ensemble_converged": int(sum(s == 1 for s in st)),
+                           "ensemble_passes_median": float(np.median(it))}
+        print(f"trained on {n} cells in {t_tr:.1f} s: loss {first:.4g} -> {last:.4g}", flush=True)
+        # 2. held-out cases, SA and closure
+        sa_dirs = [_write(os.path.join(d, f"sa{k}"), p, F.TURB_SA, nx, ny) for k, p in enumerate(test)]
+        cl_dirs = [_write(os.path.join(d, f"cl{k}"), p, F.TURB_CLOSURE, nx, ny) for k, p in enumerate(test)]
+        sa = _solve(sa_dirs, args.drop, args.max_iter, "held-out SA")
+        cl = _solve(cl_dirs, args.drop, args.max_iter, "held-out closure", theta, norm, args.relax)
+    per = []
+    for k in range(len(test)):
+        fell = (cl["status"][k] != solver.Batch.CONVERGED) or (cl["out_of_range"][k] > args.range_frac)
+        (cl_s, cd_s), (cl_c, cd_c) = sa["coef"][k], cl["coef"][k]
+        if fell:
+            cl_c, cd_c = cl_s, cd_s
+        t_c = cl["seconds"][k] + (sa["seconds"][k] if fell else 0.0)
+        per.append({"case": k, "Ma": test[k]["Ma"], "aoa": test[k]["aoa"], "Re": test[k]["Re"],
+                    "sa_status": sa["status"][k], "sa_passes": sa["iters"][k], "sa_seconds": sa["seconds"][k],
+                    "closure_status": cl["status"][k], "closure_passes": cl["iters"][k],
+                    "closure_seconds": cl["seconds"][k], "out_of_range": cl["out_of_range"][k], "fallback": fell,
+                    "seconds_with_fallback": t_c, "Cl_sa": cl_s, "Cd_sa": cd_s,
+                    "Cl_closure": cl["coef"][k][0], "Cd_closure": cl["coef"][k][1],
+                    "Cl_rel_err": abs(cl_c - cl_s) / max(abs(cl_s), 1e-12), "Cd_err_counts": 1e4 * abs(cd_c - cd_s)})
+    ok = [p for p in per if p["sa_status"] == solver.Batch.CONVERGED]
+    out["held_out"] = per
+    out["summary"] = {
+        "sa_converged": len(ok),
+        "closure_converged": sum(p["closure_status"] == solver.Batch.CONVERGED for p in per),
+        "fallbacks": sum(p["fallback"] for p in per),
+        "sa_batch_seconds": sa["wall_s"], "closure_batch_seconds": cl["wall_s"],
+        "median_time_ratio_closure_over_sa": (float(np.median([p["seconds_with_fallback"] / p["sa_seconds"]
+                                                               for p in ok])) if ok else None),
+        "cl_within_0.5pct": sum(p["Cl_rel_err"] <= 5e-3 for p in per),
+        "cd_within_1_count": sum(p["Cd_err_counts"] <= 1.0 for p in per),
+        "no_fallback_cl_err_median": (float(np.median([p["Cl_rel_err"] for p in per if not p["fallback"]]))
+                                      if any(not p["fallback"] for p in per) else None)}
+    line = json.dumps(out)
+    print(line)
+    if args.out:
+        with open(args.out, "w") as f:
+            f.write(line + "\n")
     return out
 
 

@@ -1,8 +1,10 @@
 """Host-side set-up of one OpenCFD-EC2D case: mesh levels, geometry, ghost
 coordinates, wall distance and finite-difference metrics.
 
-This is the set-up the reference performs once before its main loop; it stays
-on the host (it is not part of the per-iteration path) and feeds the device.
+This is the set-up the reference performs once before its main loop.  It
+feeds the device, which can also make all of it but the finite-difference
+metrics itself from the nodes (node_levels; ec2d_commit then runs the ghost
+nodes, centres, metrics and wall distance on the GPU).
 Every quantity is computed with the same floating-point operations, in the
 same order, as the reference, so that the arrays handed to the device are
 bit-identical to the reference's own:
@@ -284,6 +286,35 @@ def mesh_depth(blocks_dims, bcs) -> int:
                     break
             km_grid = min(km_grid, km)
     return km_grid
+
+
+def node_levels(xs, ys, bcs, num_mesh: int) -> List[List[BlockGeom]]:
+    """All mesh levels of one case with the interior nodes only (every second
+    node for each coarser level, sub_init.f90:276-373): ghost nodes, cell
+    centres, metrics and wall distance are left to the device (ec2d_commit)."""
+    depth = mesh_depth([(x.shape[1], x.shape[0]) for x in xs], bcs)
+    num_mesh = min(3, depth, num_mesh)
+    levels = []
+    for lev in range(num_mesh):
+        blocks = []
+        for k, (x, y, bc) in enumerate(zip(xs, ys, bcs)):
+            if lev == 0:
+                nx, ny, b, xn, yn = x.shape[1], x.shape[0], [list(s) for s in bc], x, y
+            else:
+                F_ = levels[-1][k]
+                nx, ny = (F_.nx - 1) // 2 + 1, (F_.ny - 1) // 2 + 1
+                xn = F_.x[LAP:LAP + F_.ny:2, LAP:LAP + F_.nx:2]
+                yn = F_.y[LAP:LAP + F_.ny:2, LAP:LAP + F_.nx:2]
+                b = [[s[0], s[1], (s[2] - 1) // 2 + 1, (s[3] - 1) // 2 + 1, (s[4] - 1) // 2 + 1, (s[5] - 1) // 2 + 1,
+                      s[6], s[7], s[8]] for s in F_.bc]
+            B = BlockGeom(nx=nx, ny=ny, bc=b)
+            B.x = np.full((ny + 2 * LAP, nx + 2 * LAP), np.nan)
+            B.y = np.full((ny + 2 * LAP, nx + 2 * LAP), np.nan)
+            B.x[LAP:LAP + ny, LAP:LAP + nx] = xn
+            B.y[LAP:LAP + ny, LAP:LAP + nx] = yn
+            blocks.append(B)
+        levels.append(blocks)
+    return levels
 
 
 def build_levels(xs, ys, bcs, num_mesh: int, fdm_blocks=(), need_dw: bool = False) -> List[List[BlockGeom]]:

@@ -45,13 +45,20 @@ class Case:
         self.id = -1
 
     @classmethod
-    def from_dir(cls, case_dir: str, device_dw: bool = False) -> "Case":
+    def from_dir(cls, case_dir: str, device_dw: bool = False, device_geometry: bool = False) -> "Case":
         """device_dw: leave the wall distance to the device (ec2d_commit runs
-        comput_dw there) instead of computing it on the host."""
+        comput_dw there) instead of computing it on the host.
+        device_geometry: leave ghost nodes, cell centres, face metrics and the
+        wall distance to the device (the host only reads the nodes and takes
+        every second one for the coarse levels); not with an FDM.in block."""
         ctl = F.read_control(os.path.join(case_dir, "control.in"))
         xs, ys = F.read_mesh2d(os.path.join(case_dir, "Mesh2d.dat"))
         bcs = F.read_bcin(os.path.join(case_dir, "bc2d_1.in"))
         fdm = F.read_fdm_in(os.path.join(case_dir, "FDM.in"))
+        if device_geometry:
+            if fdm:
+                raise ValueError("finite-difference blocks need their metrics from the host")
+            return cls(ctl, G.node_levels(xs, ys, bcs, ctl.Num_Mesh), case_dir)
         need_dw = ctl.Iflag_turbulence_model in (F.TURB_SA, F.TURB_SST, F.TURB_CLOSURE) and not device_dw
         levels = G.build_levels(xs, ys, bcs, ctl.Num_Mesh, fdm, need_dw)
         return cls(ctl, levels, case_dir)
@@ -134,7 +141,7 @@ class Batch:
                     fd = None
                     if B.fvm_fdm == 1 and lev == 0:
                         fd = np.ascontiguousarray(np.stack([B.fd[k] for k in ("Akx", "Aky", "Aix", "Aiy", "Ajac")]))
-                    arrs = [np.ascontiguousarray(getattr(B, n)) for n in
+                    arrs = [None if getattr(B, n) is None else np.ascontiguousarray(getattr(B, n)) for n in
                             ("x", "y", "x1", "y1", "vol", "si", "sj", "ni1", "ni2", "nj1", "nj2")]
                     dw = np.ascontiguousarray(B.dw) if (B.dw is not None and lev == 0) else None
                     capi.call("ec2d_define_block", cid, lev + 1, mb + 1, B.nx, B.ny, len(B.bc),
@@ -160,13 +167,24 @@ class Batch:
         capi.EC2D_DW: lambda B, nv: (B.ny + 1, B.nx + 1),
         capi.EC2D_QF: lambda B, nv: (B.ny - 1, B.nx - 1, 4),
         capi.EC2D_DELTU: lambda B, nv: (B.ny + 3, B.nx + 3, 4),
+        capi.EC2D_VOL: lambda B, nv: (B.ny, B.nx),
+        capi.EC2D_SI: lambda B, nv: (B.ny, B.nx),
+        capi.EC2D_SJ: lambda B, nv: (B.ny, B.nx),
+        capi.EC2D_NI1: lambda B, nv: (B.ny, B.nx),
+        capi.EC2D_NI2: lambda B, nv: (B.ny, B.nx),
+        capi.EC2D_NJ1: lambda B, nv: (B.ny, B.nx),
+        capi.EC2D_NJ2: lambda B, nv: (B.ny, B.nx),
+        capi.EC2D_X1: lambda B, nv: (B.ny + 3, B.nx + 3),
+        capi.EC2D_Y1: lambda B, nv: (B.ny + 3, B.nx + 3),
         capi.EC2D_CLOSURE_MUT: lambda B, nv: (B.ny + 3, B.nx + 3),
         capi.EC2D_CLOSURE_MASK: lambda B, nv: (B.ny + 3, B.nx + 3),
     }
     NAMES = {"U": capi.EC2D_U, "Res": capi.EC2D_RES, "dt": capi.EC2D_DT, "dU": capi.EC2D_DU, "Lci": capi.EC2D_LCI,
              "Lcj": capi.EC2D_LCJ, "Lvi": capi.EC2D_LVI, "Lvj": capi.EC2D_LVJ, "Amu": capi.EC2D_AMU,
              "Amu_t": capi.EC2D_AMU_T, "Un": capi.EC2D_UN, "Un1": capi.EC2D_UN1, "dw": capi.EC2D_DW, "QF": capi.EC2D_QF,
-             "deltU": capi.EC2D_DELTU,
+             "deltU": capi.EC2D_DELTU, "vol": capi.EC2D_VOL, "si": capi.EC2D_SI, "sj": capi.EC2D_SJ,
+             "ni1": capi.EC2D_NI1, "ni2": capi.EC2D_NI2, "nj1": capi.EC2D_NJ1, "nj2": capi.EC2D_NJ2,
+             "x1": capi.EC2D_X1, "y1": capi.EC2D_Y1,
              "closure_mut": capi.EC2D_CLOSURE_MUT, "closure_mask": capi.EC2D_CLOSURE_MASK}
 
     def _nv(self, case: Case, nmesh: int) -> int:
@@ -425,6 +443,13 @@ class Batch:
         f, l = C.c_double(), C.c_double()
         capi.call("ec2d_closure_train", epochs, batch, lr, seed, C.byref(f), C.byref(l))
         return f.value, l.value
+
+    def closure_gradient(self, n: int):
+        """(gradient of the loss over the first n valid samples as one batch, that batch's loss); no update."""
+        g = np.zeros(capi.CLOSURE_NPARAM, dtype=np.float32)
+        loss = C.c_double()
+        capi.call("ec2d_closure_gradient", int(n), g.ctypes.data_as(C.POINTER(C.c_float)), C.byref(loss))
+        return g, loss.value
 
     def closure_params(self):
         th = np.zeros(capi.CLOSURE_NPARAM, dtype=np.float32)

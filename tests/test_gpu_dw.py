@@ -53,3 +53,49 @@ def test_device_wall_distance_equals_host(kind):
         assert a[1:-1, 1:-1].max() < 1e9  # every cell found a wall
     for mb, (a, b) in enumerate(zip(U_d, U_h), start=1):
         assert first_diff(a, b) is None, f"U block {mb}: {first_diff(a, b)}"
+
+
+GEOM = ("vol", "si", "sj", "ni1", "ni2", "nj1", "nj2", "x1", "y1")
+
+
+@pytest.mark.parametrize("kind", ["cgrid_sa_mg2", "multiblock", "rae2822"])
+def test_device_geometry_equals_host(kind, tmp_path):
+    """Ghost nodes, cell centres, face metrics (and, for SA / SST, the wall
+    distance) made on the device at ec2d_commit from the interior nodes
+    equal the host restatement (pinned to the reference's own geometry) bit
+    for bit on every level and block, and so do two passes run on them."""
+    import os
+
+    from ec2d import cases as C_
+
+    if kind == "cgrid_sa_mg2":
+        c = C_.control_sa_naca(iters=10, Re=1e5, Ma=0.5, num_mesh=2, time_method=F.TIME_RK3)
+        c.CFL = 0.3
+        d = naca_case("g_devgeo_c", c, nx=129, ny=33, ds1=1e-4)
+    elif kind == "multiblock":
+        from helpers import vortex_case
+        d = vortex_case("g_devgeo_mb", C_.control_vortex(iters=10))
+    else:
+        from test_golden import load_case
+        load_case(os.path.join(os.path.dirname(__file__), "golden", "rae2822_sample_sst_isothermal.npz"),
+                  str(tmp_path))
+        d = str(tmp_path)
+    host = solver.Case.from_dir(d)
+    dev = solver.Case.from_dir(d, device_geometry=True)
+    U = {}
+    for label, case in (("host", host), ("device", dev)):
+        with solver.Batch([case]) as b:
+            geo = {(lev, mb, n): b.get(n, 0, lev, mb) for lev in range(1, host.num_mesh + 1)
+                   for mb in range(1, len(host.levels[lev - 1]) + 1) for n in GEOM}
+            if host.ctl.Iflag_turbulence_model in (F.TURB_SA, F.TURB_SST):
+                geo.update({(1, mb, "dw"): b.get("dw", 0, 1, mb) for mb in range(1, len(host.levels[0]) + 1)})
+            b.init_flow()
+            b.run(2)
+            U[label] = [b.get("U", 0, 1, mb) for mb in range(1, len(host.levels[0]) + 1)]
+            if label == "host":
+                ref_geo = geo
+    for key, v in geo.items():
+        diff = first_diff(v, ref_geo[key])
+        assert diff is None, f"{key}: {diff}"
+    for mb, (a_, b_) in enumerate(zip(U["device"], U["host"]), start=1):
+        assert first_diff(a_, b_) is None, f"U block {mb}"

@@ -381,3 +381,57 @@ def test_multigrid_scratch_arrays(tm):
                         continue
                     diff = first_diff(b.get(a, 0, nm, 1), getattr(rb, a))
                     assert diff is None, f"pass {k} level {nm} {a}: {diff}"
+
+
+@pytest.mark.parametrize("kind", ["euler_64rows", "sa_48rows", "multiblock", "sst_48rows"])
+def test_fused_lusgs_preparation(kind):
+    """k_lusgs_fused (the records made in the sweep's workgroup, for blocks of
+    at most 64 rows) against the oracle: bit-exact for Euler, 1e-11 with the
+    turbulence models; eager passes, then graph replays."""
+    if kind == "euler_64rows":
+        d, exact = naca_case("g_fused_e", cases.control_euler_naca(iters=10), nx=129, ny=65), True
+    elif kind == "sa_48rows":
+        d, exact = naca_case("g_fused_sa", cases.control_sa_naca(iters=10, Re=1e6, Ma=0.6), nx=129, ny=49,
+                             ds1=1e-5), False
+    elif kind == "sst_48rows":
+        c = _sst()
+        c.Twall = 300.0
+        d, exact = naca_case("g_fused_sst", c, nx=97, ny=49, ds1=1e-4), False
+    else:
+        d, exact = vortex_case("g_fused_mb", cases.control_vortex(iters=10), nbx=3, nby=3, wmin=9, wmax=41, hmin=9,
+                               hmax=65), True
+    case = solver.Case.from_dir(d)
+    ref = O.OracleCase(d)
+    ref.init_flow()
+    nb = len(case.levels[0])
+    with solver.Batch([case]) as b:
+        b.init_flow()
+        b.lusgs_shape(1, 1, 0)
+        for k in range(3):
+            ref.main_step()
+            b.main_step()
+            for mb in range(1, nb + 1):
+                g, r = b.get("U", 0, 1, mb), ref.block(1, mb).U
+                if exact:
+                    assert first_diff(g, r) is None, f"pass {k} block {mb}: {first_diff(g, r)}"
+                else:
+                    np.testing.assert_allclose(g, r, rtol=1e-11, atol=1e-300, err_msg=f"pass {k} block {mb}")
+        b.run(3)
+        for _ in range(3):
+            ref.main_step()
+        for mb in range(1, nb + 1):
+            g, r = b.get("U", 0, 1, mb), ref.block(1, mb).U
+            if exact:
+                assert first_diff(g, r) is None, f"graph, block {mb}: {first_diff(g, r)}"
+            else:
+                np.testing.assert_allclose(g, r, rtol=1e-11, atol=1e-300, err_msg=f"graph, block {mb}")
+        b.res_norms()  # reports a wait that ran out of its budget, if any
+
+
+def test_fused_lusgs_refuses_tall_blocks():
+    from ec2d import capi
+
+    d = naca_case("g_fused_tall", cases.control_euler_naca(iters=10), nx=129, ny=73)
+    with solver.Batch([solver.Case.from_dir(d)]) as b:
+        with pytest.raises(capi.Ec2dError, match="at most 64 rows"):
+            b.lusgs_shape(1, 1, 0)
