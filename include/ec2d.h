@@ -111,6 +111,17 @@ int ec2d_comput_residual_one_mesh(int nMesh);
 /* du_LU_SGS_2D (sub_LU_SGS.f90:4-108); mBlock = 0 sweeps every block. */
 int ec2d_du_lu_sgs_2d(int nMesh, int mBlock, double alfa1);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The same as ec2d_du_lu_sgs_2d, result for result, but
+ * the LU-SGS records (and the rings of split sweeps) live in a static device
+ * array of the library, at most 64 MiB per GPU (held until the process ends):
+ * no hipMalloc, hipFree or allocating routine is called.  Fails with a message
+ * when the level's records need more (a level of up to about 100,000 SA cells
+ * fits).  Every call on a GPU shares that GPU's array and clears it first, so
+ * calls that overlap on one GPU (several host threads, several streams) race. */
+int ec2d_du_lu_sgs_2d_noalloc(int nMesh, int mBlock, double alfa1);
+
 /* the update loop U = U + dU of NS_Time_advance_LU_SGS (sub_NS_singlegrid.f90:203-211) */
 int ec2d_update_u(int nMesh);
 

@@ -86,6 +86,11 @@ module ec2d_bind
       integer(c_int), value :: nMesh, mBlock
       real(c_double), value :: alfa1
     end function
+    integer(c_int) function ec2d_du_lu_sgs_2d_noalloc(nMesh, mBlock, alfa1) bind(C, name="ec2d_du_lu_sgs_2d_noalloc")
+      import
+      integer(c_int), value :: nMesh, mBlock
+      real(c_double), value :: alfa1
+    end function
     integer(c_int) function ec2d_update_u(nMesh) bind(C, name="ec2d_update_u")
       import
       integer(c_int), value :: nMesh

@@ -148,6 +148,17 @@ struct Runtime {
 
 Runtime R;
 
+// Scratch of ec2d_du_lu_sgs_2d_noalloc: the LU-SGS records (and split-sweep
+// rings) of one level in a device array that exists as soon as the library
+// is loaded on a device, so that the entry point allocates nothing.  Each GPU
+// holds its own copy until the process ends; host code uses only the address
+// hipGetSymbolAddress gave for the device (scratch_addr), never the array's
+// name, which in host code is a stand-in and not a device address.
+constexpr size_t SCRATCH_BYTES = (size_t)64 << 20;
+static __device__ __align__(256) unsigned long long scratch[SCRATCH_BYTES / 8];
+constexpr int SCRATCH_DEVICES = 64;
+void* scratch_addr[SCRATCH_DEVICES] = {nullptr};  // per device, set once by ec2d_init
+
 int fail(const std::string& m) {
   R.err = m;
   return 1;
@@ -1159,6 +1170,18 @@ int ec2d_init(void) {
   HIPCHK(hipGetDeviceCount(&n));
   if (n < 1) return fail("no HIP device");
   HIPCHK(hipSetDevice(0));
+  {
+    // the device address of the no-allocation scratch array on this device
+    int dev = -1;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= SCRATCH_DEVICES) return fail("device index out of range for the scratch table");
+    if (!scratch_addr[dev]) {
+      void* p = nullptr;
+      const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || !p) return fail(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+      scratch_addr[dev] = p;
+    }
+  }
   HIPCHK(hipStreamCreateWithFlags(&R.s, hipStreamNonBlocking));
   HIPCHK(hipEventCreate(&R.ev0));
   HIPCHK(hipEventCreate(&R.ev1));
@@ -1857,6 +1880,42 @@ int ec2d_du_lu_sgs_2d(int nMesh, int mBlock, double alfa1) {
   if (R.cases.size() != 1) return fail("a single block can only be selected in a one-case batch");
   if (mBlock < 1 || mBlock > (int)L.hb.size()) return fail("bad block");
   return launch_lusgs(nMesh - 1, L.dblocks_all + (mBlock - 1), 1, alfa1, 0);
+}
+
+int ec2d_du_lu_sgs_2d_noalloc(int nMesh, int mBlock, double alfa1) {
+  if (check_ready() || nMesh < 1 || nMesh > R.num_mesh) return fail("bad mesh level");
+  if (R.profile) return fail("ec2d_du_lu_sgs_2d_noalloc: profiling creates events; switch it off first");
+  int dev = -1;
+  HIPCHK(hipGetDevice(&dev));
+  void* base = (dev >= 0 && dev < SCRATCH_DEVICES) ? scratch_addr[dev] : nullptr;
+  if (!base) return fail("ec2d_du_lu_sgs_2d_noalloc: no scratch address for this device (ec2d_init sets it)");
+  DLevel& L = R.lv[nMesh - 1];
+  if (mBlock != 0) {
+    if (R.cases.size() != 1) return fail("a single block can only be selected in a one-case batch");
+    if (mBlock < 1 || mBlock > (int)L.hb.size()) return fail("bad block");
+  }
+  const size_t rec_bytes = ((size_t)L.sk_total * 8 + 255) / 256 * 256;
+  const int64_t nring = L.dgring ? (int64_t)L.hb.size() * (L.lusgs_split - 1) * 2 * L.gstride : 0;
+  const size_t need = rec_bytes + (size_t)nring * 8;
+  if (need > SCRATCH_BYTES) {
+    char msg[200];
+    std::snprintf(msg, sizeof(msg),
+                  "ec2d_du_lu_sgs_2d_noalloc: level %d needs %.1f MiB of LU-SGS records, more than the %zu MiB "
+                  "scratch (use ec2d_du_lu_sgs_2d)", nMesh, need / 1048576.0, SCRATCH_BYTES >> 20);
+    return fail(msg);
+  }
+  double* sk = static_cast<double*>(base);
+  unsigned long long* gr = nring ? reinterpret_cast<unsigned long long*>(static_cast<char*>(base) + rec_bytes) : nullptr;
+  // the array keeps the last call's contents: records start at zero (padding
+  // positions are read, never written), ring slots at the sentinel
+  hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((L.sk_total + 255) / 256)), dim3(256), 0, R.s,
+                     reinterpret_cast<unsigned long long*>(sk), L.sk_total, 0ull);
+  if (nring)
+    hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((nring + 255) / 256)), dim3(256), 0, R.s, gr, nring, GSENT);
+  HIPCHK(hipGetLastError());
+  if (flush_clip(nMesh - 1)) return 1;
+  if (mBlock == 0) return launch_lusgs(nMesh - 1, L.dblocks_all, (int)L.hb.size(), alfa1, 0, sk, gr);
+  return launch_lusgs(nMesh - 1, L.dblocks_all + (mBlock - 1), 1, alfa1, 0, sk, gr);
 }
 
 int ec2d_update_u(int nMesh) {

@@ -51,6 +51,7 @@ SIGNATURES = {
     "ec2d_init_flow": [],
     "ec2d_comput_residual_one_mesh": [_i],
     "ec2d_du_lu_sgs_2d": [_i, _i, C.c_double],
+    "ec2d_du_lu_sgs_2d_noalloc": [_i, _i, C.c_double],
     "ec2d_update_u": [_i],
     "ec2d_boundary_condition_onemesh": [_i],
     "ec2d_update_buffer_onemesh": [_i],

@@ -221,6 +221,11 @@ class Batch:
     def du_lu_sgs_2d(self, nmesh=1, mblock=0, alfa1=0.0):
         capi.call("ec2d_du_lu_sgs_2d", nmesh, mblock, alfa1)
 
+    def du_lu_sgs_2d_noalloc(self, nmesh=1, mblock=0, alfa1=0.0):
+        """du_LU_SGS_2D with the records in the library's static scratch array
+        (see ec2d_du_lu_sgs_2d_noalloc): nothing is allocated."""
+        capi.call("ec2d_du_lu_sgs_2d_noalloc", nmesh, mblock, alfa1)
+
     def update_u(self, nmesh=1):
         capi.call("ec2d_update_u", nmesh)
 
