@@ -327,7 +327,8 @@ def converge(args):
     out = {"metric": f"wall time to a {args.converge:g} density-residual drop, every case",
            "workload": desc["workload"], "cases": len(batch), "cells": cells,
            "converged": int(ok.sum()), "not_finite": int((st == solver.Batch.NOT_FINITE).sum()),
-           "pass_limit": int((st == solver.Batch.LIMIT).sum()), "max_iter": args.max_iter,
+           "pass_limit": int((st == solver.Batch.LIMIT).sum()), "cold_cell": int((st == solver.Batch.COLD).sum()),
+           "max_iter": args.max_iter,
            "wall_s": wall,
            "passes_converged": ({"min": int(it[ok].min()), "median": float(np.median(it[ok])), "max": int(it[ok].max())}
                                 if ok.any() else None),
@@ -355,6 +356,8 @@ def main():
     ap.add_argument("--tile-rows", type=int, default=4)
     ap.add_argument("--calibrate-copy", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--lusgs-shape", default="", metavar="LANES,WORKGROUPS",
+                    help="fine-level LU-SGS shape set by hand after autotuning (workgroups 0: the fused launch)")
     ap.add_argument("--converge", type=float, default=0.0, metavar="DROP",
                     help="instead of the throughput line: run every case of the workload to a density-residual drop "
                          "of DROP (e.g. 1e-8) and report the passes and wall time each case needed")
@@ -393,6 +396,10 @@ def main():
             # launch shapes per level (residual tile rows, LU-SGS lanes per
             # cell), timed on this case before the measured run
             tuning = b.autotune(args.tune_cache, reps=5) if not args.no_tune else "off"
+            if args.lusgs_shape:
+                lanes, wgs = (int(v) for v in args.lusgs_shape.split(","))
+                b.lusgs_shape(1, lanes, wgs)
+                tuning += f"\nlevel 1: LU-SGS shape set by hand: lanes per cell {lanes}, workgroups per block {wgs}"
             b.run(max(args.warmup, 1))
             b.synchronize()
             t0 = time.perf_counter()

@@ -95,13 +95,16 @@ int ec2d_define_block(int case_id, int nMesh, int mBlock, int nx, int ny, int ns
 int ec2d_commit(void);
 
 int ec2d_put_array(int case_id, int nMesh, int mBlock, int which, const double *src);
+/* Synchronises the stream; fails when an LU-SGS wait timed out (see
+ * ec2d_synchronize).  The arrays of a run stopped by T < 1e-5 stay readable. */
 int ec2d_get_array(int case_id, int nMesh, int mBlock, int which, double *dst);
 
 /* ---- the per-iteration path (acts on all cases of the batch) ------------ */
 
 /* Init_flow (sub_init.f90:383-495): freestream start with coarse-level
  * pre-iterations and prolongation (Iflag_init = 0), or ghost fill of a state
- * put with ec2d_put_array (Iflag_init = 1, as init_flow_read :499-560). */
+ * put with ec2d_put_array (Iflag_init = 1, as init_flow_read :499-560).
+ * Clears both device error words (see ec2d_synchronize). */
 int ec2d_init_flow(void);
 
 /* Comput_Residual_one_mesh (sub_Residual.f90:13-107): Res, dt, Lci..Lvj and
@@ -151,7 +154,8 @@ int ec2d_set_un(int nMesh);
 int ec2d_comput_new_u(int nMesh);
 
 /* n passes of the main-loop body (opencfd_ec2d_v1.5.4.f90:172-186) as one
- * HIP graph replayed n times; no host synchronisation inside. */
+ * HIP graph replayed n times; no host synchronisation inside, one at the end,
+ * where the device error words are checked as in ec2d_synchronize. */
 int ec2d_run(int n_iter);
 
 /* Runs main-loop passes (as ec2d_run, graph replays of check_every passes)
@@ -167,9 +171,10 @@ int ec2d_run(int n_iter);
  * (arrays of the batch's case count, each may be NULL): iters = residual
  * checks (the state has had iters - 1 updates in this call, and its Kstep and
  * tt count those), status, ms = host wall time from the call's start to the
- * check that saw it stop (resolution: check_every passes).  Returns 2 with
- * the reference's message if a cell's T fell below 1e-5 (sub_Residual.f90:
- * 143-146).  No reference counterpart: the reference runs a fixed
+ * check that saw it stop (resolution: check_every passes).  The reference's
+ * T < 1e-5 stop (sub_Residual.f90:143-146) ends only the case it occurs in:
+ * that case stops, frozen with the state of that residual, with status 4.
+ * No reference counterpart: the reference runs a fixed
  * t_end / dt_global passes (opencfd_ec2d_v1.5.4.f90:172) and prints the
  * residual; this is its per-case stop for ensembles.  With the environment
  * variable EC2D_PROGRESS=1 it prints a progress line to stderr about every
@@ -210,6 +215,11 @@ int ec2d_set_time(int case_id, int nMesh, int kstep, double tt);
 int ec2d_get_run_counters(long long *state);
 int ec2d_set_run_counters(const long long *state);
 
+/* Waits for the stream, then checks the two device error words, which stay
+ * set until ec2d_init_flow: an LU-SGS wait between waves or workgroups that
+ * ran out of its budget (status 1, results invalid; the split-sweep rings are
+ * reset) and the reference's T < 1e-5 stop (sub_Residual.f90:143-146,
+ * status 2 with the cell in ec2d_last_error). */
 int ec2d_synchronize(void);
 
 /* Cells of the batch, and of one case, on a mesh level (Num_Cell, sub_init.f90:134). */
@@ -285,13 +295,21 @@ int ec2d_kernel_time(int nMesh, int kind, double *ms, int *launches);
  * No reference counterpart: measurement support. */
 int ec2d_set_profile(int on);
 
+/* The budget of the LU-SGS sweeps' bounded waits between waves and between
+ * workgroups, in polls (default 2^20, about a second); past it the wave gives
+ * up and the device error word is set (see ec2d_synchronize).  With 0 every
+ * receiving wave of a split sweep gives up at its first slot: a host-forced
+ * way to exercise the error path.  No reference counterpart. */
+int ec2d_set_wait_budget(int polls);
+
 /* No reference counterpart: the LU-SGS launch shape of one level, chosen by
  * hand (ec2d_autotune chooses it by timing otherwise): lanes per cell 1 or 2,
  * workgroups per block 1, 2 or 4 (each block's rows split between that many
  * CUs, for levels of at most 256/split blocks taller than split/2 waves), or
  * 0 with one lane per cell: the fused launch, whose workgroups make each
  * block's records themselves (blocks of at most 64 rows of cells), so that
- * they never go through device memory.  Results do not depend on it. */
+ * they never go through device memory, with 7 producer waves beside the
+ * sweep wave (-3: with 3).  Results do not depend on it. */
 int ec2d_lusgs_shape(int nMesh, int lanes_per_cell, int workgroups_per_block);
 
 /* Autotune the launch shapes of every mesh level on the committed batch:

@@ -214,6 +214,10 @@ module ec2d_bind
       import
       integer(c_int), value :: on
     end function
+    integer(c_int) function ec2d_set_wait_budget(polls) bind(C, name="ec2d_set_wait_budget")
+      import
+      integer(c_int), value :: polls
+    end function
     integer(c_int) function ec2d_timing(run_ms, residual_ms, lusgs_ms, ghost_ms) bind(C, name="ec2d_timing")
       import
       real(c_double) :: run_ms, residual_ms, lusgs_ms, ghost_ms

@@ -81,6 +81,7 @@ struct DLevel {
   int lusgs_split = 1;        // LU-SGS sweep workgroups per block (2 or 4: a block's rows split between CUs)
   int lusgs_u = 2;            // LU-SGS sweep steps per register set (record prefetch depth), up to 4 waves
   bool lusgs_fused = false;   // LU-SGS records made inside the sweep workgroup (k_lusgs_fused)
+  int fused_np = 7;           // its producer waves (3 or 7)
   unsigned long long* dgring = nullptr;  // split sweeps: one ring per boundary between workgroups and direction
   int64_t gstride = 0;                   // slots of one ring
   int4* dwall = nullptr;      // wall faces (block, face, i, j), cases in order
@@ -104,6 +105,7 @@ struct Runtime {
   CaseParams* dcps = nullptr;
   unsigned long long* derr = nullptr;
   int* dsweep_err = nullptr;  // a split LU-SGS sweep ran out of its wait budget
+  int wait_budget = SPIN_MAX;  // polls of a bounded LU-SGS wait (ec2d_set_wait_budget)
   hipStream_t s = nullptr;
   std::string err;
   int num_mesh = 1;
@@ -122,6 +124,7 @@ struct Runtime {
   // run to a residual drop (ec2d_run_to_drop): per-case mask and state
   int* dactive = nullptr;
   ConvState* dconv = nullptr;
+  int* dcold = nullptr;  // per case: a cell fell below T = 1e-5 in a per-case run
   bool conv_on = false, conv_pending = false;
   double conv_drop = 0.0;
   int conv_max = 0;
@@ -422,6 +425,54 @@ int check_ready() {
   return 0;
 }
 
+// The device's two error words, read once the stream has drained.
+//   dsweep_err: an LU-SGS wait between waves or workgroups ran out of its
+//     budget; the results since are invalid.  The split-sweep rings are set
+//     back to the sentinel here (a receiver that gave up may have left values
+//     in them) and the word stays set until ec2d_init_flow: every later check
+//     fails.
+//   derr: the first cell with T < 1e-5 (sub_Residual.f90:143-146), status 2
+//     with the reference's message; also kept until ec2d_init_flow.
+// cold: also report derr (the caller may read the arrays of a stopped run).
+int device_errors(bool cold) {
+  unsigned long long e = ~0ull;
+  int se = 0;
+  HIPCHK(hipMemcpyAsync(&se, R.dsweep_err, sizeof(se), hipMemcpyDeviceToHost, R.s));
+  if (cold) HIPCHK(hipMemcpyAsync(&e, R.derr, sizeof(e), hipMemcpyDeviceToHost, R.s));
+  HIPCHK(hipStreamSynchronize(R.s));
+  if (se) {
+    for (auto& L : R.lv)
+      if (L.dgring) {
+        const int64_t n = (int64_t)L.hb.size() * (L.lusgs_split - 1) * 2 * L.gstride;
+        hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, R.s, L.dgring, n, GSENT);
+      }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(R.s));
+    return fail("an LU-SGS sweep timed out waiting for another wave or workgroup (results invalid until "
+                "ec2d_init_flow)");
+  }
+  if (e != ~0ull) {
+    int blk = (int)(e >> 40), j = (int)((e >> 20) & 0xFFFFF), i = (int)(e & 0xFFFFF);
+    char msg[160];
+    // the level is not encoded in the key: report the block of the finest level
+    int cs = (blk < (int)R.lv[0].blk_case.size()) ? R.lv[0].blk_case[blk] : -1;
+    int lb = (blk < (int)R.lv[0].blk_local.size()) ? R.lv[0].blk_local[blk] + 1 : -1;
+    std::snprintf(msg, sizeof(msg), "T <1.d-5 ! %d %d %d (case %d)", i, j, lb, cs);
+    R.err = msg;
+    return 2;
+  }
+  return 0;
+}
+
+// both error words back to "none" (a fresh start of the flow)
+int reset_device_errors() {
+  const int zero = 0;
+  const unsigned long long none = ~0ull;
+  HIPCHK(hipMemcpy(R.dsweep_err, &zero, sizeof(zero), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(R.derr, &none, sizeof(none), hipMemcpyHostToDevice));
+  return 0;
+}
+
 // ---------------------------------------------------------------- launches
 int launch_pw(int level, int op, int krk = 0);
 
@@ -479,6 +530,7 @@ int launch_residual(int level, int clip, int force = 0, int set_un = 0) {
   a.arena = L.arena;
   a.partials = L.dpartials;
   a.err = R.derr;
+  a.cold = R.conv_on ? R.dcold : nullptr;
   a.o = L.o;
   a.pm = L.pm;
   a.clip = clip;
@@ -612,6 +664,7 @@ int launch_lusgs(int level, const int* dlist, int nblk, double alfa1, int update
   a.gring = gring ? gring : L.dgring;
   a.gstride = L.gstride;
   a.err = R.dsweep_err;
+  a.spin_max = R.wait_budget;
   bool visc = R.cases[0].cp.If_viscous == 1;
   // prep and scatter grid: tiles of 32 steps x 8 rows of the largest block's skewed space
   dim3 gprep((unsigned)L.prep_tiles, nblk), gsweep(nblk * L.lusgs_split), blk(L.lusgs_threads);
@@ -628,7 +681,10 @@ int launch_lusgs(int level, const int* dlist, int nblk, double alfa1, int update
 #define LLAUNCH(NV, V)                                                                      \
   if (L.lusgs_fused) {                                                                      \
     prof_mark(5 + pk);                                                                      \
-    hipLaunchKernelGGL((k_lusgs_fused<NV, V>), dim3(nblk), dim3(64 * (1 + FPROD)), 0, R.s, a); \
+    if (L.fused_np == 3)                                                                    \
+      hipLaunchKernelGGL((k_lusgs_fused<NV, V, 3>), dim3(nblk), dim3(256), 0, R.s, a);      \
+    else                                                                                    \
+      hipLaunchKernelGGL((k_lusgs_fused<NV, V, 7>), dim3(nblk), dim3(512), 0, R.s, a);      \
     prof_mark(-(5 + pk));                                                                   \
   } else {                                                                                  \
   prof_mark(4 + pk);                                                                        \
@@ -791,7 +847,7 @@ int residual(int level, int force = 0, int set_un = 0) {
     R.conv_pending = false;
     DLevel& L = R.lv[0];
     hipLaunchKernelGGL(k_converge, dim3((int)R.cases.size()), dim3(256), 0, R.s, L.dpartials, L.dcase_tile0,
-                       L.dncell, L.o.nvar, R.dactive, R.dconv, R.conv_drop, R.conv_max);
+                       L.dncell, L.o.nvar, R.dactive, R.dconv, R.conv_drop, R.conv_max, R.dcold);
     rc = hipGetLastError() == hipSuccess ? 0 : fail("convergence check launch failed");
   }
   prof_mark(-(1 + 10 * level));
@@ -1158,8 +1214,9 @@ struct Lcg {
 };
 
 // autotune report text of an LU-SGS shape (0: the fused launch)
-std::string shape_text(int sp) {
-  return sp == 0 ? std::string("fused preparation and sweeps") : "workgroups per block " + std::to_string(sp);
+std::string shape_text(int sp, int u) {
+  if (sp == 0) return "fused preparation and sweeps, " + std::to_string(u == 3 ? 3 : 7) + " producer waves";
+  return "workgroups per block " + std::to_string(sp) + ", steps per set " + std::to_string(u);
 }
 
 extern "C" {
@@ -1207,6 +1264,7 @@ int ec2d_finalize(void) {
   hipFree(R.dsweep_err);
   hipFree(R.dactive);
   hipFree(R.dconv);
+  hipFree(R.dcold);
   for (auto e : R.pev) hipEventDestroy(e);
   hipEventDestroy(R.ev0);
   hipEventDestroy(R.ev1);
@@ -1801,12 +1859,17 @@ int ec2d_get_array(int case_id, int nMesh, int mBlock, int which, double* dst) {
     HIPCHK(hipStreamSynchronize(R.s));
     from_plane(bd, plane.data(), dst, kind, nc, m);
   }
-  return 0;
+  // the arrays of a run stopped by T < 1e-5 stay readable; a timed-out sweep fails
+  return device_errors(false);
 }
 
 int ec2d_init_flow(void) {
   if (check_ready()) return 1;
   HIPCHK(hipStreamSynchronize(R.s));  // the plane fills below copy on the null stream
+  // a fresh start: the rings go back to the sentinel if a sweep timed out,
+  // then both error words are cleared
+  if (device_errors(false) == 1) R.err.clear();
+  if (reset_device_errors()) return 1;
   const ec2d_control& c0 = R.cases[0].ctl;
   R.residual_calls = 0;
   R.dual_steps = 0;
@@ -2047,6 +2110,7 @@ int ec2d_run(int n_iter) {
   float ms = 0;
   HIPCHK(hipEventElapsedTime(&ms, R.ev0, R.ev1));
   R.last_run_ms = ms;
+  if (int rc = device_errors(true)) return rc;
   if (R.profile) {
     // mean launch time per (kernel, level); ec2d_timing reports the finest level
     double acc[40] = {0};
@@ -2077,6 +2141,13 @@ int ec2d_run(int n_iter) {
   return 0;
 }
 
+int ec2d_set_wait_budget(int polls) {
+  if (polls < 0) return fail("the wait budget is a number of polls >= 0");
+  if (polls == R.wait_budget) return 0;
+  R.wait_budget = polls;
+  return drop_graph();  // a captured graph holds the old budget in its launch arguments
+}
+
 int ec2d_set_profile(int on) {
   R.profile = on;
   return 0;
@@ -2092,7 +2163,9 @@ int ec2d_run_to_drop(double drop, int max_iter, int check_every, int* iters, int
   if (!R.dactive) {
     HIPCHK(hipMalloc(&R.dactive, sizeof(int) * nc));
     HIPCHK(hipMalloc(&R.dconv, sizeof(ConvState) * nc));
+    HIPCHK(hipMalloc(&R.dcold, sizeof(int) * nc));
   }
+  HIPCHK(hipMemset(R.dcold, 0, sizeof(int) * nc));
   std::vector<int> act(nc, 1);
   std::vector<ConvState> st(nc);
   std::memset(st.data(), 0, sizeof(ConvState) * nc);
@@ -2175,7 +2248,9 @@ int ec2d_lusgs_shape(int nMesh, int lanes_per_cell, int workgroups_per_block) {
   if (lanes_per_cell != 1 && lanes_per_cell != 2) return fail("lanes per cell must be 1 or 2");
   HIPCHK(hipStreamSynchronize(R.s));
   DLevel& L = R.lv[nMesh - 1];
-  const bool fused = workgroups_per_block == 0;
+  // 0: fused with 7 producer waves, -3: fused with 3
+  const bool fused = workgroups_per_block == 0 || workgroups_per_block == -3;
+  if (workgroups_per_block < 0 && !fused) return fail("workgroups per block 1, 2, 4, or 0 / -3 (fused)");
   if (fused) {
     if (lanes_per_cell != 1) return fail("the fused LU-SGS launch takes one lane per cell");
     for (auto& b : L.hb)
@@ -2184,6 +2259,7 @@ int ec2d_lusgs_shape(int nMesh, int lanes_per_cell, int workgroups_per_block) {
   if (lusgs_layout(L, lanes_per_cell, fused ? 1 : workgroups_per_block, true)) return 1;
   L.lusgs_u = 2;
   L.lusgs_fused = fused;
+  L.fused_np = workgroups_per_block == -3 ? 3 : 7;
   if (R.graph) {  // a captured graph holds the old launch shape
     HIPCHK(hipGraphExecDestroy(R.graph));
     R.graph = nullptr;
@@ -2219,7 +2295,7 @@ int ec2d_autotune(const char* cache_path, int reps, char* report, int report_len
     int best_tr = -1, best_lp = -1, best_sp = 1, best_u = 2;
     for (auto& e : cache)
       if (e.first == key) { best_tr = e.second.tr; best_lp = e.second.lp; best_sp = e.second.sp; best_u = e.second.u; }
-    char line[256];
+    char line[1024];
     if (best_tr < 0) {
       // the timed launches write the level's state (with a learned closure,
       // its under-relaxed mu_t carries over to the next residual): keep a copy
@@ -2240,6 +2316,12 @@ int ec2d_autotune(const char* cache_path, int reps, char* report, int report_len
       // residual; dU only is written)
       best = 1e30;
       L.lusgs_fused = false;
+      std::string cand;  // every timed LU-SGS shape, for the report
+      auto note = [&](const char* tag, double ms) {
+        char t[64];
+        std::snprintf(t, sizeof(t), " %s %.3f", tag, ms);
+        cand += t;
+      };
       for (int sp : {1, 2, 4})
         for (int lp : {1, 2}) {
           if (lusgs_layout(L, lp, sp, true)) {
@@ -2254,6 +2336,9 @@ int ec2d_autotune(const char* cache_path, int reps, char* report, int report_len
             double ms;
             if (time_launches(reps, [&] { return launch_lusgs(l, L.dblocks_all, (int)L.hb.size(), 0.0, 0); }, &ms))
               return 1;
+            char tag[32];
+            std::snprintf(tag, sizeof(tag), "w%dl%du%d", sp, lp, u);
+            note(tag, ms);
             if (ms < best) { best = ms; best_lp = lp; best_sp = sp; best_u = u; }
           }
           L.lusgs_u = 2;
@@ -2263,10 +2348,15 @@ int ec2d_autotune(const char* cache_path, int reps, char* report, int report_len
       for (auto& b : L.hb) fits = fits && b.ny - 1 <= 64;
       if (fits && !lusgs_layout(L, 1, 1, true)) {
         L.lusgs_fused = true;
-        double ms;
-        if (time_launches(reps, [&] { return launch_lusgs(l, L.dblocks_all, (int)L.hb.size(), 0.0, 0); }, &ms))
-          return 1;
-        if (ms < best) { best = ms; best_lp = 1; best_sp = 0; best_u = 2; }
+        for (int np : {3, 7}) {
+          L.fused_np = np;
+          double ms;
+          if (time_launches(reps, [&] { return launch_lusgs(l, L.dblocks_all, (int)L.hb.size(), 0.0, 0); }, &ms))
+            return 1;
+          note(np == 3 ? "fused3" : "fused7", ms);
+          // the cache's fourth field holds the producer waves of a fused choice
+          if (ms < best) { best = ms; best_lp = 1; best_sp = 0; best_u = np; }
+        }
         L.lusgs_fused = false;
       }
       R.err.clear();
@@ -2275,18 +2365,22 @@ int ec2d_autotune(const char* cache_path, int reps, char* report, int report_len
       HIPCHK(hipFree(keep));
       if (best_lp < 0) return fail("no LU-SGS layout fits this level");
       added.push_back({key, {best_tr, best_lp, best_sp, best_u}});
-      std::snprintf(line, sizeof(line), "level %d: measured tile_rows %d, lanes per cell %d, %s, steps per set %d\n",
-                    l + 1, best_tr, best_lp, shape_text(best_sp).c_str(), best_u);
+      std::snprintf(line, sizeof(line), "level %d: measured tile_rows %d, lanes per cell %d, %s;", l + 1, best_tr,
+                    best_lp, shape_text(best_sp, best_u).c_str());
+      rep_txt += line;
+      // LU-SGS times of every shape (w: workgroups per block, l: lanes per cell, u: steps per set)
+      std::snprintf(line, sizeof(line), " LU-SGS ms:%s\n", cand.c_str());
     } else {
-      std::snprintf(line, sizeof(line), "level %d: cached tile_rows %d, lanes per cell %d, %s, steps per set %d\n",
-                    l + 1, best_tr, best_lp, shape_text(best_sp).c_str(), best_u);
+      std::snprintf(line, sizeof(line), "level %d: cached tile_rows %d, lanes per cell %d, %s\n", l + 1, best_tr,
+                    best_lp, shape_text(best_sp, best_u).c_str());
     }
     rep_txt += line;
     L.tile_rows = best_tr;
     if (build_tiles(L)) return 1;
     if (lusgs_layout(L, best_lp, best_sp == 0 ? 1 : best_sp, true)) return 1;
-    L.lusgs_u = best_u;
     L.lusgs_fused = best_sp == 0;
+    L.lusgs_u = L.lusgs_fused ? 2 : best_u;
+    if (L.lusgs_fused) L.fused_np = best_u == 3 ? 3 : 7;
   }
   if (!added.empty() && cache_path && cache_path[0]) {
     if (FILE* f = std::fopen(cache_path, "a")) {
@@ -2598,7 +2692,7 @@ int ec2d_timing_level(int nMesh, double* residual_ms, double* lusgs_ms, double* 
 int ec2d_synchronize(void) {
   if (!R.inited) return 0;
   HIPCHK(hipStreamSynchronize(R.s));
-  return 0;
+  return R.committed ? device_errors(true) : 0;
 }
 
 int ec2d_res_norms(int case_id, int nMesh, double* res_max, double* res_rms) {
@@ -2608,27 +2702,15 @@ int ec2d_res_norms(int case_id, int nMesh, double* res_max, double* res_rms) {
   double buf[12];
   if (L.residual_done && reduce_norms(L)) return 1;
   HIPCHK(hipMemcpyAsync(buf, L.dnorms + 12 * case_id, sizeof(buf), hipMemcpyDeviceToHost, R.s));
-  unsigned long long e = 0;
-  HIPCHK(hipMemcpyAsync(&e, R.derr, sizeof(e), hipMemcpyDeviceToHost, R.s));
-  int se = 0;
-  HIPCHK(hipMemcpyAsync(&se, R.dsweep_err, sizeof(se), hipMemcpyDeviceToHost, R.s));
   HIPCHK(hipStreamSynchronize(R.s));
-  if (se) return fail("an LU-SGS sweep timed out waiting for another wave or workgroup (results invalid)");
+  int se = 0;
+  HIPCHK(hipMemcpy(&se, R.dsweep_err, sizeof(se), hipMemcpyDeviceToHost));
+  if (se) return device_errors(true);
   for (int m = 0; m < 6; m++) {
     res_max[m] = buf[m];
     res_rms[m] = buf[6 + m];
   }
-  if (e != ~0ull) {
-    int blk = (int)(e >> 40), j = (int)((e >> 20) & 0xFFFFF), i = (int)(e & 0xFFFFF);
-    char msg[160];
-    // the level is not encoded in the key: report the block of the finest level
-    int cs = (blk < (int)R.lv[0].blk_case.size()) ? R.lv[0].blk_case[blk] : -1;
-    int lb = (blk < (int)R.lv[0].blk_local.size()) ? R.lv[0].blk_local[blk] + 1 : -1;
-    std::snprintf(msg, sizeof(msg), "T <1.d-5 ! %d %d %d (case %d)", i, j, lb, cs);
-    R.err = msg;
-    return 2;
-  }
-  return 0;
+  return device_errors(true);
 }
 
 int ec2d_time(int case_id, int nMesh, int* kstep, double* tt) {

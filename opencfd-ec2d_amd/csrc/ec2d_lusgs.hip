@@ -57,6 +57,7 @@
 // the next use of a loaded register).
 #include "ec2d_device.h"
 #include "ec2d_physics.h"
+#include <type_traits>
 
 namespace ec2d {
 
@@ -74,6 +75,7 @@ struct LusgsArgs {
   unsigned long long* gring;  // sweep, split: one ring per boundary between workgroups and direction
   int64_t gstride;        // sweep, split: slots of one ring (at least npad * 2 * NV)
   int* err;               // sweep, split: set when a cross-workgroup wait ran out of its budget
+  int spin_max;           // the budget of a bounded wait, in polls
 };
 
 // Field layout of one record set (one sweep direction) of a block.  A
@@ -578,12 +580,13 @@ __device__ __forceinline__ void sweep(__amdgpu_buffer_rsrc_t rs, uint32_t rec, u
 // own 8-byte handshake, so no ordering between doubles is needed) and, when
 // its sweep is done, sets all its slots back to the sentinel for the next
 // launch.  The sentinel is a NaN payload that arithmetic never produces (it
-// yields the default NaN).  Global waits are bounded: past SPIN_MAX polls
+// yields the default NaN).  Global waits are bounded: past a.spin_max polls
 // the wave sets err and stops waiting for the rest of the sweep, so the
 // kernel always ends within one budget (its results are then wrong, and the
-// host reports err).
+// host reports err).  A budget of 0 makes every receiver give up at its
+// first slot (the host's way to test the error path).
 constexpr unsigned long long GSENT = 0x7FF4DEADBEEF0001ull;
-constexpr int SPIN_MAX = 1 << 20;
+constexpr int SPIN_MAX = 1 << 20;  // the default budget
 
 __device__ __forceinline__ unsigned long long gload(unsigned long long* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -625,7 +628,7 @@ __device__ void comm_forward(Chain<NV>& ch, int nw, unsigned long long* gr, int 
 
 // receiving side: take each batch of rows and feed ring[0]; then reset
 template <int NV>
-__device__ void comm_receive(Chain<NV>& ch, unsigned long long* gr, int npad, int* err) {
+__device__ void comm_receive(Chain<NV>& ch, unsigned long long* gr, int npad, int* err, int spin_max) {
   const CommLane<NV> c;
   static_assert(CommLane<NV>::CK <= RING, "a batch must fit the LDS ring");
   bool gone = false;  // the other workgroup stopped answering: wait no more
@@ -637,8 +640,8 @@ __device__ void comm_receive(Chain<NV>& ch, unsigned long long* gr, int npad, in
       unsigned long long* slot = gr + (s0 + c.kk) * (2 * NV) + c.f;
       unsigned long long x;
       int n = 0;
-      while ((x = gload(slot)) == GSENT && ++n < SPIN_MAX) __builtin_amdgcn_s_sleep(1);
-      if (n >= SPIN_MAX) {
+      while ((x = gload(slot)) == GSENT && ++n < spin_max) __builtin_amdgcn_s_sleep(1);
+      if (n >= spin_max) {
         atomicOr(err, 1);
         gone = true;
       }
@@ -693,7 +696,7 @@ __global__ __launch_bounds__(MAXT) void k_lusgs_sweep(LusgsArgs a) {
   const bool rcv = has_in && wave == nw, snd = has_out && wave == nw + (has_in ? 1 : 0);
   if (a.dir != 2) {
     if (wave < nw) sweep<NV, VISC, true, U, LPC>(rs, fw, bw, n16, S, R, ncol, nrow, t, k, lane, wave, cin, cout, g, ch[0]);
-    else if (rcv) comm_receive<NV>(ch[0], gr(half - 1, 0), npad, a.err);
+    else if (rcv) comm_receive<NV>(ch[0], gr(half - 1, 0), npad, a.err, a.spin_max);
     else if (snd) comm_forward<NV>(ch[0], nw, gr(half, 0), npad);
   }
   if (a.dir == 0) {
@@ -702,7 +705,7 @@ __global__ __launch_bounds__(MAXT) void k_lusgs_sweep(LusgsArgs a) {
   }
   if (a.dir != 1) {
     if (wave < nw) sweep<NV, VISC, false, U, LPC>(rs, bw, out, n16, S, R, ncol, nrow, t, k, lane, wave, cin, cout, g, ch[1]);
-    else if (rcv) comm_receive<NV>(ch[1], gr(half - 1, 1), npad, a.err);
+    else if (rcv) comm_receive<NV>(ch[1], gr(half - 1, 1), npad, a.err, a.spin_max);
     else if (snd) comm_forward<NV>(ch[1], nw, gr(half, 1), npad);
   }
 }
@@ -711,38 +714,38 @@ __global__ __launch_bounds__(MAXT) void k_lusgs_sweep(LusgsArgs a) {
 // For levels whose blocks have at most 64 rows (one wave per block, one lane
 // per row): k_lusgs_fused does the work of k_lusgs_prep inside the sweep's
 // workgroup, so the records never go through HBM.  Wave 0 runs both sweeps
-// exactly as k_lusgs_sweep<..., LPC = 1> does (the same sweep_step); waves
-// 1..FPROD produce the records, each every FPROD-th step of the sequence
-// (forward steps 0..npad-1, then backward steps 0..npad-1), lane t the cell
-// of row t, straight from the block planes (make_rec, the arithmetic of
-// k_lusgs_prep) into a ring of FRING LDS slots.  The forward dU still goes to
-// the backward records' X pairs in global memory (the backward sweep reads it
-// there, after a fence), and the backward dU to the output pairs that
-// k_lusgs_scatter reads.
+// exactly as k_lusgs_sweep<..., LPC = 1> does (the same sweep_step), at the
+// highest issue priority of its SIMD; waves 1..NP produce the records, each
+// every NP-th step of the sequence (forward steps 0..npad-1, then backward
+// steps 0..npad-1), lane t the cell of row t, straight from the block planes
+// (make_rec, the arithmetic of k_lusgs_prep) into a ring of LDS slots
+// (6 for NP = 3, NP + 1 for NP = 7).  The forward dU still goes to the backward records' X pairs in
+// global memory (the backward sweep reads it there, after a fence), and the
+// backward dU to the output pairs that k_lusgs_scatter reads.  NP = 3 or 7
+// (4 or 8 waves; with 7 the ring of a six-equation record takes 156 KB of
+// the CU's 160 KB).
 //
 // Protocol: produced[p] counts the steps producer p has written, consumed
 // the steps the sweep wave has read.  The sweep wave waits, before step g,
-// for produced[g % FPROD] > g / FPROD; producer p waits, before writing
-// step g into slot g % FRING, for consumed >= g - FRING + 1.  A producer
-// waits only for the sweep wave and the sweep wave only for producers that
-// are never more than FRING steps ahead, so every wait ends; waits are also
-// bounded (SPIN_MAX polls, then err is set and the wave stops waiting).
-constexpr int FRING = 6;  // LDS record slots
-constexpr int FPROD = 3;  // producer waves
-
-template <int NV, bool VISC>
+// for produced[g % NP] > g / NP; producer p waits, before writing step g
+// into slot g % NRING, for consumed >= g - NRING + 1.  A producer waits only
+// for the sweep wave and the sweep wave only for producers that are never
+// more than NRING steps ahead, so every wait ends; waits are also bounded
+// (a.spin_max polls, then err is set and the wave stops waiting).
+template <int NV, bool VISC, int NP>
 struct FusedLds {
   using F = RecF<NV, VISC>;
-  double sh[FRING][F::NSH][64];     // shared doubles of slot, lane
-  double sd[FRING][2][F::NSD][64];  // side doubles
-  int produced[FPROD];
+  static constexpr int NRING = NP == 3 ? 6 : NP + 1;  // LDS record slots
+  double sh[NRING][F::NSH][64];     // shared doubles of slot, lane
+  double sd[NRING][2][F::NSD][64];  // side doubles
+  int produced[NP];
   int consumed;
 };
 
-__device__ __forceinline__ bool wait_bounded(const int* p, int v, int* err) {
+__device__ __forceinline__ bool wait_bounded(const int* p, int v, int* err, int spin_max) {
   int n = 0;
   while (lds_get(p) < v) {
-    if (++n >= SPIN_MAX) {
+    if (++n >= spin_max) {
       atomicOr(err, 2);
       return false;
     }
@@ -752,10 +755,12 @@ __device__ __forceinline__ bool wait_bounded(const int* p, int v, int* err) {
   return true;
 }
 
-template <int NV, bool VISC>
-__global__ __launch_bounds__(64 * (1 + FPROD)) void k_lusgs_fused(LusgsArgs a) {
+template <int NV, bool VISC, int NP>
+__global__ __launch_bounds__(64 * (1 + NP)) void k_lusgs_fused(LusgsArgs a) {
   using F = RecF<NV, VISC>;
-  __shared__ FusedLds<NV, VISC> L;
+  using LDS = FusedLds<NV, VISC, NP>;
+  constexpr int FPROD = NP, FRING = LDS::NRING;
+  __shared__ LDS L;
   const BlockDesc B = a.blocks[a.block_list[blockIdx.x]];
   if (case_frozen(B.case_id)) return;
   const double g = a.cps[B.case_id].gamma;
@@ -780,7 +785,7 @@ __global__ __launch_bounds__(64 * (1 + FPROD)) void k_lusgs_fused(LusgsArgs a) {
     const double* base = a.arena + B.base;
     const int p = wave - 1;
     for (int gs = p, k = 0; gs < ntot && ok; gs += FPROD, k++) {
-      ok = wait_bounded(&L.consumed, gs - FRING + 1, a.err);
+      ok = wait_bounded(&L.consumed, gs - FRING + 1, a.err, a.spin_max);
       const int slot = gs % FRING;
       const bool fwd = gs < npad;
       const int st = fwd ? gs : gs - npad;
@@ -794,7 +799,8 @@ __global__ __launch_bounds__(64 * (1 + FPROD)) void k_lusgs_fused(LusgsArgs a) {
     }
     return;
   }
-  // ---- the sweep wave: lane t = row t
+  // ---- the sweep wave: lane t = row t; it issues first on its SIMD
+  __builtin_amdgcn_s_setprio(3);
   const int t = lane;
   const int tl = min(t, nrow - 1);
   Chain<NV>& ch = *reinterpret_cast<Chain<NV>*>(&L);  // no other compute wave: never touched
@@ -835,37 +841,42 @@ __global__ __launch_bounds__(64 * (1 + FPROD)) void k_lusgs_fused(LusgsArgs a) {
       if (2 * j + 1 < NV) x[2 * j + 1] = v.y;
     }
   };
-  for (int dir = 0; dir < 2; dir++) {
-    const bool fwd = dir == 0;
+  // one direction of the sweep; the direction is a compile-time constant so
+  // that every register array keeps fixed indices
+  auto run_dir = [&](auto fwd_c) {
+    constexpr bool fwd = decltype(fwd_c)::value;
+    constexpr int dir = fwd ? 0 : 1;
     Ahead<NV> ah = {};
     Out<NV> own = {};
-    double xa[2 * F::XP], xb[2 * F::XP];
+    // X of this step and of the next (loaded a step ahead); register arrays
+    // with fixed indices only
+    double xc[2 * F::XP] = {}, xn[2 * F::XP] = {};
     if (!fwd) {
       // the forward sweep's stores to the X pairs before these loads
       __threadfence();
-      load_x(xa, 0);
+      load_x(xc, 0);
     }
     for (int st = 0; st < npad; st++) {
       const int gs = dir * npad + st;
-      if (!fwd && st + 1 < npad) load_x((st & 1) ? xa : xb, st + 1);
-      if (ok) ok = wait_bounded(&L.produced[gs % FPROD], gs / FPROD + 1, a.err);
+      if (!fwd && st + 1 < npad) load_x(xn, st + 1);
+      if (ok) ok = wait_bounded(&L.produced[gs % FPROD], gs / FPROD + 1, a.err, a.spin_max);
       Rec<NV, 2> r;
       load_lds(r, gs % FRING, fwd);
       if (!fwd) {
-        const double* x = (st & 1) ? xb : xa;
 #pragma unroll
-        for (int m = 0; m < NV; m++) r.X[m] = x[m];
+        for (int m = 0; m < NV; m++) r.X[m] = xc[m];
+#pragma unroll
+        for (int m = 0; m < 2 * F::XP; m++) xc[m] = xn[m];
       }
       __atomic_signal_fence(__ATOMIC_SEQ_CST);
       __builtin_amdgcn_s_waitcnt(0xc07f);  // the slot's LDS reads done before it is released
       lds_put(&L.consumed, gs + 1);
-      if (fwd)
-        sweep_step<NV, VISC, true, 1>(r, own, rs, bw, n16, st, S, R, ncol, nrow, t, 0, lane, 0, false, false, g, ch, ah);
-      else
-        sweep_step<NV, VISC, false, 1>(r, own, rs, out, n16, st, S, R, ncol, nrow, t, 0, lane, 0, false, false, g, ch,
-                                       ah);
+      sweep_step<NV, VISC, fwd, 1>(r, own, rs, fwd ? bw : out, n16, st, S, R, ncol, nrow, t, 0, lane, 0, false, false,
+                                   g, ch, ah);
     }
-  }
+  };
+  run_dir(std::true_type{});
+  run_dir(std::false_type{});
 }
 
 // ----------------------------------------------------------------- scatter

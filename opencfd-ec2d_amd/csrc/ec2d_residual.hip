@@ -29,6 +29,7 @@ struct ResArgs {
   double* arena;
   double* partials;           // [logical tile][2*NV]: max |Res|, sum Res^2
   unsigned long long* err;    // first cell with T < 1e-5 (block<<40 | j<<20 | i), ~0 if none
+  int* cold;                  // per case, in per-case runs: set for a cell with T < 1e-5 (in place of err)
   MeshOpts o;
   PlaneMap pm;
   int clip;                   // apply force_vt_kw to interior nu_t on load
@@ -86,8 +87,12 @@ __global__ __launch_bounds__(TX* TY) void k_residual(ResArgs a) {
     sp[lj][li] = cp.p00 * d * T;
     bool interior = (i >= 1 && i <= nx - 1 && j >= 1 && j <= ny - 1);
     if (interior && T < 1.0e-5) {
-      unsigned long long key = ((unsigned long long)t.block << 40) | ((unsigned long long)j << 20) | (unsigned)i;
-      atomicMin(a.err, key);
+      if (a.cold) {
+        atomicOr(a.cold + B.case_id, 1);
+      } else {
+        unsigned long long key = ((unsigned long long)t.block << 40) | ((unsigned long long)j << 20) | (unsigned)i;
+        atomicMin(a.err, key);
+      }
     }
     if constexpr (VISC) {
       sT[lj][li] = T;
@@ -261,7 +266,10 @@ __global__ __launch_bounds__(TX* TY) void k_residual(ResArgs a) {
     sFi[3][fj][fi] = F4;
   }
   // ---- 2b. j-faces ---------------------------------------------------------
-  for (int f = tid; f < NFJ; f += NT) {
+  // NFJ = NT + 64: the 64 faces beyond one per thread go to the last wave
+  // (the first wave already takes the TY i-faces beyond one per thread), so
+  // that no wave evaluates more than three faces
+  for (int f = (tid + 64) % NT; f < NFJ; f += NT) {
     int fi = f % TX, fj = f / TX;
     int i = i0 + fi, j = j0 + fj;
     if (i > nx - 1 || j > ny) continue;
@@ -592,7 +600,8 @@ struct ConvState {
 };
 
 __global__ __launch_bounds__(256) void k_converge(const double* partials, const int* case_tile0, const double* ncell,
-                                                  int nv, int* active, ConvState* st, double drop, int max_iter) {
+                                                  int nv, int* active, ConvState* st, double drop, int max_iter,
+                                                  const int* cold) {
   const int c = blockIdx.x;
   if (active[c] == 0) return;
   const int t0 = case_tile0[c], t1 = case_tile0[c + 1];
@@ -611,7 +620,8 @@ __global__ __launch_bounds__(256) void k_converge(const double* partials, const 
     s.iters += 1;
     if (s.iters == 1) s.r0 = rms;
     s.last = rms;
-    if (!isfinite(rms)) s.status = 2;
+    if (cold[c]) s.status = 4;  // the reference's T < 1e-5 stop, for this case only
+    else if (!isfinite(rms)) s.status = 2;
     else if (rms <= drop * s.r0) s.status = 1;
     else if (s.iters >= max_iter) s.status = 3;
     if (s.status != 0) active[c] = 0;

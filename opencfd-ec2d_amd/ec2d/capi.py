@@ -77,6 +77,7 @@ SIGNATURES = {
     "ec2d_num_cells": [_i, _i],
     "ec2d_timing": [_dp, _dp, _dp, _dp],
     "ec2d_set_profile": [_i],
+    "ec2d_set_wait_budget": [_i],
     "ec2d_timing_level": [_i, _dp, _dp, _dp, _ip],
     "ec2d_kernel_time": [_i, _i, _dp, _ip],
     "ec2d_force_coefficients": [_i, _dp, _dp, _dp],

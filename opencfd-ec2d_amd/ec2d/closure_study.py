@@ -12,7 +12,8 @@
    (four equations, turbulence model 4).  SA judges the closure's lift and
    drag.
 3. Per-case fallback: a closure case falls back to SA when its residual does
-   not reach the drop (it stalls: the pass limit, or a non-finite residual)
+   not reach the drop (it stalls: the pass limit, a non-finite residual, or
+   a cell below T = 1e-5, the reference's stop, which ends that case only)
    or when more than `range_frac` of its cells have features outside the
    training range (the model then extrapolates); a fallen-back case costs its
    closure time plus its SA time and reports SA's coefficients.
@@ -84,7 +85,8 @@ def main(argv=None):
     ap.add_argument("--seed-test", type=int, default=1)
     ap.add_argument("--test-grid", default="513x129")
     ap.add_argument("--drop", type=float, default=1e-8)
-    ap.add_argument("--max-iter", type=int, default=60000)
+    ap.add_argument("--max-iter", type=int, default=60000, help="pass limit of the training ensemble")
+    ap.add_argument("--max-iter-test", type=int, default=100000, help="pass limit of the held-out runs")
     ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--relax", type=float, default=1.0, help="under-relaxation of the closure's mu_t")
@@ -95,7 +97,8 @@ def main(argv=None):
     nx, ny = (int(v) for v in args.test_grid.split("x"))
     train = cases.ensemble_params(seed=args.seed_train, n=args.n_train)
     test = cases.ensemble_params(seed=args.seed_test, n=args.n_test)
-    out = {"drop": args.drop, "max_iter": args.max_iter, "relax": args.relax, "train_cases": len(train),
+    out = {"drop": args.drop, "max_iter": args.max_iter, "max_iter_test": args.max_iter_test, "relax": args.relax,
+           "train_cases": len(train),
            "test_cases": len(test), "test_grid": args.test_grid}
     with tempfile.TemporaryDirectory() as d:
         # 1. training fields and training
@@ -111,18 +114,26 @@ def main(argv=None):
                   f"in {t_sa:.1f} s", flush=True)
             n = b.closure_collect()
             t0 = time.perf_counter()
-            first, last = b.closure_train(epochs=args.epochs, batch=1024, lr=args.lr, seed=1)
+            # one call per epoch (Adam's moments and step count carry over), a
+            # progress line each
+            curve = []
+            for e in range(args.epochs):
+                f, last = b.closure_train(epochs=1, batch=1024, lr=args.lr, seed=1 + e)
+                curve.append(last)
+                first = curve[0] if e else f
+                print(f"epoch {e + 1}/{args.epochs}: mean loss {last:.5g}", flush=True)
             t_tr = time.perf_counter() - t0
             theta, norm = b.closure_params()
-        out["training"] = {"samples": n, "first_epoch_loss": first, "last_epoch_loss": last, "train_seconds": t_tr,
+        out["training"] = {"samples": n, "first_epoch_loss": first, "last_epoch_loss": last, "epoch_losses": curve,
+                           "train_seconds": t_tr,
                            "ensemble_seconds": t_sa, "ensemble_converged": int(sum(s == 1 for s in st)),
                            "ensemble_passes_median": float(np.median(it))}
         print(f"trained on {n} cells in {t_tr:.1f} s: loss {first:.4g} -> {last:.4g}", flush=True)
         # 2. held-out cases, SA and closure
         sa_dirs = [_write(os.path.join(d, f"sa{k}"), p, F.TURB_SA, nx, ny) for k, p in enumerate(test)]
         cl_dirs = [_write(os.path.join(d, f"cl{k}"), p, F.TURB_CLOSURE, nx, ny) for k, p in enumerate(test)]
-        sa = _solve(sa_dirs, args.drop, args.max_iter, "held-out SA")
-        cl = _solve(cl_dirs, args.drop, args.max_iter, "held-out closure", theta, norm, args.relax)
+        sa = _solve(sa_dirs, args.drop, args.max_iter_test, "held-out SA")
+        cl = _solve(cl_dirs, args.drop, args.max_iter_test, "held-out closure", theta, norm, args.relax)
     per = []
     for k in range(len(test)):
         fell = (cl["status"][k] != solver.Batch.CONVERGED) or (cl["out_of_range"][k] > args.range_frac)

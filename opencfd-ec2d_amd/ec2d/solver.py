@@ -254,13 +254,14 @@ class Batch:
     def run(self, n_iter: int):
         capi.call("ec2d_run", int(n_iter))
 
-    CONVERGED, NOT_FINITE, LIMIT = 1, 2, 3
+    CONVERGED, NOT_FINITE, LIMIT, COLD = 1, 2, 3, 4
 
     def run_to_drop(self, drop: float = 1e-8, max_iter: int = 100000, check_every: int = 100):
         """Runs every case of the batch until its density Res_rms has dropped
         by `drop` from its value at the first pass (see ec2d_run_to_drop).
         Returns per case (iters, status, ms): residual checks (the state has
-        had iters - 1 updates), status CONVERGED / NOT_FINITE / LIMIT, host
+        had iters - 1 updates), status CONVERGED / NOT_FINITE / LIMIT / COLD
+        (a cell's T fell below 1e-5: the reference's stop, for that case), host
         wall time in ms from the call's start to the check that saw it stop."""
         n = len(self.cases)
         it = (C.c_int * n)()
@@ -406,6 +407,13 @@ class Batch:
     def set_profile(self, on: bool):
         capi.call("ec2d_set_profile", 1 if on else 0)
 
+    WAIT_BUDGET_DEFAULT = 1 << 20
+
+    def set_wait_budget(self, polls: int):
+        """Polls of a bounded LU-SGS wait (see ec2d_set_wait_budget); 0 forces
+        every split-sweep receiver to give up."""
+        capi.call("ec2d_set_wait_budget", polls)
+
     def timing(self):
         v = [C.c_double() for _ in range(4)]
         capi.call("ec2d_timing", *[C.byref(x) for x in v])
@@ -432,8 +440,8 @@ class Batch:
     def autotune(self, cache_path: str = "", reps: int = 5) -> str:
         """Time and pick residual tile rows and LU-SGS lanes per cell per level
         (see ec2d_autotune); returns the per-level report."""
-        buf = C.create_string_buffer(1024)
-        capi.call("ec2d_autotune", cache_path.encode(), reps, buf, 1024)
+        buf = C.create_string_buffer(8192)
+        capi.call("ec2d_autotune", cache_path.encode(), reps, buf, 8192)
         return buf.value.decode()
 
     # ---- learned closure (SA cases; see ec2d_closure.hip) -------------------

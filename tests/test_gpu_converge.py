@@ -120,3 +120,30 @@ def test_cold_cell_reports_reference_cell():
     assert "T <1.d-5 ! 7 4 2" in str(e.value), str(e.value)
     assert "status 2" in str(e.value)
     assert str(eo.value).startswith("T <1.d-5 ! 7 4 2"), str(eo.value)
+
+
+def test_run_to_drop_cold_cell_stops_only_its_case():
+    """A cell with T < 1e-5 in case 1 (internal energy 0): in a per-case run
+    the reference's stop ends that case alone (status 4 at the first check,
+    state untouched) while cases 0 and 2 run on and equal their oracle runs;
+    a plain pass afterwards, outside the per-case mode, stops as the
+    reference does (status 2, the cell in the message)."""
+    from ec2d import capi
+
+    ds = [naca_case(f"g_cold_{k}", _ud1(Ma, aoa)) for k, (Ma, aoa) in enumerate([(0.5, 1.25), (0.4, 2.0), (0.3, 3.0)])]
+    batch = [solver.Case.from_dir(d) for d in ds]
+    with solver.Batch(batch) as b:
+        b.init_flow()
+        U1 = b.get("U", 1)
+        q = U1[5, 9]
+        q[3] = 0.5 * (q[1] ** 2 + q[2] ** 2) / q[0]
+        b.put("U", U1, 1)
+        iters, status, ms = b.run_to_drop(1e-12, max_iter=200, check_every=50)
+        assert status[1] == solver.Batch.COLD and iters[1] == 1, (iters, status)
+        assert status[0] == solver.Batch.LIMIT and status[2] == solver.Batch.LIMIT, (iters, status)
+        assert first_diff(b.get("U", 1), U1) is None, "the cold case must stay frozen"
+        for k in (0, 2):
+            oc, _ = _oracle_history(ds[k], iters[k])
+            assert first_diff(b.get("U", k), oc.block().U) is None, k
+        with pytest.raises(capi.Ec2dError, match=r"T <1.d-5 ! 8 4 1 \(case 1\)"):
+            b.run(1)

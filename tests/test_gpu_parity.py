@@ -383,11 +383,13 @@ def test_multigrid_scratch_arrays(tm):
                     assert diff is None, f"pass {k} level {nm} {a}: {diff}"
 
 
-@pytest.mark.parametrize("kind", ["euler_64rows", "sa_48rows", "multiblock", "sst_48rows"])
-def test_fused_lusgs_preparation(kind):
+@pytest.mark.parametrize("kind,wg", [("euler_64rows", 0), ("sa_48rows", 0), ("multiblock", 0), ("sst_48rows", 0),
+                                     ("euler_64rows", -3), ("sa_48rows", -3), ("sst_48rows", -3)])
+def test_fused_lusgs_preparation(kind, wg):
     """k_lusgs_fused (the records made in the sweep's workgroup, for blocks of
-    at most 64 rows) against the oracle: bit-exact for Euler, 1e-11 with the
-    turbulence models; eager passes, then graph replays."""
+    at most 64 rows; 7 producer waves, or 3 with wg = -3) against the oracle:
+    bit-exact for Euler, 1e-11 with the turbulence models; eager passes, then
+    graph replays."""
     if kind == "euler_64rows":
         d, exact = naca_case("g_fused_e", cases.control_euler_naca(iters=10), nx=129, ny=65), True
     elif kind == "sa_48rows":
@@ -406,7 +408,7 @@ def test_fused_lusgs_preparation(kind):
     nb = len(case.levels[0])
     with solver.Batch([case]) as b:
         b.init_flow()
-        b.lusgs_shape(1, 1, 0)
+        b.lusgs_shape(1, 1, wg)
         for k in range(3):
             ref.main_step()
             b.main_step()
@@ -435,3 +437,36 @@ def test_fused_lusgs_refuses_tall_blocks():
     with solver.Batch([solver.Case.from_dir(d)]) as b:
         with pytest.raises(capi.Ec2dError, match="at most 64 rows"):
             b.lusgs_shape(1, 1, 0)
+
+
+def test_split_sweep_timeout_is_reported_and_cleared():
+    """The error path of a split sweep's bounded waits, forced from the host:
+    with a wait budget of 0 every receiving wave gives up at its first slot
+    (the kernels still end normally).  ec2d_run then fails with the timeout,
+    ec2d_get_array refuses the invalid arrays, and the error stays set; after
+    ec2d_init_flow (rings back to the sentinel, error words cleared) and the
+    default budget, the run is bit-identical to the oracle again."""
+    from ec2d import capi
+
+    c = cases.control_euler_naca(iters=10)
+    c.Time_Method, c.CFL = F.TIME_LU_SGS, 2.0
+    d = naca_case("g_split_timeout", c, nx=129, ny=97)
+    ref = O.OracleCase(d)
+    ref.init_flow()
+    with solver.Batch([solver.Case.from_dir(d)]) as b:
+        b.init_flow()
+        b.lusgs_shape(1, 1, 2)
+        b.set_wait_budget(0)
+        with pytest.raises(capi.Ec2dError, match="timed out"):
+            b.run(2)
+        with pytest.raises(capi.Ec2dError, match="timed out"):
+            b.get("U")
+        with pytest.raises(capi.Ec2dError, match="timed out"):
+            b.synchronize()
+        b.set_wait_budget(solver.Batch.WAIT_BUDGET_DEFAULT)
+        b.init_flow()
+        b.run(3)
+        for _ in range(3):
+            ref.main_step()
+        assert first_diff(b.get("U"), ref.block().U) is None
+        b.res_norms()
