@@ -264,8 +264,9 @@ int ec2d_closure_set_params(const float *theta, const float *norm);
 int ec2d_closure_infer(void);             /* fills EC2D_CLOSURE_MUT / _MASK from the current state */
 int ec2d_closure_enable(int on);          /* use the closure in every following residual */
 /* Model-4 cases: under-relax the network's mu_t from one residual to the next,
- * mu_t <- mu_t + w (mu_t(network) - mu_t), w in (0, 1] (1, the default: none;
- * the plane starts at 0, so mu_t ramps up from laminar). */
+ * mu_t <- mu_t + w (mu_t(network) - mu_t), w in [0, 1] (1, the default: none;
+ * the plane starts at 0, so mu_t ramps up from laminar; 0: mu_t frozen at its
+ * current values, the network is not evaluated). */
 int ec2d_closure_relaxation(double w);
 /* Fallback on a stall: if the largest density Res_rms of the batch exceeds
  * growth times the smallest seen since enabling, switch the closure off
@@ -308,8 +309,9 @@ int ec2d_set_wait_budget(int polls);
  * CUs, for levels of at most 256/split blocks taller than split/2 waves), or
  * 0 with one lane per cell: the fused launch, whose workgroups make each
  * block's records themselves (blocks of at most 64 rows of cells), so that
- * they never go through device memory, with 7 producer waves beside the
- * sweep wave (-3: with 3).  Results do not depend on it. */
+ * they never go through device memory, with 6 producer waves and a writer
+ * wave beside the sweep wave (-3: with 3 producers).  Results do not depend
+ * on it. */
 int ec2d_lusgs_shape(int nMesh, int lanes_per_cell, int workgroups_per_block);
 
 /* Autotune the launch shapes of every mesh level on the committed batch:

@@ -81,7 +81,8 @@ struct DLevel {
   int lusgs_split = 1;        // LU-SGS sweep workgroups per block (2 or 4: a block's rows split between CUs)
   int lusgs_u = 2;            // LU-SGS sweep steps per register set (record prefetch depth), up to 4 waves
   bool lusgs_fused = false;   // LU-SGS records made inside the sweep workgroup (k_lusgs_fused)
-  int fused_np = 7;           // its producer waves (3 or 7)
+  bool skew_static = false;   // the record arena holds the skewed copy of the mesh planes
+  int fused_np = 6;           // its producer waves (3 or 6)
   unsigned long long* dgring = nullptr;  // split sweeps: one ring per boundary between workgroups and direction
   int64_t gstride = 0;                   // slots of one ring
   int4* dwall = nullptr;      // wall faces (block, face, i, j), cases in order
@@ -245,6 +246,7 @@ void ghost_phases(const BlockDesc& B, SubfaceDesc* s, int* units) {
 // LU-SGS record layout of a level for lpc lanes per cell (see ec2d_lusgs.hip);
 // upload: also refresh the device block table (after ec2d_commit).
 int lusgs_layout(DLevel& L, int lpc, int split, bool upload) {
+  L.skew_static = false;  // the arena is cleared below
   int max_rows = 1;
   for (auto& d : L.hb) max_rows = std::max(max_rows, d.ny - 1);
   const int rq = 64 / lpc;  // rows per wave
@@ -559,7 +561,8 @@ int launch_residual(int level, int clip, int force = 0, int set_un = 0) {
     }
     HIPCHK(hipGetLastError());
   }
-  if ((a.closure == 1 || a.closure == 2) && closure_infer()) return 1;
+  // model 4 with relaxation 0: the closure's mu_t stays frozen, the network is not evaluated
+  if ((a.closure == 1 || (a.closure == 2 && R.clo.relax > 0.0)) && closure_infer()) return 1;
   bool visc = R.cases[0].cp.If_viscous == 1;
   int nv = L.o.nvar;
   int turb = L.o.turb;
@@ -678,13 +681,23 @@ int launch_lusgs(int level, const int* dlist, int nblk, double alfa1, int update
   // every instantiation below is compiled for at most its MAXT threads
   if (nth < 64 || nth > 1024 || nth % 64 != 0) return fail("LU-SGS launch shape out of range");
   const int pk = 10 * level;
+  // the fused launch's skewed copy of the mesh planes: once per layout for
+  // the level's own record arena (when a launch covers every block), on
+  // every launch into a caller's arena
+  const bool static_skew = L.lusgs_fused && (sk != nullptr || !L.skew_static);
+  if (L.lusgs_fused && sk == nullptr && dlist == L.dblocks_all && nblk == (int)L.hb.size()) L.skew_static = true;
 #define LLAUNCH(NV, V)                                                                      \
   if (L.lusgs_fused) {                                                                      \
+    prof_mark(4 + pk);                                                                      \
+    if (static_skew)                                                                        \
+      hipLaunchKernelGGL((k_lusgs_skew<NV, V, true>), gprep, dim3(256), 0, R.s, a);         \
+    hipLaunchKernelGGL((k_lusgs_skew<NV, V, false>), gprep, dim3(256), 0, R.s, a);          \
+    prof_mark(-(4 + pk));                                                                   \
     prof_mark(5 + pk);                                                                      \
     if (L.fused_np == 3)                                                                    \
-      hipLaunchKernelGGL((k_lusgs_fused<NV, V, 3>), dim3(nblk), dim3(256), 0, R.s, a);      \
+      hipLaunchKernelGGL((k_lusgs_fused<NV, V, 3>), dim3(nblk), dim3(320), 0, R.s, a);      \
     else                                                                                    \
-      hipLaunchKernelGGL((k_lusgs_fused<NV, V, 7>), dim3(nblk), dim3(512), 0, R.s, a);      \
+      hipLaunchKernelGGL((k_lusgs_fused<NV, V, 6>), dim3(nblk), dim3(512), 0, R.s, a);      \
     prof_mark(-(5 + pk));                                                                   \
   } else {                                                                                  \
   prof_mark(4 + pk);                                                                        \
@@ -708,10 +721,10 @@ int launch_lusgs(int level, const int* dlist, int nblk, double alfa1, int update
   }                                                                                         \
   }                                                                                         \
   prof_mark(-(5 + pk));                                                                     \
-  }                                                                                         \
   prof_mark(6 + pk);                                                                        \
   hipLaunchKernelGGL((k_lusgs_scatter<NV, V>), gscat, dim3(256), 0, R.s, a);                \
-  prof_mark(-(6 + pk));
+  prof_mark(-(6 + pk));                                                                     \
+  }
   if (nv == 4 && !visc) { LLAUNCH(4, false) }
   else if (nv == 4) { LLAUNCH(4, true) }
   else if (nv == 5) { LLAUNCH(5, true) }
@@ -1215,7 +1228,7 @@ struct Lcg {
 
 // autotune report text of an LU-SGS shape (0: the fused launch)
 std::string shape_text(int sp, int u) {
-  if (sp == 0) return "fused preparation and sweeps, " + std::to_string(u == 3 ? 3 : 7) + " producer waves";
+  if (sp == 0) return "fused preparation and sweeps, " + std::to_string(u == 3 ? 3 : 6) + " producer waves";
   return "workgroups per block " + std::to_string(sp) + ", steps per set " + std::to_string(u);
 }
 
@@ -2248,7 +2261,7 @@ int ec2d_lusgs_shape(int nMesh, int lanes_per_cell, int workgroups_per_block) {
   if (lanes_per_cell != 1 && lanes_per_cell != 2) return fail("lanes per cell must be 1 or 2");
   HIPCHK(hipStreamSynchronize(R.s));
   DLevel& L = R.lv[nMesh - 1];
-  // 0: fused with 7 producer waves, -3: fused with 3
+  // 0: fused with 6 producer waves, -3: fused with 3
   const bool fused = workgroups_per_block == 0 || workgroups_per_block == -3;
   if (workgroups_per_block < 0 && !fused) return fail("workgroups per block 1, 2, 4, or 0 / -3 (fused)");
   if (fused) {
@@ -2259,7 +2272,7 @@ int ec2d_lusgs_shape(int nMesh, int lanes_per_cell, int workgroups_per_block) {
   if (lusgs_layout(L, lanes_per_cell, fused ? 1 : workgroups_per_block, true)) return 1;
   L.lusgs_u = 2;
   L.lusgs_fused = fused;
-  L.fused_np = workgroups_per_block == -3 ? 3 : 7;
+  L.fused_np = workgroups_per_block == -3 ? 3 : 6;
   if (R.graph) {  // a captured graph holds the old launch shape
     HIPCHK(hipGraphExecDestroy(R.graph));
     R.graph = nullptr;
@@ -2348,12 +2361,12 @@ int ec2d_autotune(const char* cache_path, int reps, char* report, int report_len
       for (auto& b : L.hb) fits = fits && b.ny - 1 <= 64;
       if (fits && !lusgs_layout(L, 1, 1, true)) {
         L.lusgs_fused = true;
-        for (int np : {3, 7}) {
+        for (int np : {3, 6}) {
           L.fused_np = np;
           double ms;
           if (time_launches(reps, [&] { return launch_lusgs(l, L.dblocks_all, (int)L.hb.size(), 0.0, 0); }, &ms))
             return 1;
-          note(np == 3 ? "fused3" : "fused7", ms);
+          note(np == 3 ? "fused3" : "fused6", ms);
           // the cache's fourth field holds the producer waves of a fused choice
           if (ms < best) { best = ms; best_lp = 1; best_sp = 0; best_u = np; }
         }
@@ -2380,7 +2393,7 @@ int ec2d_autotune(const char* cache_path, int reps, char* report, int report_len
     if (lusgs_layout(L, best_lp, best_sp == 0 ? 1 : best_sp, true)) return 1;
     L.lusgs_fused = best_sp == 0;
     L.lusgs_u = L.lusgs_fused ? 2 : best_u;
-    if (L.lusgs_fused) L.fused_np = best_u == 3 ? 3 : 7;
+    if (L.lusgs_fused) L.fused_np = best_u == 3 ? 3 : 6;
   }
   if (!added.empty() && cache_path && cache_path[0]) {
     if (FILE* f = std::fopen(cache_path, "a")) {
@@ -2580,7 +2593,7 @@ int ec2d_closure_set_params(const float* theta, const float* norm) {
 }
 
 int ec2d_closure_relaxation(double w) {
-  if (!(w > 0.0 && w <= 1.0)) return fail("relaxation factor must lie in (0, 1]");
+  if (!(w >= 0.0 && w <= 1.0)) return fail("relaxation factor must lie in [0, 1]");
   R.clo.relax = w;
   if (R.graph) {  // the factor is a kernel argument of the captured graph
     HIPCHK(hipGraphExecDestroy(R.graph));

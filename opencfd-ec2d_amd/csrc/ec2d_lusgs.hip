@@ -102,6 +102,26 @@ struct RecF {
   static constexpr int NREC = NSH + 2 * NSD;  // doubles per position
 };
 
+// The inputs of one record, as logical planes (the block planes they come
+// from, or their skewed copies of k_lusgs_skew); U and Res take NV planes.
+struct RecIn {
+  static constexpr int U = 0;
+  template <int NV> static constexpr int RES = NV;
+  template <int NV> static constexpr int VOL = 2 * NV;
+  template <int NV> static constexpr int DT = 2 * NV + 1;
+  template <int NV> static constexpr int LCI = 2 * NV + 2;
+  template <int NV> static constexpr int LCJ = 2 * NV + 3;
+  template <int NV> static constexpr int LVI = 2 * NV + 4;
+  template <int NV> static constexpr int LVJ = 2 * NV + 5;
+  template <int NV> static constexpr int NI1 = 2 * NV + 6;
+  template <int NV> static constexpr int NI2 = 2 * NV + 7;
+  template <int NV> static constexpr int NJ1 = 2 * NV + 8;
+  template <int NV> static constexpr int NJ2 = 2 * NV + 9;
+  template <int NV> static constexpr int SI = 2 * NV + 10;
+  template <int NV> static constexpr int SJ = 2 * NV + 11;
+  template <int NV> static constexpr int COUNT = 2 * NV + 12;  // at most NREC of RecF: the skewed copy fits a record set
+};
+
 // host side: doubles per position of one record set (RecF<nv, visc>::NREC)
 // and of the output pairs
 inline int lusgs_rec_doubles(int nv, bool visc) {
@@ -216,19 +236,85 @@ __device__ __forceinline__ void load_rec(Rec<NV, NS>& p, __amdgpu_buffer_rsrc_t 
 }
 
 // ----------------------------------------------------------------- prep
-// One record set at position (c, r).  Forward: cell (c+1, r+1), neighbours
-// (i-1,j) and (i,j-1); the face areas are the cell's own (sub_LU_SGS.f90:43-56).
-// Backward: cell (ncol-c, nrow-r), neighbours (i+1,j) and (i,j+1), areas of the
-// neighbour, normals of the cell itself (sub_LU_SGS.f90:86-99).
-// SH(d) and SD(k, e) give the destination of shared double d and of side
-// double e of side k (the global skewed records of k_lusgs_prep, or an LDS
-// slot of k_lusgs_fused).
+// The arithmetic of one record (sub_LU_SGS.f90:31-58 forward, :76-100
+// backward) from a loader LD of its inputs; SH(d) and SD(k, e) give the
+// destination of shared double d and of side double e of side k (the global
+// skewed records of k_lusgs_prep, or an LDS slot of k_lusgs_fused).
+template <int NV, bool VISC, bool FWD, class SHF, class SDF, class LDF>
+__device__ __forceinline__ void rec_body(SHF SH, SDF SD, LDF LD, bool has_j, bool has_i, double g, double alfa1) {
+  using F = RecF<NV, VISC>;
+  using I = RecIn;
+  // LD(plane, who): who 0 the cell, 1 its j-neighbour, 2 its i-neighbour
+#pragma unroll
+  for (int m = 0; m < NV; m++) {
+    SH(F::U + m) = LD(I::U + m, 0);
+    if (FWD) SH(m) = LD(I::RES<NV> + m, 0);
+  }
+  // diagonal (sub_LU_SGS.f90:31-39, 76-83)
+  const double vol = LD(I::VOL<NV>, 0), dt = LD(I::DT<NV>, 0);
+  double alfa = vol / dt + 1.0 * (LD(I::LCI<NV>, 0) + LD(I::LCJ<NV>, 0)) + vol * alfa1;
+  if constexpr (VISC) alfa = alfa + 2.0 * (LD(I::LVI<NV>, 0) + LD(I::LVJ<NV>, 0));
+  SH(F::AL0) = alfa;
+  SH(F::Y0) = 1.0 / alfa;
+  if constexpr (NV == 5) {
+    const double alfa4 = alfa + LD(I::LVI<NV>, 0) + LD(I::LVJ<NV>, 0);
+    SH(F::AL4) = alfa4;
+    SH(F::Y4) = 1.0 / alfa4;
+  }
+  // sides: k = 0 the j-neighbour, k = 1 the i-neighbour
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    const bool has = k == 0 ? has_j : has_i;
+    if (!has) continue;
+    const int who = 1 + k;
+    const double n1 = LD(k == 0 ? I::NJ1<NV> : I::NI1<NV>, 0), n2 = LD(k == 0 ? I::NJ2<NV> : I::NI2<NV>, 0);
+    double Un[NV], Fo[NV];
+#pragma unroll
+    for (int m = 0; m < NV; m++) Un[m] = LD(I::U + m, who);
+    flux_n_old<NV>(Fo, Un, n1, n2, g);
+#pragma unroll
+    for (int m = 0; m < NV; m++) SD(k, F::F + m) = Fo[m];
+    SD(k, F::N1) = n1;
+    SD(k, F::N2) = n2;
+    SD(k, F::A) = LD(k == 0 ? I::SJ<NV> : I::SI<NV>, FWD ? 0 : who);
+    SD(k, F::LC) = LD(k == 0 ? I::LCJ<NV> : I::LCI<NV>, who);
+    if constexpr (VISC) SD(k, F::LV) = LD(k == 0 ? I::LVJ<NV> : I::LVI<NV>, who);
+  }
+}
+
+// block plane slot of a logical input plane
+template <int NV>
+__device__ __forceinline__ int rec_in_slot(const PlaneMap& pm, int plane) {
+  using I = RecIn;
+  if (plane < NV) return pm.U + plane;
+  if (plane < 2 * NV) return pm.Res + plane - NV;
+  switch (plane - 2 * NV) {
+    case 0: return pm.vol;
+    case 1: return pm.dt;
+    case 2: return pm.Lci;
+    case 3: return pm.Lcj;
+    case 4: return pm.Lvi;
+    case 5: return pm.Lvj;
+    case 6: return pm.ni1;
+    case 7: return pm.ni2;
+    case 8: return pm.nj1;
+    case 9: return pm.nj2;
+    case 10: return pm.si;
+    default: return pm.sj;
+  }
+  (void)sizeof(I);
+}
+
+// One record set at position (c, r) from the block planes.  Forward: cell
+// (c+1, r+1), neighbours (i-1,j) and (i,j-1); the face areas are the cell's
+// own (sub_LU_SGS.f90:43-56).  Backward: cell (ncol-c, nrow-r), neighbours
+// (i+1,j) and (i,j+1), areas of the neighbour, normals of the cell itself
+// (sub_LU_SGS.f90:86-99).  SH(d) and SD(k, e) give the destination of shared
+// double d and of side double e of side k.
 template <int NV, bool VISC, bool FWD, class SHF, class SDF>
 __device__ __forceinline__ void make_rec(SHF SH, SDF SD, const BlockDesc& B, const double* base, const PlaneMap& pm,
                                          int c, int r, double g, double alfa1) {
-  using F = RecF<NV, VISC>;
   const int ncol = B.nx - 1, nrow = B.ny - 1;
-  auto PL = [&](int slot) { return base + (int64_t)slot * B.plane; };
   // A padding position is never active in a sweep, and a side without a
   // neighbour is never used (the sweep selects 0 for it): neither is
   // written.  They keep the zeros the record arena starts with, which no
@@ -238,47 +324,62 @@ __device__ __forceinline__ void make_rec(SHF SH, SDF SD, const BlockDesc& B, con
   const int i = FWD ? c + 1 : ncol - c;
   const int j = FWD ? r + 1 : nrow - r;
   const int d = FWD ? -1 : 1;
-  const int64_t q = pidx(B, i, j);
+  const int64_t qw[3] = {pidx(B, i, j), pidx(B, i, j + d), pidx(B, i + d, j)};
+  auto LD = [&](int plane, int who) { return base[(int64_t)rec_in_slot<NV>(pm, plane) * B.plane + qw[who]]; };
+  rec_body<NV, VISC, FWD>(SH, SD, LD, r != 0, c != 0, g, alfa1);
+}
+
+// The same record from the skewed copy of the inputs (k_lusgs_skew: logical
+// plane k of the block at sk + k n, a cell at its forward position
+// (i-1 + j-1) R + j-1).  Backward position (c, r) is the cell whose forward
+// position is (ncol-1-c + nrow-1-r) R + nrow-1-r; its neighbours (i+1, j),
+// (i, j+1) sit R and R + 1 further, the forward neighbours (i-1, j), (i, j-1)
+// R and R + 1 before.  Lanes of consecutive rows read consecutive positions.
+// Every input is loaded (one batch of independent loads) before the
+// arithmetic starts: a producer of the fused launch then waits for memory
+// once per record.
+template <int NV, bool VISC, bool FWD, class SHF, class SDF>
+__device__ __forceinline__ void make_rec_sk(SHF SH, SDF SD, const BlockDesc& B, const double* sk, int c, int r,
+                                            double g, double alfa1) {
+  using I = RecIn;
+  const int ncol = B.nx - 1, nrow = B.ny - 1;
+  if (r >= nrow || c < 0 || c >= ncol) return;
+  const int64_t R = B.sk_R, n = B.sk_n;
+  const int cf = FWD ? c : ncol - 1 - c, rf = FWD ? r : nrow - 1 - r;
+  const int64_t p = (int64_t)(cf + rf) * R + rf;
+  const int64_t pw[3] = {p, FWD ? p - R - 1 : p + R + 1, FWD ? p - R : p + R};
+  const bool has[2] = {r != 0, c != 0};
+  // the cell's planes; per side the neighbour's U, Lc, Lv and face area
+  constexpr int NK = I::COUNT<NV>, NB = NV + 3;
+  double own[NK], nb[2][NB];
 #pragma unroll
-  for (int m = 0; m < NV; m++) {
-    SH(F::U + m) = PL(pm.U + m)[q];
-    if (FWD) SH(m) = PL(pm.Res + m)[q];
+  for (int k = 0; k < NK; k++) {
+    const bool need = (FWD || k < I::RES<NV> || k >= I::VOL<NV>) &&
+                      (VISC || (k != I::LVI<NV> && k != I::LVJ<NV>));
+    own[k] = need ? sk[(int64_t)k * n + p] : 0.0;
   }
-  const double* Lci = PL(pm.Lci);
-  const double* Lcj = PL(pm.Lcj);
-  const double* Lvi = PL(pm.Lvi);
-  const double* Lvj = PL(pm.Lvj);
-  // diagonal (sub_LU_SGS.f90:31-39, 76-83)
-  const double vol = PL(pm.vol)[q], dt = PL(pm.dt)[q];
-  double alfa = vol / dt + 1.0 * (Lci[q] + Lcj[q]) + vol * alfa1;
-  if constexpr (VISC) alfa = alfa + 2.0 * (Lvi[q] + Lvj[q]);
-  SH(F::AL0) = alfa;
-  SH(F::Y0) = 1.0 / alfa;
-  if constexpr (NV == 5) {
-    const double alfa4 = alfa + Lvi[q] + Lvj[q];
-    SH(F::AL4) = alfa4;
-    SH(F::Y4) = 1.0 / alfa4;
+#pragma unroll
+  for (int kk = 0; kk < 2; kk++) {
+    const int64_t q = pw[1 + kk];
+#pragma unroll
+    for (int m = 0; m < NB; m++) nb[kk][m] = 0.0;
+    if (has[kk]) {
+#pragma unroll
+      for (int m = 0; m < NV; m++) nb[kk][m] = sk[(int64_t)(I::U + m) * n + q];
+      nb[kk][NV] = sk[(int64_t)(kk == 0 ? I::LCJ<NV> : I::LCI<NV>) * n + q];
+      if constexpr (VISC) nb[kk][NV + 1] = sk[(int64_t)(kk == 0 ? I::LVJ<NV> : I::LVI<NV>) * n + q];
+      if constexpr (!FWD) nb[kk][NV + 2] = sk[(int64_t)(kk == 0 ? I::SJ<NV> : I::SI<NV>) * n + q];
+    }
   }
-  // sides: k = 0 the j-neighbour, k = 1 the i-neighbour
-#pragma unroll
-  for (int k = 0; k < 2; k++) {
-    const bool has = k == 0 ? r != 0 : c != 0;
-    if (!has) continue;
-    const int64_t qn = k == 0 ? pidx(B, i, j + d) : pidx(B, i + d, j);
-    const double n1 = PL(k == 0 ? pm.nj1 : pm.ni1)[q], n2 = PL(k == 0 ? pm.nj2 : pm.ni2)[q];
-    double Un[NV], Fo[NV];
-#pragma unroll
-    for (int m = 0; m < NV; m++) Un[m] = PL(pm.U + m)[qn];
-    flux_n_old<NV>(Fo, Un, n1, n2, g);
-#pragma unroll
-    for (int m = 0; m < NV; m++) SD(k, F::F + m) = Fo[m];
-    SD(k, F::N1) = n1;
-    SD(k, F::N2) = n2;
-    const double* sa = PL(k == 0 ? pm.sj : pm.si);
-    SD(k, F::A) = FWD ? sa[q] : sa[qn];
-    SD(k, F::LC) = (k == 0 ? Lcj : Lci)[qn];
-    if constexpr (VISC) SD(k, F::LV) = (k == 0 ? Lvj : Lvi)[qn];
-  }
+  auto LD = [&](int plane, int who) -> double {
+    if (who == 0) return own[plane];
+    const double* v = nb[who - 1];
+    if (plane < NV) return v[plane];
+    if (plane == I::LCI<NV> || plane == I::LCJ<NV>) return v[NV];
+    if (plane == I::LVI<NV> || plane == I::LVJ<NV>) return v[NV + 1];
+    return v[NV + 2];  // the face area
+  };
+  rec_body<NV, VISC, FWD>(SH, SD, LD, has[0], has[1], g, alfa1);
 }
 
 template <int NV, bool VISC, bool FWD>
@@ -333,6 +434,47 @@ __global__ __launch_bounds__(256) void k_lusgs_prep(LusgsArgs a) {
   const int cb = ncol - 1 - c, rb = nrow - 1 - r;
   write_rec<NV, VISC, true>(fw, n, (int64_t)(c + r) * R + r, B, base, a.pm, c, r, g, a.alfa1);
   write_rec<NV, VISC, false>(fw + F::NREC * n, n, (int64_t)(cb + rb) * R + rb, B, base, a.pm, cb, rb, g, a.alfa1);
+}
+
+// The inputs of the fused launch's records in skewed order: logical plane k
+// (RecIn) of cell (i, j) goes to sk + k n + (i-1 + j-1) R + j-1 (the
+// forward record region, which the fused launch leaves free).  STATIC: the
+// mesh planes (vol, normals, face areas), copied once per layout; else the
+// planes a residual and an update change.  The tiles of k_lusgs_prep: a
+// load touches 8 cache lines per wave, a store 8 (eight lanes of a step fill
+// 8 consecutive positions); every load of a thread is issued before its
+// first store.
+template <int NV, bool VISC>
+__device__ __forceinline__ bool skew_plane(int k, bool stat) {
+  using I = RecIn;
+  const bool is_static = k == I::VOL<NV> || k >= I::NI1<NV>;
+  if (!VISC && (k == I::LVI<NV> || k == I::LVJ<NV>)) return false;
+  return is_static == stat;
+}
+
+template <int NV, bool VISC, bool STATIC>
+__global__ __launch_bounds__(256) void k_lusgs_skew(LusgsArgs a) {
+  const BlockDesc B = a.blocks[a.block_list[blockIdx.y]];
+  if (case_frozen(B.case_id)) return;
+  const int ncol = B.nx - 1, nrow = B.ny - 1;
+  const int S = B.sk_S;
+  const int tiles_s = (S + 31) / 32;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int s = (blockIdx.x % tiles_s) * 32 + wave * 8 + (lane >> 3);
+  const int r = (blockIdx.x / tiles_s) * 8 + (lane & 7);
+  const int c = s - r;
+  if (s >= S || r >= nrow || c < 0 || c >= ncol) return;
+  const double* __restrict__ base = a.arena + B.base;
+  double* __restrict__ sk = a.sk + B.sk_base + (int64_t)s * B.sk_R + r;
+  const int64_t q = pidx(B, c + 1, r + 1);
+  constexpr int NK = RecIn::COUNT<NV>;
+  double v[NK];
+#pragma unroll
+  for (int k = 0; k < NK; k++)
+    if (skew_plane<NV, VISC>(k, STATIC)) v[k] = base[(int64_t)rec_in_slot<NV>(a.pm, k) * B.plane + q];
+#pragma unroll
+  for (int k = 0; k < NK; k++)
+    if (skew_plane<NV, VISC>(k, STATIC)) sk[(int64_t)k * B.sk_n] = v[k];
 }
 
 // ----------------------------------------------------------------- sweeps
@@ -412,7 +554,7 @@ __device__ __forceinline__ void wait_at_least(const int* p, int v) {
 // LPC = 2: lanes 2t (k = 0, the j-neighbour's term) and 2t+1 (k = 1, the
 // i-neighbour's term) share the cell and swap their halves; LPC = 1: lane t
 // computes both.
-template <int NV, bool VISC, bool FWD, int LPC>
+template <int NV, bool VISC, bool FWD, int LPC, bool STORE = true>
 __device__ __forceinline__ void sweep_step(const Rec<NV, 3 - LPC>& p, Out<NV>& own, __amdgpu_buffer_rsrc_t rs,
                                            uint32_t dst, uint32_t n16, int s, int S, int R, int ncol, int nrow, int t,
                                            int k, int lane, int wave, bool cin, bool cout, double g,
@@ -489,7 +631,7 @@ __device__ __forceinline__ void sweep_step(const Rec<NV, 3 - LPC>& p, Out<NV>& o
   // i-neighbour; row t+1 at step s+1 sits at the same column offset as row t
   // at step s, so it is inactive whenever row t was.
   own = o;
-  {
+  if constexpr (STORE) {
     // one lane of an active cell stores; every other lane writes its own
     // slot of the trash pad (positions S*R .. S*R+63)
     const uint32_t w = (act && k == 0) ? (FWD ? (uint32_t)((S - 1 - s) * R + (nrow - 1 - t)) : (uint32_t)(s * R + t))
@@ -717,14 +859,21 @@ __global__ __launch_bounds__(MAXT) void k_lusgs_sweep(LusgsArgs a) {
 // exactly as k_lusgs_sweep<..., LPC = 1> does (the same sweep_step), at the
 // highest issue priority of its SIMD; waves 1..NP produce the records, each
 // every NP-th step of the sequence (forward steps 0..npad-1, then backward
-// steps 0..npad-1), lane t the cell of row t, straight from the block planes
-// (make_rec, the arithmetic of k_lusgs_prep) into a ring of LDS slots
-// (6 for NP = 3, NP + 1 for NP = 7).  The forward dU still goes to the backward records' X pairs in
-// global memory (the backward sweep reads it there, after a fence), and the
-// backward dU to the output pairs that k_lusgs_scatter reads.  NP = 3 or 7
-// (4 or 8 waves; with 7 the ring of a six-equation record takes 156 KB of
-// the CU's 160 KB).
+// steps 0..npad-1), lane t the cell of row t, from the skewed copy of the
+// inputs that k_lusgs_skew made (make_rec_sk, the arithmetic of
+// k_lusgs_prep), into a ring of LDS slots (6 for NP = 3, 7 for NP = 6).  The
+// forward dU goes to the backward records' X pairs in global memory (the
+// backward sweep reads it there, after a fence).  The sweep wave puts the
+// backward dU (and U + dU) of four steps into an LDS buffer; wave NP + 1,
+// the writer, takes each chunk and stores it to the block planes four
+// consecutive cells of a row at a time: this replaces k_lusgs_scatter
+// (stores along a hyperplane, one cache line per lane, hold up whichever
+// wave issues them).  NP = 3 or 6 (5 or 8 waves; with 6 the ring and the
+// buffer of a six-equation record take 157 KB of the CU's 160 KB).
 //
+// The writer waits for ob_ready > chunk, reads the chunk, then sets
+// ob_taken = chunk + 1; the sweep wave waits for ob_taken >= chunk before
+// it writes the chunk's first step, and sets ob_ready after its last.
 // Protocol: produced[p] counts the steps producer p has written, consumed
 // the steps the sweep wave has read.  The sweep wave waits, before step g,
 // for produced[g % NP] > g / NP; producer p waits, before writing step g
@@ -735,11 +884,13 @@ __global__ __launch_bounds__(MAXT) void k_lusgs_sweep(LusgsArgs a) {
 template <int NV, bool VISC, int NP>
 struct FusedLds {
   using F = RecF<NV, VISC>;
-  static constexpr int NRING = NP == 3 ? 6 : NP + 1;  // LDS record slots
+  static constexpr int NRING = NP == 3 ? 6 : 7;  // LDS record slots
   double sh[NRING][F::NSH][64];     // shared doubles of slot, lane
   double sd[NRING][2][F::NSD][64];  // side doubles
+  double ob[4][2 * NV][64];         // the sweep's backward dU and U + dU of four steps
   int produced[NP];
   int consumed;
+  int ob_ready, ob_taken;           // chunks of four backward steps put into / taken from ob
 };
 
 __device__ __forceinline__ bool wait_bounded(const int* p, int v, int* err, int spin_max) {
@@ -756,7 +907,7 @@ __device__ __forceinline__ bool wait_bounded(const int* p, int v, int* err, int 
 }
 
 template <int NV, bool VISC, int NP>
-__global__ __launch_bounds__(64 * (1 + NP)) void k_lusgs_fused(LusgsArgs a) {
+__global__ __launch_bounds__(64 * (2 + NP)) void k_lusgs_fused(LusgsArgs a) {
   using F = RecF<NV, VISC>;
   using LDS = FusedLds<NV, VISC, NP>;
   constexpr int FPROD = NP, FRING = LDS::NRING;
@@ -767,10 +918,8 @@ __global__ __launch_bounds__(64 * (1 + NP)) void k_lusgs_fused(LusgsArgs a) {
   const int ncol = B.nx - 1, nrow = B.ny - 1;
   const int S = B.sk_S, R = B.sk_R;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  if (tid <= FPROD) {
-    if (tid < FPROD) L.produced[tid] = 0;
-    else L.consumed = 0;
-  }
+  if (tid < FPROD) L.produced[tid] = 0;
+  if (tid == 0) L.consumed = L.ob_ready = L.ob_taken = 0;
   __syncthreads();
   constexpr int U = 2;  // the sweep's step padding (npad) of k_lusgs_sweep<..., 2, ..., 1>
   const int npad = (S + 2 * U - 1) / (2 * U) * (2 * U);
@@ -778,11 +927,45 @@ __global__ __launch_bounds__(64 * (1 + NP)) void k_lusgs_fused(LusgsArgs a) {
   const uint32_t n8 = (uint32_t)B.sk_n * 8u, n16 = 2u * n8;
   const uint32_t bytes = (uint32_t)((2 * F::NREC + 2 * F::XP) * B.sk_n * 8);
   const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(a.sk + B.sk_base, 0, bytes, 0x00020000);
-  const uint32_t bw = F::NREC * n8, out = 2 * F::NREC * n8;
+  const uint32_t bw = F::NREC * n8;
+  double* planes = a.arena + B.base;
   bool ok = true;
+  if (wave == FPROD + 1) {
+    // ---- writer: each chunk of four backward steps from ob to the block
+    // planes (the stores of k_lusgs_scatter); lane = 16 rows x 4 steps, so
+    // that a store covers 4 consecutive cells of each of 16 rows
+    const int sub = lane & 3;
+    for (int ck = 0; ck < npad / 4 && ok; ck++) {
+      ok = wait_bounded(&L.ob_ready, ck + 1, a.err, a.spin_max);
+      if (!ok) break;
+      double v[4][2 * NV];
+#pragma unroll
+      for (int rg = 0; rg < 4; rg++)
+#pragma unroll
+        for (int m = 0; m < 2 * NV; m++) v[rg][m] = L.ob[sub][m][rg * 16 + (lane >> 2)];
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);
+      __builtin_amdgcn_s_waitcnt(0xc07f);  // the buffer read before it is handed back
+      lds_put(&L.ob_taken, ck + 1);
+      const int sc = ck * 4 + sub;
+#pragma unroll
+      for (int rg = 0; rg < 4; rg++) {
+        const int tr = rg * 16 + (lane >> 2), c = sc - tr;
+        if (tr < nrow && c >= 0 && c < ncol) {
+          const int64_t q = pidx(B, ncol - c, nrow - tr);
+#pragma unroll
+          for (int m = 0; m < NV; m++) {
+            planes[(int64_t)(a.pm.dU + m) * B.plane + q] = v[rg][m];
+            if (a.update) planes[(int64_t)(a.pm.U + m) * B.plane + q] = v[rg][NV + m];
+          }
+        }
+      }
+    }
+    return;
+  }
   if (wave > 0) {
-    // ---- producer
-    const double* base = a.arena + B.base;
+    // ---- producer: the inputs from their skewed copy (k_lusgs_skew), so
+    // that the lanes (rows) of a step read consecutive positions
+    const double* skin = a.sk + B.sk_base;
     const int p = wave - 1;
     for (int gs = p, k = 0; gs < ntot && ok; gs += FPROD, k++) {
       ok = wait_bounded(&L.consumed, gs - FRING + 1, a.err, a.spin_max);
@@ -791,8 +974,8 @@ __global__ __launch_bounds__(64 * (1 + NP)) void k_lusgs_fused(LusgsArgs a) {
       const int st = fwd ? gs : gs - npad;
       auto SH = [&](int d) -> double& { return L.sh[slot][d][lane]; };
       auto SD = [&](int kk, int e) -> double& { return L.sd[slot][kk][e][lane]; };
-      if (fwd) make_rec<NV, VISC, true>(SH, SD, B, base, a.pm, st - lane, lane, g, a.alfa1);
-      else make_rec<NV, VISC, false>(SH, SD, B, base, a.pm, st - lane, lane, g, a.alfa1);
+      if (fwd) make_rec_sk<NV, VISC, true>(SH, SD, B, skin, st - lane, lane, g, a.alfa1);
+      else make_rec_sk<NV, VISC, false>(SH, SD, B, skin, st - lane, lane, g, a.alfa1);
       __atomic_signal_fence(__ATOMIC_SEQ_CST);
       __builtin_amdgcn_s_waitcnt(0xc07f);  // the slot's LDS writes done (lgkmcnt 0) before publishing
       lds_put(&L.produced[p], k + 1);
@@ -871,8 +1054,26 @@ __global__ __launch_bounds__(64 * (1 + NP)) void k_lusgs_fused(LusgsArgs a) {
       __atomic_signal_fence(__ATOMIC_SEQ_CST);
       __builtin_amdgcn_s_waitcnt(0xc07f);  // the slot's LDS reads done before it is released
       lds_put(&L.consumed, gs + 1);
-      sweep_step<NV, VISC, fwd, 1>(r, own, rs, fwd ? bw : out, n16, st, S, R, ncol, nrow, t, 0, lane, 0, false, false,
-                                   g, ch, ah);
+      // forward: dU to the backward records' X pairs; backward: dU and
+      // U + dU (the record's U is the plane's value) into the output buffer,
+      // four steps at a time to the block planes
+      sweep_step<NV, VISC, fwd, 1, fwd>(r, own, rs, bw, n16, st, S, R, ncol, nrow, t, 0, lane, 0, false, false, g, ch,
+                                        ah);
+      if constexpr (!fwd) {
+        // npad is a multiple of four: every chunk has four steps
+        const int b4 = st & 3, ck = st >> 2;
+        if (b4 == 0 && ck > 0 && ok) ok = wait_bounded(&L.ob_taken, ck, a.err, a.spin_max);
+#pragma unroll
+        for (int m = 0; m < NV; m++) {
+          L.ob[b4][m][t] = own.du[m];
+          L.ob[b4][NV + m][t] = own.W[m];
+        }
+        if (b4 == 3) {
+          __atomic_signal_fence(__ATOMIC_SEQ_CST);
+          __builtin_amdgcn_s_waitcnt(0xc07f);  // the chunk's writes done before it is handed over
+          lds_put(&L.ob_ready, ck + 1);
+        }
+      }
     }
   };
   run_dir(std::true_type{});

@@ -9,8 +9,10 @@
 2. Held-out set: `n_test` parameter sets of another seed, each on a 513x129
    C-grid, solved twice in one batch each, from freestream to the same drop:
    with SA (five equations) and with the network as the eddy-viscosity model
-   (four equations, turbulence model 4).  SA judges the closure's lift and
-   drag.
+   (four equations, turbulence model 4; by default the network's mu_t is
+   frozen once the residual has dropped by `freeze_at`, and the mean flow
+   converges the rest of the way with that mu_t).  SA judges the closure's
+   lift and drag.
 3. Per-case fallback: a closure case falls back to SA when its residual does
    not reach the drop (it stalls: the pass limit, a non-finite residual, or
    a cell below T = 1e-5, the reference's stop, which ends that case only)
@@ -49,10 +51,14 @@ def _write(d, prm, model, nx=None, ny=None):
     return d
 
 
-def _solve(dirs, drop, max_iter, label, theta=None, norm=None, relax=1.0):
+def _solve(dirs, drop, max_iter, label, theta=None, norm=None, relax=1.0, freeze_at=0.0):
     """One batch to the drop; per case passes, status, seconds, (Cl, Cd) and,
     with a closure, the fraction of cells whose features lay outside the
-    training range at the end."""
+    training range at the end.  With a closure and freeze_at > 0 the run has
+    two phases: the network updates mu_t until the residual has dropped by
+    freeze_at, then mu_t stays frozen (relaxation 0) for the remaining
+    drop / freeze_at; a case's passes and seconds add up both phases (its
+    seconds in phase 2 count from that phase's start)."""
     batch = [solver.Case.from_dir(x) for x in dirs]
     with solver.Batch(batch) as b:
         if theta is not None:
@@ -61,7 +67,19 @@ def _solve(dirs, drop, max_iter, label, theta=None, norm=None, relax=1.0):
         b.init_flow()
         b.autotune(reps=3)
         t0 = time.perf_counter()
-        iters, status, ms = b.run_to_drop(drop, max_iter=max_iter, check_every=200)
+        if theta is not None and freeze_at > 0.0:
+            it1, st1, ms1 = b.run_to_drop(freeze_at, max_iter=max_iter, check_every=200)
+            t1 = time.perf_counter() - t0
+            print(f"{label}: {sum(s == solver.Batch.CONVERGED for s in st1)}/{len(batch)} reached the freeze drop "
+                  f"{freeze_at:g} in {t1:.1f} s", flush=True)
+            b.closure_relaxation(0.0)
+            it2, st2, ms2 = b.run_to_drop(drop / freeze_at, max_iter=max_iter, check_every=200)
+            ok1 = [s == solver.Batch.CONVERGED for s in st1]
+            iters = [a + c if o else a for a, c, o in zip(it1, it2, ok1)]
+            status = [c if o else a for a, c, o in zip(st1, st2, ok1)]
+            ms = [t1 * 1e3 + c if o else a for a, c, o in zip(ms1, ms2, ok1)]
+        else:
+            iters, status, ms = b.run_to_drop(drop, max_iter=max_iter, check_every=200)
         wall = time.perf_counter() - t0
         coef = [b.force_coefficients(k)[:2] for k in range(len(batch))]
         out_frac = None
@@ -90,6 +108,8 @@ def main(argv=None):
     ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--relax", type=float, default=1.0, help="under-relaxation of the closure's mu_t")
+    ap.add_argument("--freeze-at", type=float, default=1e-4,
+                    help="freeze the closure's mu_t once the residual has dropped by this factor (0: never)")
     ap.add_argument("--range-frac", type=float, default=0.01,
                     help="fall back to SA when more than this fraction of a case's cells leave the training range")
     ap.add_argument("--out", default="", help="also write the JSON result to this file")
@@ -98,6 +118,7 @@ def main(argv=None):
     train = cases.ensemble_params(seed=args.seed_train, n=args.n_train)
     test = cases.ensemble_params(seed=args.seed_test, n=args.n_test)
     out = {"drop": args.drop, "max_iter": args.max_iter, "max_iter_test": args.max_iter_test, "relax": args.relax,
+           "freeze_at": args.freeze_at,
            "train_cases": len(train),
            "test_cases": len(test), "test_grid": args.test_grid}
     with tempfile.TemporaryDirectory() as d:
@@ -133,7 +154,8 @@ def main(argv=None):
         sa_dirs = [_write(os.path.join(d, f"sa{k}"), p, F.TURB_SA, nx, ny) for k, p in enumerate(test)]
         cl_dirs = [_write(os.path.join(d, f"cl{k}"), p, F.TURB_CLOSURE, nx, ny) for k, p in enumerate(test)]
         sa = _solve(sa_dirs, args.drop, args.max_iter_test, "held-out SA")
-        cl = _solve(cl_dirs, args.drop, args.max_iter_test, "held-out closure", theta, norm, args.relax)
+        cl = _solve(cl_dirs, args.drop, args.max_iter_test, "held-out closure", theta, norm, args.relax,
+                    args.freeze_at)
     per = []
     for k in range(len(test)):
         fell = (cl["status"][k] != solver.Batch.CONVERGED) or (cl["out_of_range"][k] > args.range_frac)
@@ -157,6 +179,9 @@ def main(argv=None):
         "sa_batch_seconds": sa["wall_s"], "closure_batch_seconds": cl["wall_s"],
         "median_time_ratio_closure_over_sa": (float(np.median([p["seconds_with_fallback"] / p["sa_seconds"]
                                                                for p in ok])) if ok else None),
+        "median_pass_ratio_closure_over_sa_no_fallback": (
+            float(np.median([p["closure_passes"] / p["sa_passes"] for p in ok if not p["fallback"]]))
+            if any(not p["fallback"] for p in ok) else None),
         "cl_within_0.5pct": sum(p["Cl_rel_err"] <= 5e-3 for p in per),
         "cd_within_1_count": sum(p["Cd_err_counts"] <= 1.0 for p in per),
         "no_fallback_cl_err_median": (float(np.median([p["Cl_rel_err"] for p in per if not p["fallback"]]))

@@ -387,7 +387,7 @@ def test_multigrid_scratch_arrays(tm):
                                      ("euler_64rows", -3), ("sa_48rows", -3), ("sst_48rows", -3)])
 def test_fused_lusgs_preparation(kind, wg):
     """k_lusgs_fused (the records made in the sweep's workgroup, for blocks of
-    at most 64 rows; 7 producer waves, or 3 with wg = -3) against the oracle:
+    at most 64 rows; 6 producer waves, or 3 with wg = -3) against the oracle:
     bit-exact for Euler, 1e-11 with the turbulence models; eager passes, then
     graph replays."""
     if kind == "euler_64rows":

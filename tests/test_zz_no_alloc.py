@@ -58,6 +58,10 @@ def test_noalloc_lusgs_equals_allocating_entry_point():
         b.comput_residual_one_mesh(1)
         _both(b, mblock=0, blocks=range(1, 7))
         _both(b, mblock=3, blocks=(3,))
+        # the fused launch (its skewed input copy made in the scratch on every call)
+        b.lusgs_shape(1, 1, 0)
+        _both(b, mblock=0, blocks=range(1, 7))
+        _both(b, mblock=2, blocks=(2,))
 
 
 def test_noalloc_refuses_a_level_beyond_the_scratch():
