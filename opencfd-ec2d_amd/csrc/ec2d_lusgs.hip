@@ -1025,38 +1025,44 @@ __global__ __launch_bounds__(64 * (2 + NP)) void k_lusgs_fused(LusgsArgs a) {
     }
   };
   // one direction of the sweep; the direction is a compile-time constant so
-  // that every register array keeps fixed indices
+  // that every register array keeps fixed indices.  The next step's record
+  // is read from its slot while this step computes; in the backward sweep
+  // each step's X (the forward dU) is loaded two steps ahead, into one of
+  // two register sets (the loop runs two steps per turn: npad is even).
   auto run_dir = [&](auto fwd_c) {
     constexpr bool fwd = decltype(fwd_c)::value;
     constexpr int dir = fwd ? 0 : 1;
+    const int gs0 = dir * npad;
     Ahead<NV> ah = {};
     Out<NV> own = {};
-    // X of this step and of the next (loaded a step ahead); register arrays
-    // with fixed indices only
-    double xc[2 * F::XP] = {}, xn[2 * F::XP] = {};
+    double xa[2 * F::XP] = {}, xb[2 * F::XP] = {};
     if (!fwd) {
       // the forward sweep's stores to the X pairs before these loads
       __threadfence();
-      load_x(xc, 0);
+      load_x(xa, 0);
+      load_x(xb, 1);
     }
-    for (int st = 0; st < npad; st++) {
-      const int gs = dir * npad + st;
-      if (!fwd && st + 1 < npad) load_x(xn, st + 1);
-      if (ok) ok = wait_bounded(&L.produced[gs % FPROD], gs / FPROD + 1, a.err, a.spin_max);
-      Rec<NV, 2> r;
-      load_lds(r, gs % FRING, fwd);
-      if (!fwd) {
-#pragma unroll
-        for (int m = 0; m < NV; m++) r.X[m] = xc[m];
-#pragma unroll
-        for (int m = 0; m < 2 * F::XP; m++) xc[m] = xn[m];
-      }
+    Rec<NV, 2> rn;
+    if (ok) ok = wait_bounded(&L.produced[gs0 % FPROD], gs0 / FPROD + 1, a.err, a.spin_max);
+    load_lds(rn, gs0 % FRING, fwd);
+    auto one_step = [&](int st, double* x) {
+      const int gs = gs0 + st;
+      Rec<NV, 2> r = rn;
       __atomic_signal_fence(__ATOMIC_SEQ_CST);
       __builtin_amdgcn_s_waitcnt(0xc07f);  // the slot's LDS reads done before it is released
       lds_put(&L.consumed, gs + 1);
+      if (st + 1 < npad) {
+        if (ok) ok = wait_bounded(&L.produced[(gs + 1) % FPROD], (gs + 1) / FPROD + 1, a.err, a.spin_max);
+        load_lds(rn, (gs + 1) % FRING, fwd);
+      }
+      if constexpr (!fwd) {
+#pragma unroll
+        for (int m = 0; m < NV; m++) r.X[m] = x[m];
+        if (st + 2 < npad) load_x(x, st + 2);
+      }
       // forward: dU to the backward records' X pairs; backward: dU and
       // U + dU (the record's U is the plane's value) into the output buffer,
-      // four steps at a time to the block planes
+      // which the writer wave stores four steps at a time
       sweep_step<NV, VISC, fwd, 1, fwd>(r, own, rs, bw, n16, st, S, R, ncol, nrow, t, 0, lane, 0, false, false, g, ch,
                                         ah);
       if constexpr (!fwd) {
@@ -1074,6 +1080,10 @@ __global__ __launch_bounds__(64 * (2 + NP)) void k_lusgs_fused(LusgsArgs a) {
           lds_put(&L.ob_ready, ck + 1);
         }
       }
+    };
+    for (int st = 0; st < npad; st += 2) {
+      one_step(st, xa);
+      one_step(st + 1, xb);
     }
   };
   run_dir(std::true_type{});

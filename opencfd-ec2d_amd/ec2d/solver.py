@@ -335,8 +335,9 @@ class Batch:
         with Kstep = 0 and the dual-time first-step flags set afresh.
         Switches held on the host (closure enable, relaxation factor) are the
         caller's to set again."""
-        rec = {"format": np.array([1], np.int64),
-               "layout": np.array([[c.ctl.nvar, c.num_mesh, len(c.levels[0])] for c in self.cases], np.int64)}
+        rec = {"format": np.array([2], np.int64),
+               "layout": np.array([[c.ctl.nvar, c.num_mesh, len(c.levels[0])] for c in self.cases], np.int64),
+               "control": np.array([[float(getattr(c.ctl, f)) for f in self.CHECKPOINT_CONTROL] for c in self.cases])}
         for key, (n, ci, lev, mb) in self._state_keys():
             rec[key] = self.get(n, ci, lev, mb)
         kt = np.zeros((len(self.cases), 3), np.int64)
@@ -355,17 +356,31 @@ class Batch:
         with open(path, "wb") as f:
             np.savez(f, **rec)
 
+    # control.in records that define a run: a checkpoint resumes only a run
+    # with the same values (the physical step, the flow condition, the time
+    # method, the turbulence model and the discretisation)
+    CHECKPOINT_CONTROL = ("Time_Method", "Iflag_turbulence_model", "If_viscous", "Iflag_Scheme", "Iflag_Flux",
+                          "IFlag_Reconstruction", "Iflag_local_dt", "Num_Mesh", "dt_global", "CFL", "Ma", "Re", "AoA",
+                          "gamma", "Pr", "T_inf", "Twall", "Nstep_Inner_Limit", "Res_Inner_Limit")
+
     def load_checkpoint(self, path: str):
         """Restores what `save_checkpoint` wrote; the batch must hold the same
-        cases (variables, levels, blocks and block sizes), else ValueError.
+        cases (variables, levels, blocks and block sizes, and the run-defining
+        records of control.in, CHECKPOINT_CONTROL), else ValueError.
         Residual norms are those of the next residual evaluation."""
         with np.load(path, allow_pickle=False) as z:
             rec = {k: z[k] for k in z.files}
-        if int(rec.get("format", [0])[0]) != 1:
-            raise ValueError(f"{path}: not an ec2d checkpoint")
+        if int(rec.get("format", [0])[0]) != 2:
+            raise ValueError(f"{path}: not an ec2d checkpoint of this format")
         layout = np.array([[c.ctl.nvar, c.num_mesh, len(c.levels[0])] for c in self.cases], np.int64)
         if rec["layout"].shape != layout.shape or (rec["layout"] != layout).any():
             raise ValueError(f"{path}: checkpoint of other cases (layout {rec['layout'].tolist()})")
+        ctl = rec["control"]
+        for ci, c in enumerate(self.cases):
+            for k, f in enumerate(self.CHECKPOINT_CONTROL):
+                want = float(getattr(c.ctl, f))
+                if ctl[ci, k] != want:
+                    raise ValueError(f"{path}: case {ci} was run with {f} = {ctl[ci, k]!r}, this batch has {want!r}")
         for key, (n, ci, lev, mb) in self._state_keys():
             if key not in rec:
                 raise ValueError(f"{path}: no {key}")

@@ -96,6 +96,44 @@ def random_case(seed: int, prefix: str):
     return d, 3
 
 
+def random_turb_multiblock_case(seed: int, prefix: str):
+    """A seeded random multi-block case with a turbulence model: 2..3 x 1..3
+    blocks of 9..21 x 9..17 points, every outer subface a random boundary
+    type and the first one a wall (so the wall distance, the model's ghost
+    rules and mu_t on the first ghost layer act across block interfaces,
+    the 3-block RAE2822 file's reversed orientation included when the layout
+    has it), Spalart-Allmaras, SST (isothermal wall) or Baldwin-Lomax, random
+    scheme / flux / reconstruction, LU-SGS or RK3, freestream start.  Returns
+    (case directory, steps)."""
+    rng = np.random.default_rng(5000 + seed)
+    turb = int(rng.choice([F.TURB_SA, F.TURB_SST, F.TURB_BL]))
+    c = cases.control_vortex(iters=10)
+    c.Iflag_init = 0
+    c.If_viscous, c.Re, c.Pr = 1, float(10.0 ** rng.uniform(3.5, 5.0)), 0.72
+    c.Iflag_turbulence_model = turb
+    c.vt_inf = 3.0
+    c.Twall = 300.0 if (turb == F.TURB_SST or rng.random() < 0.5) else -1.0
+    c.Iflag_Scheme = int(rng.integers(0, 7))
+    c.Iflag_Flux = int(rng.integers(1, 7))
+    c.IFlag_Reconstruction = int(rng.integers(0, 3))
+    tm = int(rng.choice([F.TIME_LU_SGS, F.TIME_RK3]))
+    c.Time_Method = tm
+    c.CFL = {F.TIME_LU_SGS: 1.0, F.TIME_RK3: 0.3}[tm]
+    kinds = [int(v) for v in rng.integers(0, len(BC_CHOICES), size=64)]
+
+    def edit(bcs):
+        k = 0
+        for subs in bcs:
+            for s in subs:
+                if s[6] == F.BC_FARFIELD:
+                    s[6] = F.BC_WALL if k == 0 else BC_CHOICES[kinds[k % len(kinds)]]
+                    k += 1
+
+    d = vortex_case(f"{prefix}_rturb_{seed}", c, seed=seed, nbx=int(rng.integers(2, 4)), nby=int(rng.integers(1, 4)),
+                    bc_edit=edit)
+    return d, 3
+
+
 def random_naca_case(seed: int, prefix: str):
     """A seeded random single-block C-grid case: grid 49..73 x 13..21 points,
     inviscid, laminar, Baldwin-Lomax, SA or SST (isothermal wall), random

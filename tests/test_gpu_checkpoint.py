@@ -96,3 +96,23 @@ def test_checkpoint_refuses_other_cases(tmp_path):
             b.load_checkpoint(path)
     with np.load(path) as z:
         assert all(z[k].dtype != object for k in z.files)
+
+
+@pytest.mark.parametrize("field,value", [("dt_global", 0.02), ("Time_Method", 1), ("Ma", 0.55)])
+def test_checkpoint_refuses_another_run(field, value, tmp_path):
+    """Same blocks, but a run-defining record of control.in differs (the
+    physical step, the time method, the flow condition): refused by name."""
+    import dataclasses
+
+    path = str(tmp_path / "state.npz")
+    c1 = cases.control_euler_naca(iters=10)
+    d1 = naca_case("g_ckpt_run_a", c1)
+    d2 = naca_case(f"g_ckpt_run_{field}", dataclasses.replace(c1, **{field: value}))
+    with solver.Batch([solver.Case.from_dir(d1)]) as b:
+        b.init_flow()
+        b.main_step()
+        b.save_checkpoint(path)
+    with solver.Batch([solver.Case.from_dir(d2)]) as b:
+        b.init_flow()
+        with pytest.raises(ValueError, match=field):
+            b.load_checkpoint(path)

@@ -18,6 +18,20 @@ def test_random_case(seed):
     _compare_steps(d, n)
 
 
+@pytest.mark.parametrize("seed", range(12))
+def test_random_turb_multiblock_case(seed):
+    """Seeded random multi-block cases with Spalart-Allmaras, SST or
+    Baldwin-Lomax (tests/helpers.py random_turb_multiblock_case; the same
+    seeds run oracle against the reference build in
+    tests/test_oracle_reference.py::test_random_turb_multiblock_cases): U,
+    Res, dt and mu_t (its first ghost layer included) of every block within
+    1e-11 relative per pass, the norms within 1e-10."""
+    from helpers import random_turb_multiblock_case
+
+    d, n = random_turb_multiblock_case(seed, "g")
+    _compare_steps(d, n, ("U", "Res", "dt", "Amu_t"), exact=False, rtol=1e-11, norm_rtol=1e-10)
+
+
 @pytest.mark.parametrize("seed", range(24))
 def test_random_naca_case(seed):
     """Seeded random C-grid cases (tests/helpers.py random_naca_case),
@@ -52,6 +66,18 @@ def test_random_dual_time_case(seed):
     from test_gpu_dual import _check
 
     d, exact = random_dual_case(seed, "g")
+    # The device decides each inner-loop exit from tree-summed norms, the
+    # oracle from serial sums (equal to about 1e-15 relative, 1e-11 with SA):
+    # a seed whose exit test lands within 1e-8 of the limit could exit at
+    # another inner iteration, so it is left out, with its values.
+    probe = O.OracleCase(d)
+    probe.init_flow()
+    for _ in range(3):
+        probe.main_step()
+    limit = probe.ctl.Res_Inner_Limit
+    close = [v for v in probe.inner_max_res if abs(v - limit) <= 1e-8 * limit]
+    if close:
+        pytest.skip(f"an inner-loop exit test lies within 1e-8 of the limit {limit:g}: {close}")
     ref = O.OracleCase(d)
     ref.init_flow()
     with solver.Batch([solver.Case.from_dir(d)]) as b:

@@ -325,6 +325,18 @@ def test_random_cases(seed):
     run_both(d, n, ("U", "Res", "dt"))
 
 
+@pytest.mark.parametrize("seed", range(12))
+def test_random_turb_multiblock_cases(seed):
+    """Seeded random multi-block cases with Spalart-Allmaras, SST or
+    Baldwin-Lomax (tests/helpers.py random_turb_multiblock_case): the oracle
+    equals the reference bit for bit, U, Res, dt and mu_t (its first ghost
+    layer included) of every block after every pass."""
+    from helpers import random_turb_multiblock_case
+
+    d, n = random_turb_multiblock_case(seed, "r")
+    run_both(d, n, ("U", "Res", "dt", "Amu_t"))
+
+
 @pytest.mark.parametrize("seed", range(24))
 def test_random_naca_cases(seed):
     """Seeded random C-grid cases (turbulence model, scheme, flux,
