@@ -93,6 +93,7 @@ struct DLevel {
   int64_t sk_total = 0;       // doubles of the level's record arena
   int max_cells = 0;          // largest block (cells)
   int prep_tiles = 0;         // LU-SGS prep workgroups per block: 32 x 8 cell tiles of the largest block
+  int skew_tiles = 0;         // k_lusgs_skew workgroups per block: 16 x 16 tiles of the largest block's skewed space
   std::vector<long long> case_cells;
   bool fdm = false;
   int2 *d_ilines = nullptr, *d_frows = nullptr;  // d_ilines: i-lines then j-lines
@@ -266,6 +267,7 @@ int lusgs_layout(DLevel& L, int lpc, int split, bool upload) {
   L.sk_max_n = 0;
   L.max_cells = 0;
   L.prep_tiles = 0;
+  L.skew_tiles = 0;
   for (auto& d : L.hb) {
     const int ncol = d.nx - 1, nrow = d.ny - 1;
     d.sk_S = ncol + nrow - 1;
@@ -277,6 +279,7 @@ int lusgs_layout(DLevel& L, int lpc, int split, bool upload) {
     L.sk_max_n = std::max(L.sk_max_n, d.sk_n);
     L.max_cells = std::max(L.max_cells, ncol * nrow);
     L.prep_tiles = std::max(L.prep_tiles, ((d.sk_S + 31) / 32) * ((nrow + 7) / 8));
+    L.skew_tiles = std::max(L.skew_tiles, ((d.sk_S + 15) / 16) * ((nrow + 15) / 16));
   }
   if (L.sk) HIPCHK(hipFree(L.sk));
   L.sk = nullptr;
@@ -671,7 +674,7 @@ int launch_lusgs(int level, const int* dlist, int nblk, double alfa1, int update
   bool visc = R.cases[0].cp.If_viscous == 1;
   // prep and scatter grid: tiles of 32 steps x 8 rows of the largest block's skewed space
   dim3 gprep((unsigned)L.prep_tiles, nblk), gsweep(nblk * L.lusgs_split), blk(L.lusgs_threads);
-  dim3 gscat((unsigned)L.prep_tiles, nblk);
+  dim3 gscat((unsigned)L.prep_tiles, nblk), gskew((unsigned)L.skew_tiles, nblk);
   int nv = L.o.nvar;
   // the instantiation is chosen by thread count (its launch bound): up to 4
   // waves one wave per SIMD and two steps per register set; more waves (up
@@ -690,8 +693,8 @@ int launch_lusgs(int level, const int* dlist, int nblk, double alfa1, int update
   if (L.lusgs_fused) {                                                                      \
     prof_mark(4 + pk);                                                                      \
     if (static_skew)                                                                        \
-      hipLaunchKernelGGL((k_lusgs_skew<NV, V, true>), gprep, dim3(256), 0, R.s, a);         \
-    hipLaunchKernelGGL((k_lusgs_skew<NV, V, false>), gprep, dim3(256), 0, R.s, a);          \
+      hipLaunchKernelGGL((k_lusgs_skew<NV, V, true>), gskew, dim3(256), 0, R.s, a);         \
+    hipLaunchKernelGGL((k_lusgs_skew<NV, V, false>), gskew, dim3(256), 0, R.s, a);          \
     prof_mark(-(4 + pk));                                                                   \
     prof_mark(5 + pk);                                                                      \
     if (L.fused_np == 3)                                                                    \

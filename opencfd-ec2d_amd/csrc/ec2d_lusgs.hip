@@ -440,12 +440,11 @@ __global__ __launch_bounds__(256) void k_lusgs_prep(LusgsArgs a) {
 // (RecIn) of cell (i, j) goes to sk + k n + (i-1 + j-1) R + j-1 (the
 // forward record region, which the fused launch leaves free).  STATIC: the
 // mesh planes (vol, normals, face areas), copied once per layout; else the
-// planes a residual and an update change.  The tiles of k_lusgs_prep: a
-// load touches 8 cache lines per wave, a store 8 (eight lanes of a step fill
-// 8 consecutive positions); every load of a thread is issued before its
-// first store.
+// planes a residual and an update change.  A workgroup moves a tile of 16
+// steps x 16 rows through LDS: 16 lanes read 16 consecutive cells of a row,
+// and 16 lanes write 16 consecutive positions of a step (128 bytes each).
 template <int NV, bool VISC>
-__device__ __forceinline__ bool skew_plane(int k, bool stat) {
+__host__ __device__ constexpr bool skew_plane(int k, bool stat) {
   using I = RecIn;
   const bool is_static = k == I::VOL<NV> || k >= I::NI1<NV>;
   if (!VISC && (k == I::LVI<NV> || k == I::LVJ<NV>)) return false;
@@ -454,27 +453,46 @@ __device__ __forceinline__ bool skew_plane(int k, bool stat) {
 
 template <int NV, bool VISC, bool STATIC>
 __global__ __launch_bounds__(256) void k_lusgs_skew(LusgsArgs a) {
+  constexpr int NK = RecIn::COUNT<NV>;
+  // the planes this launch copies, in order
+  constexpr int NC = [] {
+    int n = 0;
+    for (int k = 0; k < NK; k++) n += skew_plane<NV, VISC>(k, STATIC) ? 1 : 0;
+    return n;
+  }();
+  __shared__ double tile[NC][16][17];  // [plane][row][step], padded against bank conflicts
   const BlockDesc B = a.blocks[a.block_list[blockIdx.y]];
   if (case_frozen(B.case_id)) return;
   const int ncol = B.nx - 1, nrow = B.ny - 1;
   const int S = B.sk_S;
-  const int tiles_s = (S + 31) / 32;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int s = (blockIdx.x % tiles_s) * 32 + wave * 8 + (lane >> 3);
-  const int r = (blockIdx.x / tiles_s) * 8 + (lane & 7);
-  const int c = s - r;
-  if (s >= S || r >= nrow || c < 0 || c >= ncol) return;
+  const int tiles_s = (S + 15) / 16;
+  if ((int)blockIdx.x >= tiles_s * ((nrow + 15) / 16)) return;  // the launch fits the level's largest block
+  const int s0 = (blockIdx.x % tiles_s) * 16, r0 = (blockIdx.x / tiles_s) * 16;
+  const int tid = threadIdx.x;
   const double* __restrict__ base = a.arena + B.base;
-  double* __restrict__ sk = a.sk + B.sk_base + (int64_t)s * B.sk_R + r;
-  const int64_t q = pidx(B, c + 1, r + 1);
-  constexpr int NK = RecIn::COUNT<NV>;
-  double v[NK];
+  // load: 16 lanes of a row read 16 consecutive cells (step s, cell c = s - r)
+  {
+    const int rr = tid >> 4, ss = tid & 15, r = r0 + rr, s = s0 + ss, c = s - r;
+    if (s < S && r < nrow && c >= 0 && c < ncol) {
+      const int64_t q = pidx(B, c + 1, r + 1);
+      int j = 0;
 #pragma unroll
-  for (int k = 0; k < NK; k++)
-    if (skew_plane<NV, VISC>(k, STATIC)) v[k] = base[(int64_t)rec_in_slot<NV>(a.pm, k) * B.plane + q];
+      for (int k = 0; k < NK; k++)
+        if (skew_plane<NV, VISC>(k, STATIC)) tile[j++][rr][ss] = base[(int64_t)rec_in_slot<NV>(a.pm, k) * B.plane + q];
+    }
+  }
+  __syncthreads();
+  // store: 16 lanes of a step write 16 consecutive positions s R + r
+  {
+    const int ss = tid >> 4, rr = tid & 15, r = r0 + rr, s = s0 + ss, c = s - r;
+    if (s < S && r < nrow && c >= 0 && c < ncol) {
+      double* __restrict__ sk = a.sk + B.sk_base + (int64_t)s * B.sk_R + r;
+      int j = 0;
 #pragma unroll
-  for (int k = 0; k < NK; k++)
-    if (skew_plane<NV, VISC>(k, STATIC)) sk[(int64_t)k * B.sk_n] = v[k];
+      for (int k = 0; k < NK; k++)
+        if (skew_plane<NV, VISC>(k, STATIC)) sk[(int64_t)k * B.sk_n] = tile[j++][rr][ss];
+    }
+  }
 }
 
 // ----------------------------------------------------------------- sweeps
