@@ -2307,7 +2307,17 @@ int ec2d_autotune(const char* cache_path, int reps, char* report, int report_len
     std::string key = "nv" + std::to_string(L.o.nvar) + "_s" + std::to_string(L.o.scheme) + "_f" +
                       std::to_string(L.o.flux) + "_r" + std::to_string(L.o.recon) + "_t" + std::to_string(L.o.turb) +
                       "_v" + std::to_string(R.cases[0].cp.If_viscous) + (L.fdm ? "_fdm" : "");
-    for (auto& b : L.hb) key += "_" + std::to_string(b.nx) + "x" + std::to_string(b.ny);
+    // the block shapes in order, as a count and a 64-bit FNV-1a hash (a key
+    // stays one short word however many blocks the level has)
+    {
+      std::string shapes;
+      for (auto& b : L.hb) shapes += "_" + std::to_string(b.nx) + "x" + std::to_string(b.ny);
+      uint64_t h = 1469598103934665603ull;
+      for (unsigned char ch : shapes) h = (h ^ ch) * 1099511628211ull;
+      char hx[48];
+      std::snprintf(hx, sizeof(hx), "_b%zu_%016llx", L.hb.size(), (unsigned long long)h);
+      key += hx;
+    }
     int best_tr = -1, best_lp = -1, best_sp = 1, best_u = 2;
     for (auto& e : cache)
       if (e.first == key) { best_tr = e.second.tr; best_lp = e.second.lp; best_sp = e.second.sp; best_u = e.second.u; }

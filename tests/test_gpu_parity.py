@@ -290,6 +290,24 @@ def test_autotune_keeps_results_and_caches(tmp_path):
         assert rep2.count("cached") == 2, rep2
 
 
+def test_autotune_cache_of_a_large_batch(tmp_path):
+    """A batch of 140 small cases of several sizes (its list of block
+    shapes runs to about 900 characters): a fresh batch of the same cases
+    finds the first batch's choice in the cache file, which holds one short
+    line per level configuration."""
+    dirs = [naca_case(f"g_tune_many_{k}", cases.control_euler_naca(iters=10), nx=33 + 8 * (k % 3), ny=9)
+            for k in range(140)]
+    cache = str(tmp_path / "tune.txt")
+    for want in ("measured", "cached"):
+        with solver.Batch([solver.Case.from_dir(d) for d in dirs]) as b:
+            b.init_flow()
+            rep = b.autotune(cache, reps=1)
+            assert rep.count(want) == 1, rep
+    with open(cache) as f:
+        lines = f.read().splitlines()
+    assert len(lines) == 1 and len(lines[0]) < 120, lines
+
+
 @pytest.mark.parametrize("lpc,split,ny", [(1, 2, 73), (2, 2, 73), (2, 4, 73), (2, 4, 129)])
 def test_split_sweep_tall_block_multigrid(lpc, split, ny):
     """Rows of a block split between two or four workgroups (one CU each):
