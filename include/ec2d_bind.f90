@@ -210,6 +210,10 @@ module ec2d_bind
       import
       integer(c_int), value :: rows
     end function
+    integer(c_int) function ec2d_set_tile_shape(cols, rows) bind(C, name="ec2d_set_tile_shape")
+      import
+      integer(c_int), value :: cols, rows
+    end function
     integer(c_int) function ec2d_set_profile(on) bind(C, name="ec2d_set_profile")
       import
       integer(c_int), value :: on

@@ -238,7 +238,7 @@ __global__ __launch_bounds__(256) void k_fdm_res(FdmArgs a) {
 // exist, because their interior residual is only known after k_fdm_res)
 template <int NV>
 __global__ __launch_bounds__(256) void k_tile_norms(const TileDesc* tiles, const BlockDesc* blocks, double* arena,
-                                                    PlaneMap pm, double* partials, int tile_rows) {
+                                                    PlaneMap pm, double* partials, int tile_cols, int tile_rows) {
   const int lt = blockIdx.x;
   const TileDesc t = tiles[lt];
   const BlockDesc B = blocks[t.block];
@@ -248,9 +248,9 @@ __global__ __launch_bounds__(256) void k_tile_norms(const TileDesc* tiles, const
   double mx[NV], sm[NV];
   for (int m = 0; m < NV; m++) { mx[m] = 0.0; sm[m] = 0.0; }
   const int tid = threadIdx.x;
-  const int ncell = 64 * tile_rows;
+  const int ncell = tile_cols * tile_rows;
   for (int c = tid; c < ncell; c += blockDim.x) {
-    int i = t.i0 + c % 64, j = t.j0 + c / 64;
+    int i = t.i0 + c % tile_cols, j = t.j0 + c / tile_cols;
     if (i > B.nx - 1 || j > B.ny - 1) continue;
     int64_t q = pidx(B, i, j);
     for (int m = 0; m < NV; m++) {

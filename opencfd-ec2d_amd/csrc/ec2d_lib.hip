@@ -65,7 +65,8 @@ struct DLevel {
   TileDesc* dt = nullptr;
   int* dperm = nullptr;
   int ntiles = 0, nlaunch = 0;
-  int tile_rows = 4;          // residual tile: 64 x tile_rows cells
+  int tile_rows = 4;          // residual tile: tile_cols x tile_rows cells
+  int tile_cols = 64;         // 64, or 32 with 8 rows
   std::vector<int> case_tile0;
   int* dcase_tile0 = nullptr;
   double* dncell = nullptr;
@@ -112,6 +113,7 @@ struct Runtime {
   std::string err;
   int num_mesh = 1;
   int tile_rows = 4;
+  int tile_cols = 64;
   bool clip_pending[3] = {false, false, false};
   // dual-time stepping: the reference's first-call flags (integer, save) of
   // Comput_Residual_one_mesh (Sfac, sub_Residual.f90:18,28-34) and of
@@ -317,7 +319,7 @@ int build_tiles(DLevel& L) {
   L.ht.clear();
   L.case_tile0.clear();
   // tiles (logical order: case, block, row of tiles, column of tiles)
-  int TX = 64, TY = L.tile_rows;
+  int TX = L.tile_cols, TY = L.tile_rows;
   for (size_t ci = 0; ci < R.cases.size(); ci++) {
     L.case_tile0.push_back((int)L.ht.size());
     for (int b = L.case_first_block[ci]; b < L.case_first_block[ci + 1]; b++) {
@@ -586,13 +588,17 @@ int launch_residual(int level, int clip, int force = 0, int set_un = 0) {
     hipLaunchKernelGGL(k_sst_cell, dim3(L.nrows), dim3(256), 0, R.s, sa);
     hipLaunchKernelGGL(k_sst_res, dim3(L.nrows), dim3(256), 0, R.s, sa);
   }
-#define RLAUNCH(TY, NV, V, S) hipLaunchKernelGGL((k_residual<64, TY, NV, V, S>), grid, dim3(64 * TY), 0, R.s, a)
-#define RBY_TY(NV, V, S)                      \
-  switch (L.tile_rows) {                      \
-    case 1: RLAUNCH(1, NV, V, S); break;      \
-    case 2: RLAUNCH(2, NV, V, S); break;      \
-    case 8: RLAUNCH(8, NV, V, S); break;      \
-    default: RLAUNCH(4, NV, V, S); break;     \
+#define RLAUNCH(TX, TY, NV, V, S) hipLaunchKernelGGL((k_residual<TX, TY, NV, V, S>), grid, dim3(TX * TY), 0, R.s, a)
+#define RBY_TY(NV, V, S)                        \
+  if (L.tile_cols == 32) {                      \
+    RLAUNCH(32, 8, NV, V, S);                   \
+  } else {                                      \
+    switch (L.tile_rows) {                      \
+      case 1: RLAUNCH(64, 1, NV, V, S); break;  \
+      case 2: RLAUNCH(64, 2, NV, V, S); break;  \
+      case 8: RLAUNCH(64, 8, NV, V, S); break;  \
+      default: RLAUNCH(64, 4, NV, V, S); break; \
+    }                                           \
   }
   if (nv == 4 && !visc) {
     RBY_TY(4, false, false)
@@ -627,9 +633,9 @@ int launch_residual(int level, int clip, int force = 0, int set_un = 0) {
     // k_residual has just written) in one launch
     hipLaunchKernelGGL(k_fdm_lines, dim3(f.n_lines + (visc ? L.n_frows : 0)), dim3(256), 0, R.s, f);
     hipLaunchKernelGGL(k_fdm_res, dim3(L.n_frows), dim3(256), 0, R.s, f);
-    if (nv == 4) hipLaunchKernelGGL(k_tile_norms<4>, dim3(L.ntiles), dim3(256), 0, R.s, L.dt, L.db, L.arena, L.pm, L.dpartials, L.tile_rows);
-    else if (nv == 5) hipLaunchKernelGGL(k_tile_norms<5>, dim3(L.ntiles), dim3(256), 0, R.s, L.dt, L.db, L.arena, L.pm, L.dpartials, L.tile_rows);
-    else hipLaunchKernelGGL(k_tile_norms<6>, dim3(L.ntiles), dim3(256), 0, R.s, L.dt, L.db, L.arena, L.pm, L.dpartials, L.tile_rows);
+    if (nv == 4) hipLaunchKernelGGL(k_tile_norms<4>, dim3(L.ntiles), dim3(256), 0, R.s, L.dt, L.db, L.arena, L.pm, L.dpartials, L.tile_cols, L.tile_rows);
+    else if (nv == 5) hipLaunchKernelGGL(k_tile_norms<5>, dim3(L.ntiles), dim3(256), 0, R.s, L.dt, L.db, L.arena, L.pm, L.dpartials, L.tile_cols, L.tile_rows);
+    else hipLaunchKernelGGL(k_tile_norms<6>, dim3(L.ntiles), dim3(256), 0, R.s, L.dt, L.db, L.arena, L.pm, L.dpartials, L.tile_cols, L.tile_rows);
     HIPCHK(hipGetLastError());
   }
   // the per-tile partials hold this residual's norms; they are reduced when
@@ -851,9 +857,9 @@ int residual(int level, int force = 0, int set_un = 0) {
     a.sfac = R.residual_calls == 0 ? 1.0 / (3.0 * dtg) : 1.0 / (2.0 * dtg);
     hipLaunchKernelGGL(k_pointwise, dim3(L.nrows), dim3(256), 0, R.s, a);
     const int nv = L.o.nvar;
-    if (nv == 4) hipLaunchKernelGGL(k_tile_norms<4>, dim3(L.ntiles), dim3(256), 0, R.s, L.dt, L.db, L.arena, L.pm, L.dpartials, L.tile_rows);
-    else if (nv == 5) hipLaunchKernelGGL(k_tile_norms<5>, dim3(L.ntiles), dim3(256), 0, R.s, L.dt, L.db, L.arena, L.pm, L.dpartials, L.tile_rows);
-    else hipLaunchKernelGGL(k_tile_norms<6>, dim3(L.ntiles), dim3(256), 0, R.s, L.dt, L.db, L.arena, L.pm, L.dpartials, L.tile_rows);
+    if (nv == 4) hipLaunchKernelGGL(k_tile_norms<4>, dim3(L.ntiles), dim3(256), 0, R.s, L.dt, L.db, L.arena, L.pm, L.dpartials, L.tile_cols, L.tile_rows);
+    else if (nv == 5) hipLaunchKernelGGL(k_tile_norms<5>, dim3(L.ntiles), dim3(256), 0, R.s, L.dt, L.db, L.arena, L.pm, L.dpartials, L.tile_cols, L.tile_rows);
+    else hipLaunchKernelGGL(k_tile_norms<6>, dim3(L.ntiles), dim3(256), 0, R.s, L.dt, L.db, L.arena, L.pm, L.dpartials, L.tile_cols, L.tile_rows);
     rc = hipGetLastError() == hipSuccess ? 0 : fail("dual-time source launch failed");
   }
   R.residual_calls++;
@@ -1299,6 +1305,16 @@ int ec2d_set_tile_rows(int rows) {
   if (R.committed) return fail("tile shape must be set before ec2d_commit");
   if (rows != 1 && rows != 2 && rows != 4 && rows != 8) return fail("tile rows must be 1, 2, 4 or 8");
   R.tile_rows = rows;
+  R.tile_cols = 64;
+  return 0;
+}
+
+int ec2d_set_tile_shape(int cols, int rows) {
+  if (R.committed) return fail("tile shape must be set before ec2d_commit");
+  if (cols == 64) return ec2d_set_tile_rows(rows);
+  if (cols != 32 || rows != 8) return fail("tile shapes: 64 x 1, 2, 4 or 8, or 32 x 8");
+  R.tile_cols = 32;
+  R.tile_rows = 8;
   return 0;
 }
 
@@ -1763,6 +1779,7 @@ int ec2d_commit(void) {
     HIPCHK(hipMalloc(&L.dnorm, sizeof(double) * std::max<size_t>(norms.size(), 2)));
     if (!norms.empty()) HIPCHK(hipMemcpy(L.dnorm, norms.data(), sizeof(double) * norms.size(), hipMemcpyHostToDevice));
     L.tile_rows = R.tile_rows;
+    L.tile_cols = R.tile_cols;
     if (build_tiles(L)) return 1;
     std::vector<double> ncell;
     for (auto v : L.case_cells) ncell.push_back(1.0 * (double)v);
@@ -2329,10 +2346,12 @@ int ec2d_autotune(const char* cache_path, int reps, char* report, int report_len
       double* keep = nullptr;
       HIPCHK(hipMalloc(&keep, sizeof(double) * (size_t)L.arena_n));
       HIPCHK(hipMemcpyAsync(keep, L.arena, sizeof(double) * (size_t)L.arena_n, hipMemcpyDeviceToDevice, R.s));
-      // residual tile rows
+      // residual tile shape: 64 x 1, 2, 4 or 8 cells, or 32 x 8 (the cache
+      // holds the rows, plus 100 for 32 columns)
       double best = 1e30;
-      for (int tr : {1, 2, 4, 8}) {
-        L.tile_rows = tr;
+      for (int tr : {1, 2, 4, 8, 108}) {
+        L.tile_cols = tr >= 100 ? 32 : 64;
+        L.tile_rows = tr % 100;
         if (build_tiles(L)) return 1;
         double ms;
         if (time_launches(reps, [&] { return launch_residual(l, 0); }, &ms)) return 1;
@@ -2391,17 +2410,18 @@ int ec2d_autotune(const char* cache_path, int reps, char* report, int report_len
       HIPCHK(hipFree(keep));
       if (best_lp < 0) return fail("no LU-SGS layout fits this level");
       added.push_back({key, {best_tr, best_lp, best_sp, best_u}});
-      std::snprintf(line, sizeof(line), "level %d: measured tile_rows %d, lanes per cell %d, %s;", l + 1, best_tr,
-                    best_lp, shape_text(best_sp, best_u).c_str());
+      std::snprintf(line, sizeof(line), "level %d: measured tile %dx%d, lanes per cell %d, %s;", l + 1,
+                    best_tr >= 100 ? 32 : 64, best_tr % 100, best_lp, shape_text(best_sp, best_u).c_str());
       rep_txt += line;
       // LU-SGS times of every shape (w: workgroups per block, l: lanes per cell, u: steps per set)
       std::snprintf(line, sizeof(line), " LU-SGS ms:%s\n", cand.c_str());
     } else {
-      std::snprintf(line, sizeof(line), "level %d: cached tile_rows %d, lanes per cell %d, %s\n", l + 1, best_tr,
-                    best_lp, shape_text(best_sp, best_u).c_str());
+      std::snprintf(line, sizeof(line), "level %d: cached tile %dx%d, lanes per cell %d, %s\n", l + 1,
+                    best_tr >= 100 ? 32 : 64, best_tr % 100, best_lp, shape_text(best_sp, best_u).c_str());
     }
     rep_txt += line;
-    L.tile_rows = best_tr;
+    L.tile_cols = best_tr >= 100 ? 32 : 64;
+    L.tile_rows = best_tr % 100;
     if (build_tiles(L)) return 1;
     if (lusgs_layout(L, best_lp, best_sp == 0 ? 1 : best_sp, true)) return 1;
     L.lusgs_fused = best_sp == 0;

@@ -46,10 +46,12 @@ template <int TX, int TY, int NV, bool VISC, bool SA>
 __global__ __launch_bounds__(TX* TY) void k_residual(ResArgs a) {
   constexpr int RW = TX + 4, RH = TY + 4, NT = TX * TY;
   constexpr int NFI = (TX + 1) * TY, NFJ = TX * (TY + 1);
-  __shared__ double sd[RH][RW], su[RH][RW], sv[RH][RW], sp[RH][RW];
+  // viscous cases stage T and rebuild p = p00 d T from it (the staging's own
+  // expression, so the same value); the cell centres of the viscous
+  // gradients are read from their planes, not staged
+  __shared__ double sd[RH][RW], su[RH][RW], sv[RH][RW], sp[VISC ? 1 : RH][VISC ? 1 : RW];
   __shared__ double sT[VISC ? RH : 1][VISC ? RW : 1];
   __shared__ double sAm[VISC ? RH : 1][VISC ? RW : 1], sAt[VISC ? RH : 1][VISC ? RW : 1];
-  __shared__ double sx[VISC ? RH : 1][VISC ? RW : 1], sy[VISC ? RH : 1][VISC ? RW : 1];
   __shared__ double svt[SA ? RH : 1][SA ? RW : 1];
   __shared__ double sFi[4][TY][TX + 1], sFj[4][TY + 1][TX];
   __shared__ double sFv[SA ? 2 : 1][SA ? TY : 1][SA ? TX + 1 : 1], sFw[SA ? 2 : 1][SA ? TY + 1 : 1][SA ? TX : 1];
@@ -70,6 +72,18 @@ __global__ __launch_bounds__(TX* TY) void k_residual(ResArgs a) {
   const double* U[NV];
 #pragma unroll
   for (int m = 0; m < NV; m++) U[m] = PL(pm.U + m);
+  // staged values of local cell (y, x): pressure, and the cell centres
+  const double* X1p = PL(pm.x1);
+  const double* Y1p = PL(pm.y1);
+  auto SX = [&](int y, int x) { return X1p[pidx(B, i0 - 2 + x, j0 - 2 + y)]; };
+  auto SY = [&](int y, int x) { return Y1p[pidx(B, i0 - 2 + x, j0 - 2 + y)]; };
+  auto SP = [&](int y, int x) -> double {
+    if constexpr (VISC) return cp.p00 * sd[y][x] * sT[y][x];
+    else return sp[y][x];
+  };
+  auto SD = [&](int y, int x) { return sd[y][x]; };
+  auto SU = [&](int y, int x) { return su[y][x]; };
+  auto SV = [&](int y, int x) { return sv[y][x]; };
 
   // ---- 1. stage tile + halo ------------------------------------------------
   for (int idx = tid; idx < RW * RH; idx += NT) {
@@ -84,7 +98,7 @@ __global__ __launch_bounds__(TX* TY) void k_residual(ResArgs a) {
     sd[lj][li] = d;
     su[lj][li] = uu;
     sv[lj][li] = vv;
-    sp[lj][li] = cp.p00 * d * T;
+    if constexpr (!VISC) sp[lj][li] = cp.p00 * d * T;
     bool interior = (i >= 1 && i <= nx - 1 && j >= 1 && j <= ny - 1);
     if (interior && T < 1.0e-5) {
       if (a.cold) {
@@ -104,8 +118,6 @@ __global__ __launch_bounds__(TX* TY) void k_residual(ResArgs a) {
         if (interior) am = (am * cp.Re) / cp.Re;
       }
       sAm[lj][li] = am;
-      sx[lj][li] = PL(pm.x1)[q];
-      sy[lj][li] = PL(pm.y1)[q];
       double at = 0.0;
       if constexpr (SA) {
         double vt = U[4][q];
@@ -177,9 +189,9 @@ __global__ __launch_bounds__(TX* TY) void k_residual(ResArgs a) {
     if (!skip_fvm) {
     if (a.o.recon == 0) {
 #define RECON(ARR, m)                                                                                           \
-  UL[m] = scheme_fP(ARR[lj][li - 2], ARR[lj][li - 1], ARR[lj][li], ARR[lj][li + 1], a.o.scheme);              \
-  UR[m] = scheme_fm(ARR[lj][li - 2], ARR[lj][li - 1], ARR[lj][li], ARR[lj][li + 1], a.o.scheme);
-      RECON(sd, 0) RECON(su, 1) RECON(sv, 2) RECON(sp, 3)
+  UL[m] = scheme_fP(ARR(lj, li - 2), ARR(lj, li - 1), ARR(lj, li), ARR(lj, li + 1), a.o.scheme);              \
+  UR[m] = scheme_fm(ARR(lj, li - 2), ARR(lj, li - 1), ARR(lj, li), ARR(lj, li + 1), a.o.scheme);
+      RECON(SD, 0) RECON(SU, 1) RECON(SV, 2) RECON(SP, 3)
 #undef RECON
     } else {
       double U0[4][4];
@@ -220,10 +232,10 @@ __global__ __launch_bounds__(TX* TY) void k_residual(ResArgs a) {
         Amu1 = (sAm[lj][li - 1] + sAm[lj][li] + sAt[lj][li - 1] + sAt[lj][li]) * 0.5;
         Amk1 = cp.Cp * ((sAm[lj][li - 1] + sAm[lj][li]) / cp.Pr + (sAt[lj][li - 1] + sAt[lj][li]) / EC2D_PRT) * 0.5;
       }
-      double Dix = sx[lj][li] - sx[lj][li - 1];
-      double Diy = sy[lj][li] - sy[lj][li - 1];
-      double Djx = (sx[lj + 1][li - 1] + sx[lj + 1][li] - sx[lj - 1][li - 1] - sx[lj - 1][li]) * 0.25;
-      double Djy = (sy[lj + 1][li - 1] + sy[lj + 1][li] - sy[lj - 1][li - 1] - sy[lj - 1][li]) * 0.25;
+      double Dix = SX(lj, li) - SX(lj, li - 1);
+      double Diy = SY(lj, li) - SY(lj, li - 1);
+      double Djx = (SX(lj + 1, li - 1) + SX(lj + 1, li) - SX(lj - 1, li - 1) - SX(lj - 1, li)) * 0.25;
+      double Djy = (SY(lj + 1, li - 1) + SY(lj + 1, li) - SY(lj - 1, li - 1) - SY(lj - 1, li)) * 0.25;
       double Ds = 1.0 / (Dix * Djy - Djx * Diy);
       double Diu = su[lj][li] - su[lj][li - 1];
       double Div = sv[lj][li] - sv[lj][li - 1];
@@ -285,9 +297,9 @@ __global__ __launch_bounds__(TX* TY) void k_residual(ResArgs a) {
       // the j-direction original-variable reconstruction uses the global
       // scheme option, not the mesh's own (sub_Residual.f90:367)
 #define RECON(ARR, m)                                                                                              \
-  UL[m] = scheme_fP(ARR[lj - 2][li], ARR[lj - 1][li], ARR[lj][li], ARR[lj + 1][li], a.o.global_scheme);          \
-  UR[m] = scheme_fm(ARR[lj - 2][li], ARR[lj - 1][li], ARR[lj][li], ARR[lj + 1][li], a.o.global_scheme);
-      RECON(sd, 0) RECON(su, 1) RECON(sv, 2) RECON(sp, 3)
+  UL[m] = scheme_fP(ARR(lj - 2, li), ARR(lj - 1, li), ARR(lj, li), ARR(lj + 1, li), a.o.global_scheme);          \
+  UR[m] = scheme_fm(ARR(lj - 2, li), ARR(lj - 1, li), ARR(lj, li), ARR(lj + 1, li), a.o.global_scheme);
+      RECON(SD, 0) RECON(SU, 1) RECON(SV, 2) RECON(SP, 3)
 #undef RECON
     } else {
       double U0[4][4];
@@ -328,10 +340,10 @@ __global__ __launch_bounds__(TX* TY) void k_residual(ResArgs a) {
         Amu1 = (sAm[lj][li] + sAm[lj - 1][li] + sAt[lj][li] + sAt[lj - 1][li]) * 0.5;
         Amk1 = cp.Cp * ((sAm[lj - 1][li] + sAm[lj][li]) / cp.Pr + (sAt[lj - 1][li] + sAt[lj][li]) / EC2D_PRT) * 0.5;
       }
-      double Dix = (sx[lj - 1][li + 1] + sx[lj][li + 1] - sx[lj - 1][li - 1] - sx[lj][li - 1]) * 0.25;
-      double Diy = (sy[lj - 1][li + 1] + sy[lj][li + 1] - sy[lj - 1][li - 1] - sy[lj][li - 1]) * 0.25;
-      double Djx = sx[lj][li] - sx[lj - 1][li];
-      double Djy = sy[lj][li] - sy[lj - 1][li];
+      double Dix = (SX(lj - 1, li + 1) + SX(lj, li + 1) - SX(lj - 1, li - 1) - SX(lj, li - 1)) * 0.25;
+      double Diy = (SY(lj - 1, li + 1) + SY(lj, li + 1) - SY(lj - 1, li - 1) - SY(lj, li - 1)) * 0.25;
+      double Djx = SX(lj, li) - SX(lj - 1, li);
+      double Djy = SY(lj, li) - SY(lj - 1, li);
       double Ds = 1.0 / (Dix * Djy - Djx * Diy);
       double Diu = (su[lj - 1][li + 1] + su[lj][li + 1] - su[lj - 1][li - 1] - su[lj][li - 1]) * 0.25;
       double Div = (sv[lj - 1][li + 1] + sv[lj][li + 1] - sv[lj - 1][li - 1] - sv[lj][li - 1]) * 0.25;
@@ -399,10 +411,10 @@ __global__ __launch_bounds__(TX* TY) void k_residual(ResArgs a) {
         double v0 = (vtc + sAm[lj][li] / sd[lj][li]) * Cb2 / SA_sigma;
         double r5 = sFv[0][cj][ci + 1] - sFv[0][cj][ci] - v0 * (sFv[1][cj][ci + 1] - sFv[1][cj][ci]);
         r5 = r5 + sFw[0][cj + 1][ci] - sFw[0][cj][ci] - v0 * (sFw[1][cj + 1][ci] - sFw[1][cj][ci]);
-        double Dix = (sx[lj][li + 1] - sx[lj][li - 1]) * 0.5;
-        double Diy = (sy[lj][li + 1] - sy[lj][li - 1]) * 0.5;
-        double Djx = (sx[lj + 1][li] - sx[lj - 1][li]) * 0.5;
-        double Djy = (sy[lj + 1][li] - sy[lj - 1][li]) * 0.5;
+        double Dix = (SX(lj, li + 1) - SX(lj, li - 1)) * 0.5;
+        double Diy = (SY(lj, li + 1) - SY(lj, li - 1)) * 0.5;
+        double Djx = (SX(lj + 1, li) - SX(lj - 1, li)) * 0.5;
+        double Djy = (SY(lj + 1, li) - SY(lj - 1, li)) * 0.5;
         double Ds = 1.0 / (Dix * Djy - Djx * Diy);
         double Diu = (su[lj][li + 1] - su[lj][li - 1]) * 0.5;
         double Div = (sv[lj][li + 1] - sv[lj][li - 1]) * 0.5;

@@ -93,6 +93,7 @@ SIGNATURES = {
     "ec2d_closure_relaxation": [C.c_double],
     "ec2d_closure_monitor": [C.c_double, _ip],
     "ec2d_set_tile_rows": [_i],
+    "ec2d_set_tile_shape": [_i, _i],
 }
 
 _lib = None

@@ -124,11 +124,10 @@ class Case:
 class Batch:
     """All cases of one process; every solver call acts on all of them."""
 
-    def __init__(self, cases: List[Case], tile_rows: int = 4):
+    def __init__(self, cases: List[Case], tile_rows: int = 4, tile_cols: int = 64):
         self.L = capi.load()
         capi.call("ec2d_init")
-        if tile_rows != 4:
-            capi.call("ec2d_set_tile_rows", tile_rows)
+        capi.call("ec2d_set_tile_shape", tile_cols, tile_rows)
         self.cases = cases
         for c in cases:
             cid = self.L.ec2d_case_create(C.byref(c.c_control()))

@@ -13,7 +13,9 @@ and the device state is compared with the oracle's after each checked pass:
 * C3  64 blocks in the seeded 8x8 layout (33..257 x 33..129, 664,602 cells),
       Euler AUSM + MUSCL3, LU-SGS, tuned, graph: 3 passes, bit-exact.
 * C4  all 256 seeded SA-RANS airfoil cases (2,801,760 cells) in one batch,
-      tuned, graph: 2 passes.
+      tuned (the fused LU-SGS launch), graph: 2 passes.
+* C5  a held-out 513x129 airfoil with the learned closure as the
+      eddy-viscosity model: the residual given the network's mu_t, bit-exact.
 * The reference's RAE2822 sample (3 blocks, reversed wake-cut interface, SST,
   Van Leer, characteristic MUSCL3, LU-SGS CFL 10) from its golden fixture,
   whose oracle run tests/test_golden.py pins to the reference build: 3 passes.
@@ -158,6 +160,46 @@ def test_c4_ensemble_256_cases():
         _check(b, refs, batch, False, 1e-11, "2 passes")
         for ci in range(0, 256, 17):
             np.testing.assert_allclose(b.get("Amu_t", ci), refs[ci].block().Amu_t, rtol=1e-11, atol=1e-300)
+
+
+def test_c5_closure_model4_513x129_held_out():
+    """C5: the learned closure as the eddy-viscosity model (turbulence model
+    4) on a held-out 513 x 129 C-grid case (the first seed-1 ensemble
+    parameter set, as the closure study's held-out airfoils).  After five
+    passes, given the state and the network's mu_t field, the residual, time
+    step and spectral radii equal the oracle's bit for bit (the oracle takes
+    Amu_t from the caller and applies SA's wall-ghost rule to it)."""
+    import dataclasses
+
+    from test_gpu_closure import _trained_params
+
+    theta, norm = _trained_params("cfg_c5_train")
+    prm = dict(cases.ensemble_params(seed=1, n=32)[0], nx=513, ny=129)
+    d = cases.write_ensemble_case(case_dir("cfg_c5"), prm, iters=10)
+    path = os.path.join(d, "control.in")
+    F.write_control(path, dataclasses.replace(F.read_control(path), Iflag_turbulence_model=F.TURB_CLOSURE))
+    case = solver.Case.from_dir(d)
+    assert case.ctl.nvar == 4 and case.levels[0][0].nx == 513 and case.levels[0][0].ny == 129
+    with solver.Batch([case]) as b:
+        b.closure_set_params(theta, norm)
+        b.init_flow()
+        b.run(5)
+        b.comput_residual_one_mesh(1)
+        U = b.get("U")
+        mut = b.get("closure_mut")
+        got = {a: b.get(a) for a in ("Res", "dt", "Lci", "Lcj", "Lvi", "Lvj")}
+    assert np.all(np.isfinite(mut[1:-1, 1:-1]))
+    ref = O.OracleCase(d)
+    rb = ref.block(1, 1)
+    rb.U[...] = U
+    rb.Amu_t[...] = mut
+    ref.residual(1)
+    for a, g in got.items():
+        r = getattr(rb, a)
+        if a in ("Lci", "Lcj", "Lvi", "Lvj"):
+            g, r = g[:-1, :-1], r[:-1, :-1]
+        diff = first_diff(g, r)
+        assert diff is None, f"{a}: {diff}"
 
 
 def test_reference_sample_rae2822(tmp_path):

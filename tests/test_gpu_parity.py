@@ -22,11 +22,11 @@ pytestmark = pytest.mark.gpu
 
 
 def _compare_steps(d, nsteps, arrays=("U", "Res", "dt", "Lci", "Lcj"), exact=True, rtol=1e-12, option=None,
-                   tile_rows=4, norm_rtol=1e-12):
+                   tile_rows=4, norm_rtol=1e-12, tile_cols=64):
     ref = O.OracleCase(d)
     ref.init_flow()
     case = solver.Case.from_dir(d)
-    with solver.Batch([case], tile_rows=tile_rows) as b:
+    with solver.Batch([case], tile_rows=tile_rows, tile_cols=tile_cols) as b:
         b.init_flow()
         for mb in range(1, len(case.levels[0]) + 1):
             diff = first_diff(b.get("U", 0, 1, mb), ref.block(1, mb).U)
@@ -62,6 +62,41 @@ def test_euler_cgrid_roe_muscl3():
 def test_euler_tile_shapes(tile_rows):
     d = naca_case("g_euler_t", cases.control_euler_naca(iters=10))
     _compare_steps(d, 2, tile_rows=tile_rows)
+
+
+@pytest.mark.parametrize("kind", ["euler", "laminar", "sa", "sst", "multiblock", "fdm_sa", "mg_lusgs_sa"])
+def test_tile_32x8(kind):
+    """The 32 x 8 residual tile (viscous kernels rebuild p from staged d and
+    T and read the cell centres from their planes) against the oracle: tile
+    edges fall on other cells than with 64-wide tiles."""
+    arrays, exact, nsteps = ("U", "Res", "dt", "Lci", "Lcj"), True, 3
+    if kind == "euler":
+        d = naca_case("g_t32_e", cases.control_euler_naca(iters=10))
+    elif kind == "laminar":
+        c = cases.control_sa_naca(iters=10, Re=1e5, Ma=0.5)
+        c.Iflag_turbulence_model, c.Twall = F.TURB_NONE, 350.0
+        d, arrays = naca_case("g_t32_lam", c, ds1=1e-4), arrays + ("Lvi", "Lvj")
+    elif kind == "sa":
+        c = cases.control_sa_naca(iters=10, Re=1e5, Ma=0.5)
+        d, arrays, exact = naca_case("g_t32_sa", c, ds1=1e-4, nx=129, ny=49), arrays + ("Lvi", "Lvj"), False
+    elif kind == "sst":
+        c = _sst()
+        c.Twall = 300.0
+        d, arrays, exact = naca_case("g_t32_sst", c, ds1=1e-4), arrays + ("Amu_t",), False
+    elif kind == "multiblock":
+        c = cases.control_vortex(iters=10)
+        c.If_viscous, c.Re, c.Pr = 1, 1e4, 0.72
+        d = vortex_case("g_t32_mb", c)
+    elif kind == "fdm_sa":
+        c = cases.control_sa_naca(iters=10, Re=1e5, Ma=0.5)
+        d, arrays, exact = naca_case("g_t32_fdm", c, ds1=1e-4, fdm=(1,)), ("U", "Res", "dt"), False
+    else:
+        c = cases.control_sa_naca(iters=10, Re=1e5, Ma=0.5, num_mesh=2, time_method=F.TIME_LU_SGS)
+        d, arrays, exact = naca_case("g_t32_mg", c, ds1=1e-4, fdm=(1,)), ("U", "Res"), False
+    if exact:
+        _compare_steps(d, nsteps, arrays, tile_rows=8, tile_cols=32)
+    else:
+        _compare_steps(d, nsteps, arrays, exact=False, rtol=1e-11, tile_rows=8, tile_cols=32)
 
 
 SCHEMES = range(7)
