@@ -163,8 +163,9 @@ int ec2d_run(int n_iter);
  * first fine-level residual of each pass a device check reduces each case's
  * density Res_rms(1) (sub_Residual.f90:104-106) and stops the case
  * (status 1) once it is at most drop times its value at the run's first
- * check; a case whose rms is not finite stops with status 2 (the NaN guard;
- * the reference has none), and one checked max_iter times without reaching
+ * check; a case whose Res_rms of any equation (density, momenta, energy,
+ * turbulence) is not finite stops with status 2 (the NaN guard; the
+ * reference has none), and one checked max_iter times without reaching
  * the drop with status 3.  A stopped case is frozen: every kernel skips its
  * blocks, so its state stays the one whose residual met the test and the
  * other cases run on at the same cost per pass for fewer cells.  Per case

@@ -617,13 +617,23 @@ __global__ __launch_bounds__(256) void k_converge(const double* partials, const 
   const int c = blockIdx.x;
   if (active[c] == 0) return;
   const int t0 = case_tile0[c], t1 = case_tile0[c + 1];
-  __shared__ double sh[256];
-  double acc = 0.0;
-  for (int t = t0 + threadIdx.x; t < t1; t += blockDim.x) acc = acc + partials[(int64_t)t * 2 * nv + 1];
+  __shared__ double sh[256], sh_all[256];
+  // acc: the density sum of squares; all: every equation's (non-negative
+  // terms, so a NaN or an overflow in any equation leaves it non-finite)
+  double acc = 0.0, all = 0.0;
+  for (int t = t0 + threadIdx.x; t < t1; t += blockDim.x) {
+    const double* pt = partials + (int64_t)t * 2 * nv;
+    acc = acc + pt[1];
+    for (int m = 0; m < nv; m++) all = all + pt[2 * m + 1];
+  }
   sh[threadIdx.x] = acc;
+  sh_all[threadIdx.x] = all;
   __syncthreads();
   for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) sh[threadIdx.x] = sh[threadIdx.x] + sh[threadIdx.x + s];
+    if (threadIdx.x < s) {
+      sh[threadIdx.x] = sh[threadIdx.x] + sh[threadIdx.x + s];
+      sh_all[threadIdx.x] = sh_all[threadIdx.x] + sh_all[threadIdx.x + s];
+    }
     __syncthreads();
   }
   if (threadIdx.x == 0) {
@@ -633,7 +643,7 @@ __global__ __launch_bounds__(256) void k_converge(const double* partials, const 
     if (s.iters == 1) s.r0 = rms;
     s.last = rms;
     if (cold[c]) s.status = 4;  // the reference's T < 1e-5 stop, for this case only
-    else if (!isfinite(rms)) s.status = 2;
+    else if (!isfinite(rms) || !isfinite(sh_all[0])) s.status = 2;
     else if (rms <= drop * s.r0) s.status = 1;
     else if (s.iters >= max_iter) s.status = 3;
     if (s.status != 0) active[c] = 0;

@@ -147,3 +147,64 @@ def test_run_to_drop_cold_cell_stops_only_its_case():
             assert first_diff(b.get("U", k), oc.block().U) is None, k
         with pytest.raises(capi.Ec2dError, match=r"T <1.d-5 ! 8 4 1 \(case 1\)"):
             b.run(1)
+
+
+def test_sa_converged_cl_cd_match_oracle():
+    """The north star's bar on converged forces: an SA-RANS C-grid case (65 x
+    17, Re 1e5, M 0.5) run on the device to a 1e-8 density drop (about 12k
+    passes) has Cl and Cd within 1e-8 relative of the oracle's run of the same
+    number of updates (the device evaluates SA's exp / pow with its own math
+    library, 1e-11 relative per step, so the two runs are not bit-identical:
+    the converged fixed point is what they share)."""
+    c = cases.control_sa_naca(iters=10, Re=1e5, Ma=0.5)
+    d = naca_case("g_conv_sa", c, ds1=1e-4)
+    with solver.Batch([solver.Case.from_dir(d)]) as b:
+        b.init_flow()
+        iters, status, _ = b.run_to_drop(1e-8, max_iter=20000, check_every=200)
+        cl_g, cd_g, _ = b.force_coefficients(0)
+    assert status == [solver.Batch.CONVERGED], (iters, status)
+    oc = O.OracleCase(d)
+    oc.init_flow()
+    for _ in range(iters[0] - 1):
+        oc.main_step()
+    cl_o, cd_o = oc.force_coefficients()[:2]
+    assert abs(cl_g - cl_o) <= 1e-8 * abs(cl_o), (cl_g, cl_o)
+    assert abs(cd_g - cd_o) <= 1e-8 * abs(cd_o), (cd_g, cd_o)
+
+
+def test_nan_guard_sees_a_turbulence_equation(tmp_path):
+    """The reference's RAE2822 sample (SST, 3 blocks; isothermal wall, as the
+    fixture states) diverges in its own numerics: density stays near 4e-5
+    while the omega residual grows from the third pass and overflows, step for
+    step in the reference build and the oracle (tests/test_oracle_reference.py
+    ::test_rae2822_sample_omega_overflow).  The NaN guard must stop it with
+    status NOT_FINITE at the first check whose turbulence rms is not finite,
+    the oracle's check, although its density rms is finite there, and freeze
+    the case with the state that check saw: that of a plain run of
+    iters - 1 passes (NaN cells included)."""
+    import os
+
+    from test_golden import load_case
+
+    load_case(os.path.join(os.path.dirname(__file__), "golden", "rae2822_sample_sst_isothermal.npz"), str(tmp_path))
+    d = str(tmp_path)
+    oc = O.OracleCase(d)
+    oc.init_flow()
+    first_bad = None
+    for k in range(1, 41):
+        oc.main_step()
+        rms = oc.norms(1)[1]
+        if not np.all(np.isfinite(rms)):
+            first_bad = k
+            break
+    assert first_bad is not None and np.isfinite(rms[0]), (first_bad, rms)
+    with solver.Batch([solver.Case.from_dir(d)]) as b:
+        b.init_flow()
+        iters, status, _ = b.run_to_drop(1e-8, max_iter=200, check_every=5)
+        assert status == [solver.Batch.NOT_FINITE] and iters == [first_bad], (iters, status, first_bad)
+        U = [b.get("U", 0, 1, mb) for mb in (1, 2, 3)]
+    with solver.Batch([solver.Case.from_dir(d)]) as b:
+        b.init_flow()
+        b.run(first_bad - 1)
+        for mb in (1, 2, 3):
+            assert first_diff(b.get("U", 0, 1, mb), U[mb - 1]) is None, f"block {mb}"

@@ -396,3 +396,20 @@ def test_cold_cell_stop_matches_reference():
     with pytest.raises(O.SolverError) as e:
         oc.residual(1)
     assert str(e.value).startswith(f"T <1.d-5 ! {i} {j} {mb}"), str(e.value)
+
+
+def test_rae2822_sample_omega_overflow(tmp_path):
+    """The reference's RAE2822 sample (SST, 3 blocks, LU-SGS CFL 10; the
+    fixture's isothermal wall) does not converge in the reference's own
+    numerics: the omega residual grows from the third pass and overflows at
+    the thirteenth while density stays near 4e-5.  The oracle follows the
+    reference build bit for bit (U, Res, dt, mu_t of every block and the
+    norms) through the overflow, so the divergence is the reference's, which
+    the device's NaN guard must then catch (tests/test_gpu_converge.py)."""
+    from test_golden import load_case
+
+    load_case(os.path.join(os.path.dirname(__file__), "golden", "rae2822_sample_sst_isothermal.npz"), str(tmp_path))
+    R, oc = run_both(str(tmp_path), 13, ("U", "Res", "dt", "Amu_t"))
+    rms = oc.norms(1)[1]
+    assert np.all(np.isfinite(rms[:5])) and rms[5] == np.inf, rms
+    assert 1e-5 < rms[0] < 1e-4, rms
