@@ -677,10 +677,15 @@ int launch_lusgs(int level, const int* dlist, int nblk, double alfa1, int update
   a.gstride = L.gstride;
   a.err = R.dsweep_err;
   a.spin_max = R.wait_budget;
+  a.nblk = nblk;
+  a.sk_tiles = std::max(L.skew_tiles, 1);
   bool visc = R.cases[0].cp.If_viscous == 1;
   // prep and scatter grid: tiles of 32 steps x 8 rows of the largest block's skewed space
   dim3 gprep((unsigned)L.prep_tiles, nblk), gsweep(nblk * L.lusgs_split), blk(L.lusgs_threads);
-  dim3 gscat((unsigned)L.prep_tiles, nblk), gskew((unsigned)L.skew_tiles, nblk);
+  dim3 gscat((unsigned)L.prep_tiles, nblk);
+  // k_lusgs_skew: a one-dimensional grid of 8 equal XCD chunks of the
+  // (block, tile) items
+  dim3 gskew((unsigned)(8 * (((int64_t)a.sk_tiles * nblk + 7) / 8)));
   int nv = L.o.nvar;
   // the instantiation is chosen by thread count (its launch bound): up to 4
   // waves one wave per SIMD and two steps per register set; more waves (up

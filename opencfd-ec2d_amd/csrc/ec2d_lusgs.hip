@@ -76,6 +76,8 @@ struct LusgsArgs {
   int64_t gstride;        // sweep, split: slots of one ring (at least npad * 2 * NV)
   int* err;               // sweep, split: set when a cross-workgroup wait ran out of its budget
   int spin_max;           // the budget of a bounded wait, in polls
+  int nblk;               // skew: blocks of this launch
+  int sk_tiles;           // skew: 16 x 16 tiles per block (the largest block's count)
 };
 
 // Field layout of one record set (one sweep direction) of a block.  A
@@ -443,6 +445,11 @@ __global__ __launch_bounds__(256) void k_lusgs_prep(LusgsArgs a) {
 // planes a residual and an update change.  A workgroup moves a tile of 16
 // steps x 16 rows through LDS: 16 lanes read 16 consecutive cells of a row,
 // and 16 lanes write 16 consecutive positions of a step (128 bytes each).
+// A row's 16 cells start at column s0 - r, so they straddle two cache lines
+// whose other halves the neighbouring tiles of the row read: the grid is
+// one-dimensional and XCD-aware (workgroup w runs on XCD w % 8, and XCD x
+// takes the contiguous run [x chunk, (x + 1) chunk) of the (block, tile)
+// items), so that neighbouring tiles fetch their shared lines into one L2.
 template <int NV, bool VISC>
 __host__ __device__ constexpr bool skew_plane(int k, bool stat) {
   using I = RecIn;
@@ -461,13 +468,17 @@ __global__ __launch_bounds__(256) void k_lusgs_skew(LusgsArgs a) {
     return n;
   }();
   __shared__ double tile[NC][16][17];  // [plane][row][step], padded against bank conflicts
-  const BlockDesc B = a.blocks[a.block_list[blockIdx.y]];
+  const int chunk = (int)gridDim.x / 8;  // the grid is 8 chunk workgroups
+  const int w = ((int)blockIdx.x % 8) * chunk + (int)blockIdx.x / 8;
+  const int bi = w / a.sk_tiles, ti = w % a.sk_tiles;
+  if (bi >= a.nblk) return;
+  const BlockDesc B = a.blocks[a.block_list[bi]];
   if (case_frozen(B.case_id)) return;
   const int ncol = B.nx - 1, nrow = B.ny - 1;
   const int S = B.sk_S;
   const int tiles_s = (S + 15) / 16;
-  if ((int)blockIdx.x >= tiles_s * ((nrow + 15) / 16)) return;  // the launch fits the level's largest block
-  const int s0 = (blockIdx.x % tiles_s) * 16, r0 = (blockIdx.x / tiles_s) * 16;
+  if (ti >= tiles_s * ((nrow + 15) / 16)) return;  // the launch fits the level's largest block
+  const int s0 = (ti % tiles_s) * 16, r0 = (ti / tiles_s) * 16;
   const int tid = threadIdx.x;
   const double* __restrict__ base = a.arena + B.base;
   // load: 16 lanes of a row read 16 consecutive cells (step s, cell c = s - r)
