@@ -904,7 +904,8 @@ __global__ __launch_bounds__(MAXT) void k_lusgs_sweep(LusgsArgs a) {
 // ob_taken = chunk + 1; the sweep wave waits for ob_taken >= chunk before
 // it writes the chunk's first step, and sets ob_ready after its last.
 // Protocol: produced[p] counts the steps producer p has written, consumed
-// the steps the sweep wave has read.  The sweep wave waits, before step g,
+// the steps whose slot the sweep wave has finished with (it releases slot g
+// after computing step g).  The sweep wave waits, before reading step g,
 // for produced[g % NP] > g / NP; producer p waits, before writing step g
 // into slot g % NRING, for consumed >= g - NRING + 1.  A producer waits only
 // for the sweep wave and the sweep wave only for producers that are never
@@ -1056,44 +1057,59 @@ __global__ __launch_bounds__(64 * (2 + NP)) void k_lusgs_fused(LusgsArgs a) {
   // one direction of the sweep; the direction is a compile-time constant so
   // that every register array keeps fixed indices.  The next step's record
   // is read from its slot while this step computes; in the backward sweep
-  // each step's X (the forward dU) is loaded two steps ahead, into one of
-  // two register sets (the loop runs two steps per turn: npad is even).
+  // each step's X (the forward dU) is loaded XD steps ahead, into one of XD
+  // register sets (the loop runs XD steps per turn: npad is a multiple of 4).
   auto run_dir = [&](auto fwd_c) {
     constexpr bool fwd = decltype(fwd_c)::value;
     constexpr int dir = fwd ? 0 : 1;
     const int gs0 = dir * npad;
     Ahead<NV> ah = {};
     Out<NV> own = {};
-    double xa[2 * F::XP] = {}, xb[2 * F::XP] = {};
+    constexpr int XD = 4;
+    double xs[XD][2 * F::XP] = {};
     if (!fwd) {
       // the forward sweep's stores to the X pairs before these loads
       __threadfence();
-      load_x(xa, 0);
-      load_x(xb, 1);
+#pragma unroll
+      for (int k = 0; k < XD; k++) load_x(xs[k], k);
     }
     Rec<NV, 2> rn;
     if (ok) ok = wait_bounded(&L.produced[gs0 % FPROD], gs0 / FPROD + 1, a.err, a.spin_max);
     load_lds(rn, gs0 % FRING, fwd);
+    int pv = -1;  // the count of the next step's producer, read one step earlier (-1: not read)
     auto one_step = [&](int st, double* x) {
       const int gs = gs0 + st;
       Rec<NV, 2> r = rn;
+      // The next step's record.  pv was read a step ago, so the wait (an LDS
+      // round trip on the chain) is taken only when that read found the
+      // record missing.  Both loads below are unconditional: a load under a
+      // branch makes its registers a phi, which the compiler resolves with
+      // copies that wait for the load in flight and so undo the prefetch.
+      // After the direction's last step the slot read is not waited for and
+      // its values are never used (the next direction reloads after its own
+      // wait), and load_x clamps the step to S - 1.
+      if (st + 1 < npad && ok && pv <= (gs + 1) / FPROD)
+        ok = wait_bounded(&L.produced[(gs + 1) % FPROD], (gs + 1) / FPROD + 1, a.err, a.spin_max);
       __atomic_signal_fence(__ATOMIC_SEQ_CST);
-      __builtin_amdgcn_s_waitcnt(0xc07f);  // the slot's LDS reads done before it is released
-      lds_put(&L.consumed, gs + 1);
-      if (st + 1 < npad) {
-        if (ok) ok = wait_bounded(&L.produced[(gs + 1) % FPROD], (gs + 1) / FPROD + 1, a.err, a.spin_max);
-        load_lds(rn, (gs + 1) % FRING, fwd);
-      }
+      load_lds(rn, (gs + 1) % FRING, fwd);
+      pv = lds_get(&L.produced[(gs + 2) % FPROD]);
       if constexpr (!fwd) {
 #pragma unroll
         for (int m = 0; m < NV; m++) r.X[m] = x[m];
-        if (st + 2 < npad) load_x(x, st + 2);
+        load_x(x, st + XD);
       }
       // forward: dU to the backward records' X pairs; backward: dU and
       // U + dU (the record's U is the plane's value) into the output buffer,
       // which the writer wave stores four steps at a time
       sweep_step<NV, VISC, fwd, 1, fwd>(r, own, rs, bw, n16, st, S, R, ncol, nrow, t, 0, lane, 0, false, false, g, ch,
                                         ah);
+      // Slot gs is released after the step: its reads were issued a step ago
+      // and the next slot's at this step's start, so the wait costs nothing
+      // here, and it comes before the output buffer's writes so that it does
+      // not wait for them.
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);
+      __builtin_amdgcn_s_waitcnt(0xc07f);
+      lds_put(&L.consumed, gs + 1);
       if constexpr (!fwd) {
         // npad is a multiple of four: every chunk has four steps
         const int b4 = st & 3, ck = st >> 2;
@@ -1110,9 +1126,9 @@ __global__ __launch_bounds__(64 * (2 + NP)) void k_lusgs_fused(LusgsArgs a) {
         }
       }
     };
-    for (int st = 0; st < npad; st += 2) {
-      one_step(st, xa);
-      one_step(st + 1, xb);
+    for (int st = 0; st < npad; st += XD) {
+#pragma unroll
+      for (int k = 0; k < XD; k++) one_step(st + k, xs[k]);
     }
   };
   run_dir(std::true_type{});
