@@ -588,7 +588,19 @@ int launch_residual(int level, int clip, int force = 0, int set_un = 0) {
     hipLaunchKernelGGL(k_sst_cell, dim3(L.nrows), dim3(256), 0, R.s, sa);
     hipLaunchKernelGGL(k_sst_res, dim3(L.nrows), dim3(256), 0, R.s, sa);
   }
-#define RLAUNCH(TX, TY, NV, V, S) hipLaunchKernelGGL((k_residual<TX, TY, NV, V, S>), grid, dim3(TX * TY), 0, R.s, a)
+  // the benched options (MUSCL3 / van Albada on both face directions, Roe,
+  // original reconstruction) as compile-time constants, for the Euler and SA
+  // equation sets
+  const bool fix = L.o.scheme == 5 && L.o.global_scheme == 5 && L.o.flux == 4 && L.o.recon == 0 &&
+                   ((nv == 4 && !visc) || (nv == 5 && turb == 2));
+#define RLAUNCH(TX, TY, NV, V, S)                                                                  \
+  do {                                                                                            \
+    if (fix && ((NV == 4 && !V) || (NV == 5 && S)))                                               \
+      hipLaunchKernelGGL((k_residual<TX, TY, NV, V, S, (NV == 4 && !V) || (NV == 5 && S)>), grid, \
+                         dim3(TX * TY), 0, R.s, a);                                               \
+    else                                                                                          \
+      hipLaunchKernelGGL((k_residual<TX, TY, NV, V, S, false>), grid, dim3(TX * TY), 0, R.s, a);  \
+  } while (0)
 #define RBY_TY(NV, V, S)                        \
   if (L.tile_cols == 32) {                      \
     RLAUNCH(32, 8, NV, V, S);                   \

@@ -42,9 +42,15 @@ struct ResArgs {
                               // 3 (4 equations, Baldwin-Lomax): mu_t of every ring cell from Amu_t as k_bl left it
 };
 
-template <int TX, int TY, int NV, bool VISC, bool SA>
+// FIX: the level's scheme, flux and reconstruction are MUSCL3 with the van
+// Albada limiter on both face directions, Roe and the original (primitive)
+// reconstruction, as compile-time constants, so that the compiler drops the
+// other branches of every face; the launch picks it when the options match.
+template <int TX, int TY, int NV, bool VISC, bool SA, bool FIX>
 __global__ __launch_bounds__(TX* TY) void k_residual(ResArgs a) {
   constexpr int RW = TX + 4, RH = TY + 4, NT = TX * TY;
+  const int o_scheme = FIX ? 5 : a.o.scheme, o_gscheme = FIX ? 5 : a.o.global_scheme;
+  const int o_flux = FIX ? 4 : a.o.flux, o_recon = FIX ? 0 : a.o.recon;
   constexpr int NFI = (TX + 1) * TY, NFJ = TX * (TY + 1);
   // viscous cases stage T and rebuild p = p00 d T from it (the staging's own
   // expression, so the same value); the cell centres of the viscous
@@ -187,10 +193,10 @@ __global__ __launch_bounds__(TX* TY) void k_residual(ResArgs a) {
     double UL[4], UR[4];
     double F1 = 0.0, F2 = 0.0, F3 = 0.0, F4 = 0.0;
     if (!skip_fvm) {
-    if (a.o.recon == 0) {
+    if (o_recon == 0) {
 #define RECON(ARR, m)                                                                                           \
-  UL[m] = scheme_fP(ARR(lj, li - 2), ARR(lj, li - 1), ARR(lj, li), ARR(lj, li + 1), a.o.scheme);              \
-  UR[m] = scheme_fm(ARR(lj, li - 2), ARR(lj, li - 1), ARR(lj, li), ARR(lj, li + 1), a.o.scheme);
+  UL[m] = scheme_fP(ARR(lj, li - 2), ARR(lj, li - 1), ARR(lj, li), ARR(lj, li + 1), o_scheme);              \
+  UR[m] = scheme_fm(ARR(lj, li - 2), ARR(lj, li - 1), ARR(lj, li), ARR(lj, li + 1), o_scheme);
       RECON(SD, 0) RECON(SU, 1) RECON(SV, 2) RECON(SP, 3)
 #undef RECON
     } else {
@@ -199,22 +205,22 @@ __global__ __launch_bounds__(TX* TY) void k_residual(ResArgs a) {
         int64_t qk = pidx(B, i - 2 + k, j);
         for (int m = 0; m < 4; m++) U0[k][m] = U[m][qk];
       }
-      if (a.o.recon == 1) {
+      if (o_recon == 1) {
         double QL[4], QR[4];
         for (int m = 0; m < 4; m++) {
-          QL[m] = scheme_fP(U0[0][m], U0[1][m], U0[2][m], U0[3][m], a.o.scheme);
-          QR[m] = scheme_fm(U0[0][m], U0[1][m], U0[2][m], U0[3][m], a.o.scheme);
+          QL[m] = scheme_fP(U0[0][m], U0[1][m], U0[2][m], U0[3][m], o_scheme);
+          QR[m] = scheme_fm(U0[0][m], U0[1][m], U0[2][m], U0[3][m], o_scheme);
         }
         cons_to_prim(QL, UL, g);
         cons_to_prim(QR, UR, g);
       } else {
-        recon_characteristic(U0, UL, UR, g, a.o.scheme);
+        recon_characteristic(U0, UL, UR, g, o_scheme);
       }
     }
     double QL[4], QR[4], F0[4];
     QL[0] = UL[0]; QL[1] = UL[1] * ni1 + UL[2] * ni2; QL[2] = -UL[1] * ni2 + UL[2] * ni1; QL[3] = UL[3];
     QR[0] = UR[0]; QR[1] = UR[1] * ni1 + UR[2] * ni2; QR[2] = -UR[1] * ni2 + UR[2] * ni1; QR[3] = UR[3];
-    face_flux(a.o.flux, QL, QR, F0, g);
+    face_flux(o_flux, QL, QR, F0, g);
     F1 = -F0[0] * si;
     F2 = -(F0[1] * ni1 - F0[2] * ni2) * si;
     F3 = -(F0[1] * ni2 + F0[2] * ni1) * si;
@@ -293,12 +299,12 @@ __global__ __launch_bounds__(TX* TY) void k_residual(ResArgs a) {
     double UL[4], UR[4];
     double F1 = 0.0, F2 = 0.0, F3 = 0.0, F4 = 0.0;
     if (!skip_fvm) {
-    if (a.o.recon == 0) {
+    if (o_recon == 0) {
       // the j-direction original-variable reconstruction uses the global
       // scheme option, not the mesh's own (sub_Residual.f90:367)
 #define RECON(ARR, m)                                                                                              \
-  UL[m] = scheme_fP(ARR(lj - 2, li), ARR(lj - 1, li), ARR(lj, li), ARR(lj + 1, li), a.o.global_scheme);          \
-  UR[m] = scheme_fm(ARR(lj - 2, li), ARR(lj - 1, li), ARR(lj, li), ARR(lj + 1, li), a.o.global_scheme);
+  UL[m] = scheme_fP(ARR(lj - 2, li), ARR(lj - 1, li), ARR(lj, li), ARR(lj + 1, li), o_gscheme);          \
+  UR[m] = scheme_fm(ARR(lj - 2, li), ARR(lj - 1, li), ARR(lj, li), ARR(lj + 1, li), o_gscheme);
       RECON(SD, 0) RECON(SU, 1) RECON(SV, 2) RECON(SP, 3)
 #undef RECON
     } else {
@@ -307,22 +313,22 @@ __global__ __launch_bounds__(TX* TY) void k_residual(ResArgs a) {
         int64_t qk = pidx(B, i, j - 2 + k);
         for (int m = 0; m < 4; m++) U0[k][m] = U[m][qk];
       }
-      if (a.o.recon == 1) {
+      if (o_recon == 1) {
         double QL[4], QR[4];
         for (int m = 0; m < 4; m++) {
-          QL[m] = scheme_fP(U0[0][m], U0[1][m], U0[2][m], U0[3][m], a.o.scheme);
-          QR[m] = scheme_fm(U0[0][m], U0[1][m], U0[2][m], U0[3][m], a.o.scheme);
+          QL[m] = scheme_fP(U0[0][m], U0[1][m], U0[2][m], U0[3][m], o_scheme);
+          QR[m] = scheme_fm(U0[0][m], U0[1][m], U0[2][m], U0[3][m], o_scheme);
         }
         cons_to_prim(QL, UL, g);
         cons_to_prim(QR, UR, g);
       } else {
-        recon_characteristic(U0, UL, UR, g, a.o.scheme);
+        recon_characteristic(U0, UL, UR, g, o_scheme);
       }
     }
     double QL[4], QR[4], F0[4];
     QL[0] = UL[0]; QL[1] = UL[1] * nj1 + UL[2] * nj2; QL[2] = -UL[1] * nj2 + UL[2] * nj1; QL[3] = UL[3];
     QR[0] = UR[0]; QR[1] = UR[1] * nj1 + UR[2] * nj2; QR[2] = -UR[1] * nj2 + UR[2] * nj1; QR[3] = UR[3];
-    face_flux(a.o.flux, QL, QR, F0, g);
+    face_flux(o_flux, QL, QR, F0, g);
     F1 = -F0[0] * sj;
     F2 = -(F0[1] * nj1 - F0[2] * nj2) * sj;
     F3 = -(F0[1] * nj2 + F0[2] * nj1) * sj;
@@ -602,7 +608,8 @@ __global__ void k_norms(const double* partials, const int* case_tile0, const dou
 // value equals Res_rms(1) of ec2d_res_norms) and decides whether the case
 // goes on.  The first check of a run records the reference value r0.  A case
 // stops (active = 0, its blocks frozen by every later kernel) when its rms
-// falls to drop * r0 (status 1), is not finite (status 2: the NaN guard), or
+// falls to drop * r0 (status 1), when the rms of any of its equations is not
+// finite (status 2: the NaN guard), or
 // has been checked max_iter times (status 3).  iters counts the checks, i.e.
 // the residual evaluations of the case's state; the state it stops in has had
 // iters - 1 updates in this run.
