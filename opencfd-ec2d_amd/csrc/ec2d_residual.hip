@@ -59,8 +59,16 @@ __global__ __launch_bounds__(TX* TY) void k_residual(ResArgs a) {
   __shared__ double sT[VISC ? RH : 1][VISC ? RW : 1];
   __shared__ double sAm[VISC ? RH : 1][VISC ? RW : 1], sAt[VISC ? RH : 1][VISC ? RW : 1];
   __shared__ double svt[SA ? RH : 1][SA ? RW : 1];
-  __shared__ double sFi[4][TY][TX + 1], sFj[4][TY + 1][TX];
-  __shared__ double sFv[SA ? 2 : 1][SA ? TY : 1][SA ? TX + 1 : 1], sFw[SA ? 2 : 1][SA ? TY + 1 : 1][SA ? TX : 1];
+  // face fluxes: the i-faces' (sFi, SA: sFv) and, after every cell has taken
+  // its i-face difference, the j-faces' (sFj, sFw) in the same LDS, so that
+  // four workgroups of a 32 x 8 SA tile fit on a CU
+  constexpr int NIF = 4 * TY * (TX + 1) + (SA ? 2 * TY * (TX + 1) : 1);
+  constexpr int NJF = 4 * (TY + 1) * TX + (SA ? 2 * (TY + 1) * TX : 1);
+  __shared__ double sF[NIF > NJF ? NIF : NJF];
+  auto& sFi = *reinterpret_cast<double(*)[4][TY][TX + 1]>(sF);
+  auto& sFv = *reinterpret_cast<double(*)[SA ? 2 : 1][SA ? TY : 1][SA ? TX + 1 : 1]>(sF + 4 * TY * (TX + 1));
+  auto& sFj = *reinterpret_cast<double(*)[4][TY + 1][TX]>(sF);
+  auto& sFw = *reinterpret_cast<double(*)[SA ? 2 : 1][SA ? TY + 1 : 1][SA ? TX : 1]>(sF + 4 * (TY + 1) * TX);
   __shared__ double sred[2 * NV][NT / 64];
 
   const int lt = a.tile_perm[blockIdx.x];
@@ -283,6 +291,30 @@ __global__ __launch_bounds__(TX* TY) void k_residual(ResArgs a) {
     sFi[2][fj][fi] = F3;
     sFi[3][fj][fi] = F4;
   }
+  __syncthreads();
+
+  // ---- 3a. each cell's i-face difference (the first terms of Res, in the
+  // reference's order), before the j-faces take over the LDS --------------
+  double R[NV];
+  double r5 = 0.0;
+  {
+    const int ci = tid % TX, cj = tid / TX;
+    const int i = i0 + ci, j = j0 + cj;
+    if (i <= nx - 1 && j <= ny - 1) {
+      const bool fdm_cell = fdm && (i >= LF + 1 && i <= nx - LF - 1) && (j >= LF + 1 && j <= ny - LF - 1);
+      if (!fdm_cell) {
+#pragma unroll
+        for (int m = 0; m < 4; m++) R[m] = sFi[m][cj][ci + 1] - sFi[m][cj][ci];
+      }
+      if constexpr (SA) {
+        const double SA_sigma = 2.0 / 3.0, Cb2 = 0.622;
+        const int li = ci + 2, lj = cj + 2;
+        const double v0 = (svt[lj][li] + sAm[lj][li] / sd[lj][li]) * Cb2 / SA_sigma;
+        r5 = sFv[0][cj][ci + 1] - sFv[0][cj][ci] - v0 * (sFv[1][cj][ci + 1] - sFv[1][cj][ci]);
+      }
+    }
+  }
+  __syncthreads();
   // ---- 2b. j-faces ---------------------------------------------------------
   // NFJ = NT + 64: the 64 faces beyond one per thread go to the last wave
   // (the first wave already takes the TY i-faces beyond one per thread), so
@@ -403,11 +435,10 @@ __global__ __launch_bounds__(TX* TY) void k_residual(ResArgs a) {
     if (i <= nx - 1 && j <= ny - 1) {
       int64_t q = pidx(B, i, j);
       int li = ci + 2, lj = cj + 2;
-      double R[NV];
       bool fdm_cell = fdm && (i >= LF + 1 && i <= nx - LF - 1) && (j >= LF + 1 && j <= ny - LF - 1);
       if (!fdm_cell) {
 #pragma unroll
-        for (int m = 0; m < 4; m++) R[m] = sFi[m][cj][ci + 1] - sFi[m][cj][ci] + sFj[m][cj + 1][ci] - sFj[m][cj][ci];
+        for (int m = 0; m < 4; m++) R[m] = R[m] + sFj[m][cj + 1][ci] - sFj[m][cj][ci];
       }
       if constexpr (SA) {
         const double SA_sigma = 2.0 / 3.0, Cv1 = 7.1, Cb1 = 0.1355, Cb2 = 0.622, SA_k = 0.41;
@@ -415,7 +446,6 @@ __global__ __launch_bounds__(TX* TY) void k_residual(ResArgs a) {
         const double Cv1_3 = Cv1 * Cv1 * Cv1, Cw3_6 = 2.0 * 2.0 * 2.0 * 2.0 * 2.0 * 2.0;
         double vtc = svt[lj][li];
         double v0 = (vtc + sAm[lj][li] / sd[lj][li]) * Cb2 / SA_sigma;
-        double r5 = sFv[0][cj][ci + 1] - sFv[0][cj][ci] - v0 * (sFv[1][cj][ci + 1] - sFv[1][cj][ci]);
         r5 = r5 + sFw[0][cj + 1][ci] - sFw[0][cj][ci] - v0 * (sFw[1][cj + 1][ci] - sFw[1][cj][ci]);
         double Dix = (SX(lj, li + 1) - SX(lj, li - 1)) * 0.5;
         double Diy = (SY(lj, li + 1) - SY(lj, li - 1)) * 0.5;
