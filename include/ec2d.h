@@ -331,9 +331,9 @@ int ec2d_autotune(const char *cache_path, int reps, char *report, int report_len
 /* Tile shape of the fused residual kernel (rows x 64 cells); no reference
  * counterpart.  Call before ec2d_commit.  Valid rows: 1, 2, 4, 8. */
 int ec2d_set_tile_rows(int rows);
-/* The same with the width: 64 x 1, 2, 4 or 8 cells, or 32 x 8 (whose smaller
- * halo lets three workgroups share a CU in the viscous kernels).  Results do
- * not depend on it (the residual norms' summation tree does, at round-off). */
+/* The same with the width: 64 x 1, 2, 4 or 8 cells, or 32 x 8 or 32 x 16
+ * (smaller halos per workgroup for the viscous kernels).  Results do not
+ * depend on it (the residual norms' summation tree does, at round-off). */
 int ec2d_set_tile_shape(int cols, int rows);
 
 #ifdef __cplusplus

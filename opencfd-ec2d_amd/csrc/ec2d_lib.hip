@@ -603,7 +603,10 @@ int launch_residual(int level, int clip, int force = 0, int set_un = 0) {
   } while (0)
 #define RBY_TY(NV, V, S)                        \
   if (L.tile_cols == 32) {                      \
-    RLAUNCH(32, 8, NV, V, S);                   \
+    if (L.tile_rows == 16)                      \
+      RLAUNCH(32, 16, NV, V, S);                \
+    else                                        \
+      RLAUNCH(32, 8, NV, V, S);                 \
   } else {                                      \
     switch (L.tile_rows) {                      \
       case 1: RLAUNCH(64, 1, NV, V, S); break;  \
@@ -1329,9 +1332,9 @@ int ec2d_set_tile_rows(int rows) {
 int ec2d_set_tile_shape(int cols, int rows) {
   if (R.committed) return fail("tile shape must be set before ec2d_commit");
   if (cols == 64) return ec2d_set_tile_rows(rows);
-  if (cols != 32 || rows != 8) return fail("tile shapes: 64 x 1, 2, 4 or 8, or 32 x 8");
+  if (cols != 32 || (rows != 8 && rows != 16)) return fail("tile shapes: 64 x 1, 2, 4 or 8, or 32 x 8 or 16");
   R.tile_cols = 32;
-  R.tile_rows = 8;
+  R.tile_rows = rows;
   return 0;
 }
 
@@ -2363,10 +2366,10 @@ int ec2d_autotune(const char* cache_path, int reps, char* report, int report_len
       double* keep = nullptr;
       HIPCHK(hipMalloc(&keep, sizeof(double) * (size_t)L.arena_n));
       HIPCHK(hipMemcpyAsync(keep, L.arena, sizeof(double) * (size_t)L.arena_n, hipMemcpyDeviceToDevice, R.s));
-      // residual tile shape: 64 x 1, 2, 4 or 8 cells, or 32 x 8 (the cache
-      // holds the rows, plus 100 for 32 columns)
+      // residual tile shape: 64 x 1, 2, 4 or 8 cells, or 32 x 8 or 16 (the
+      // cache holds the rows, plus 100 for 32 columns)
       double best = 1e30;
-      for (int tr : {1, 2, 4, 8, 108}) {
+      for (int tr : {1, 2, 4, 8, 108, 116}) {
         L.tile_cols = tr >= 100 ? 32 : 64;
         L.tile_rows = tr % 100;
         if (build_tiles(L)) return 1;

@@ -64,11 +64,13 @@ def test_euler_tile_shapes(tile_rows):
     _compare_steps(d, 2, tile_rows=tile_rows)
 
 
+@pytest.mark.parametrize("rows", [8, 16])
 @pytest.mark.parametrize("kind", ["euler", "laminar", "sa", "sst", "multiblock", "fdm_sa", "mg_lusgs_sa"])
-def test_tile_32x8(kind):
-    """The 32 x 8 residual tile (viscous kernels rebuild p from staged d and
-    T and read the cell centres from their planes) against the oracle: tile
-    edges fall on other cells than with 64-wide tiles."""
+def test_tile_32_columns(kind, rows):
+    """The 32 x 8 and 32 x 16 residual tiles (viscous kernels rebuild p from
+    staged d and T and read the cell centres from their planes; the j-face
+    fluxes reuse the i-faces' LDS) against the oracle: tile edges fall on
+    other cells than with 64-wide tiles."""
     arrays, exact, nsteps = ("U", "Res", "dt", "Lci", "Lcj"), True, 3
     if kind == "euler":
         d = naca_case("g_t32_e", cases.control_euler_naca(iters=10))
@@ -94,9 +96,9 @@ def test_tile_32x8(kind):
         c = cases.control_sa_naca(iters=10, Re=1e5, Ma=0.5, num_mesh=2, time_method=F.TIME_LU_SGS)
         d, arrays, exact = naca_case("g_t32_mg", c, ds1=1e-4, fdm=(1,)), ("U", "Res"), False
     if exact:
-        _compare_steps(d, nsteps, arrays, tile_rows=8, tile_cols=32)
+        _compare_steps(d, nsteps, arrays, tile_rows=rows, tile_cols=32)
     else:
-        _compare_steps(d, nsteps, arrays, exact=False, rtol=1e-11, tile_rows=8, tile_cols=32)
+        _compare_steps(d, nsteps, arrays, exact=False, rtol=1e-11, tile_rows=rows, tile_cols=32)
 
 
 SCHEMES = range(7)
