@@ -160,17 +160,6 @@ __device__ __forceinline__ double swap_pair(double v) {
   return __hiloint2double(hi, lo);
 }
 
-// a / b, correctly rounded, from y = RN(1/b) (Markstein): for b > 0 and
-// normal-range operands; q is faithful after the first refinement, and a
-// faithful q refined with RN(1/b) is RN(a/b)
-__device__ __forceinline__ double div_rcp(double a, double b, double y) {
-  double q = a * y;
-  double r = __builtin_fma(-b, q, a);
-  q = __builtin_fma(r, y, q);
-  r = __builtin_fma(-b, q, a);
-  return __builtin_fma(r, y, q);
-}
-
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 struct D2 {
   double x, y;

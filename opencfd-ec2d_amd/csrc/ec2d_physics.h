@@ -13,6 +13,19 @@
 
 namespace ec2d {
 
+// a / b, correctly rounded, from y = RN(1/b) (Markstein): for b > 0 and
+// normal-range operands; q is faithful after the first refinement, and a
+// faithful q refined with RN(1/b) is RN(a/b).  Five dependent operations
+// instead of the eleven of a division; used where several quotients share
+// a divisor or the divisor is a constant.
+__device__ __forceinline__ double div_rcp(double a, double b, double y) {
+  double q = a * y;
+  double r = __builtin_fma(-b, q, a);
+  q = __builtin_fma(r, y, q);
+  r = __builtin_fma(-b, q, a);
+  return __builtin_fma(r, y, q);
+}
+
 // sub_Residual.f90:477-487
 __device__ __forceinline__ double minmod(double a, double b) {
   if (a * b <= 0.0) return 0.0;
@@ -137,8 +150,13 @@ __device__ __forceinline__ void recon_characteristic(const double U0[4][4], doub
 }
 
 // ---- interface fluxes, Q = (rho, u_n, u_t, p): sub_flux_split.f90 -----------
+// The quotients keep the reference's operands; those that share a divisor
+// (1 + tmpp three times, ava*ava and 2*ava*ava = 2 RN(ava*ava) exactly) or
+// divide by a constant (2 delt) are formed with div_rcp from one correctly
+// rounded reciprocal, which gives the same RN(a/b).
 __device__ __forceinline__ void flux_roe(const double* QL, const double* QR, double* Fx, double g) {
-  const double delt = 0.1;  // :271
+  constexpr double delt = 0.1;  // :271
+  constexpr double y2d = 1.0 / (2.0 * delt);
   double dl = QL[0], uul = QL[1], vvl = QL[2], pl = QL[3];
   double dr = QR[0], uur = QR[1], vvr = QR[2], pr = QR[3];
   double hl = g * pl / ((g - 1.0) * dl) + 0.5 * (uul * uul + vvl * vvl);
@@ -147,19 +165,21 @@ __device__ __forceinline__ void flux_roe(const double* QL, const double* QR, dou
   double Fr0 = dr * uur, Fr1 = dr * uur * uur + pr, Fr2 = dr * uur * vvr, Fr3 = uur * dr * hr;
   double tmpp = sqrt(dr / dl);
   double avd = 0.25 * dl * (1.0 + tmpp) * (1.0 + tmpp);
-  double avu = (uul + tmpp * uur) / (1.0 + tmpp);
-  double avv = (vvl + tmpp * vvr) / (1.0 + tmpp);
-  double avh = (hl + tmpp * hr) / (1.0 + tmpp);
+  const double t1 = 1.0 + tmpp, yt = 1.0 / t1;
+  double avu = div_rcp(uul + tmpp * uur, t1, yt);
+  double avv = div_rcp(vvl + tmpp * vvr, t1, yt);
+  double avh = div_rcp(hl + tmpp * hr, t1, yt);
   double ava = sqrt((g - 1.0) * (avh - 0.5 * (avu * avu + avv * avv)));
   double l0 = fabs(avu - ava), l1 = fabs(avu), l2 = fabs(avu), l3 = fabs(avu + ava);
-  if (l0 < delt) l0 = (l0 * l0 + delt * delt) / (2.0 * delt);
-  if (l1 < delt) l1 = (l1 * l1 + delt * delt) / (2.0 * delt);
-  if (l2 < delt) l2 = (l2 * l2 + delt * delt) / (2.0 * delt);
-  if (l3 < delt) l3 = (l3 * l3 + delt * delt) / (2.0 * delt);
-  double a0 = (avd * ava * (-uur + uul) + pr - pl) / (2.0 * ava * ava);
-  double a1 = dr - dl - (pr - pl) / (ava * ava);
+  if (l0 < delt) l0 = div_rcp(l0 * l0 + delt * delt, 2.0 * delt, y2d);
+  if (l1 < delt) l1 = div_rcp(l1 * l1 + delt * delt, 2.0 * delt, y2d);
+  if (l2 < delt) l2 = div_rcp(l2 * l2 + delt * delt, 2.0 * delt, y2d);
+  if (l3 < delt) l3 = div_rcp(l3 * l3 + delt * delt, 2.0 * delt, y2d);
+  const double A = ava * ava, yA = 1.0 / A;
+  double a0 = div_rcp(avd * ava * (-uur + uul) + pr - pl, 2.0 * A, 0.5 * yA);
+  double a1 = dr - dl - div_rcp(pr - pl, A, yA);
   double a2 = avd * (vvr - vvl) / ava;
-  double a3 = (avd * ava * (uur - uul) + pr - pl) / (2.0 * ava * ava);
+  double a3 = div_rcp(avd * ava * (uur - uul) + pr - pl, 2.0 * A, 0.5 * yA);
   Fx[0] = 0.5 * (Fl0 + Fr0) - 0.5 * (l0 * a0 + l1 * a1 + l3 * a3);
   Fx[1] = 0.5 * (Fl1 + Fr1) - 0.5 * (l0 * a0 * (avu - ava) + l1 * a1 * avu + l3 * a3 * (avu + ava));
   Fx[2] = 0.5 * (Fl2 + Fr2) - 0.5 * (l0 * a0 * avv + l1 * a1 * avv + l2 * a2 + l3 * a3 * avv);

@@ -51,6 +51,9 @@ __global__ __launch_bounds__(TX* TY) void k_residual(ResArgs a) {
   constexpr int RW = TX + 4, RH = TY + 4, NT = TX * TY;
   const int o_scheme = FIX ? 5 : a.o.scheme, o_gscheme = FIX ? 5 : a.o.global_scheme;
   const int o_flux = FIX ? 4 : a.o.flux, o_recon = FIX ? 0 : a.o.recon;
+  // correctly rounded reciprocals of the constant divisors; div_rcp with
+  // them gives the reference's quotients (x / Pr, x / Pr_t, x / sigma, x / k^2)
+  constexpr double yPrt = 1.0 / EC2D_PRT, ySig = 1.0 / (2.0 / 3.0), yK2 = 1.0 / (0.41 * 0.41);
   constexpr int NFI = (TX + 1) * TY, NFJ = TX * (TY + 1);
   // viscous cases stage T and rebuild p = p00 d T from it (the staging's own
   // expression, so the same value); the cell centres of the viscous
@@ -76,6 +79,7 @@ __global__ __launch_bounds__(TX* TY) void k_residual(ResArgs a) {
   const BlockDesc B = a.blocks[t.block];
   if (case_frozen(B.case_id)) return;
   const CaseParams cp = a.cps[B.case_id];
+  const double yPr = 1.0 / cp.Pr;
   const PlaneMap pm = a.pm;
   const int nx = B.nx, ny = B.ny;
   const double g = cp.gamma;
@@ -238,13 +242,13 @@ __global__ __launch_bounds__(TX* TY) void k_residual(ResArgs a) {
       double Amu1, Amk1;
       if (i == 1) {
         Amu1 = sAm[lj][li] + sAt[lj][li];
-        Amk1 = cp.Cp * (sAm[lj][li] / cp.Pr + sAt[lj][li] / EC2D_PRT);
+        Amk1 = cp.Cp * (div_rcp(sAm[lj][li], cp.Pr, yPr) + div_rcp(sAt[lj][li], EC2D_PRT, yPrt));
       } else if (i == nx) {
         Amu1 = sAm[lj][li - 1] + sAt[lj][li - 1];
-        Amk1 = cp.Cp * (sAm[lj][li - 1] / cp.Pr + sAt[lj][li - 1] / EC2D_PRT);
+        Amk1 = cp.Cp * (div_rcp(sAm[lj][li - 1], cp.Pr, yPr) + div_rcp(sAt[lj][li - 1], EC2D_PRT, yPrt));
       } else {
         Amu1 = (sAm[lj][li - 1] + sAm[lj][li] + sAt[lj][li - 1] + sAt[lj][li]) * 0.5;
-        Amk1 = cp.Cp * ((sAm[lj][li - 1] + sAm[lj][li]) / cp.Pr + (sAt[lj][li - 1] + sAt[lj][li]) / EC2D_PRT) * 0.5;
+        Amk1 = cp.Cp * (div_rcp(sAm[lj][li - 1] + sAm[lj][li], cp.Pr, yPr) + div_rcp(sAt[lj][li - 1] + sAt[lj][li], EC2D_PRT, yPrt)) * 0.5;
       }
       double Dix = SX(lj, li) - SX(lj, li - 1);
       double Diy = SY(lj, li) - SY(lj, li - 1);
@@ -296,7 +300,7 @@ __global__ __launch_bounds__(TX* TY) void k_residual(ResArgs a) {
   // ---- 3a. each cell's i-face difference (the first terms of Res, in the
   // reference's order), before the j-faces take over the LDS --------------
   double R[NV];
-  double r5 = 0.0;
+  double r5 = 0.0, v0c = 0.0;  // SA: the cell's v0 of sub_turbulence_SA.f90, kept for phase 3
   {
     const int ci = tid % TX, cj = tid / TX;
     const int i = i0 + ci, j = j0 + cj;
@@ -309,8 +313,8 @@ __global__ __launch_bounds__(TX* TY) void k_residual(ResArgs a) {
       if constexpr (SA) {
         const double SA_sigma = 2.0 / 3.0, Cb2 = 0.622;
         const int li = ci + 2, lj = cj + 2;
-        const double v0 = (svt[lj][li] + sAm[lj][li] / sd[lj][li]) * Cb2 / SA_sigma;
-        r5 = sFv[0][cj][ci + 1] - sFv[0][cj][ci] - v0 * (sFv[1][cj][ci + 1] - sFv[1][cj][ci]);
+        v0c = div_rcp((svt[lj][li] + sAm[lj][li] / sd[lj][li]) * Cb2, SA_sigma, ySig);
+        r5 = sFv[0][cj][ci + 1] - sFv[0][cj][ci] - v0c * (sFv[1][cj][ci + 1] - sFv[1][cj][ci]);
       }
     }
   }
@@ -370,13 +374,13 @@ __global__ __launch_bounds__(TX* TY) void k_residual(ResArgs a) {
       double Amu1, Amk1;
       if (j == 1) {
         Amu1 = sAm[lj][li] + sAt[lj][li];
-        Amk1 = cp.Cp * (sAm[lj][li] / cp.Pr + sAt[lj][li] / EC2D_PRT);
+        Amk1 = cp.Cp * (div_rcp(sAm[lj][li], cp.Pr, yPr) + div_rcp(sAt[lj][li], EC2D_PRT, yPrt));
       } else if (j == ny) {
         Amu1 = sAm[lj - 1][li] + sAt[lj - 1][li];
-        Amk1 = cp.Cp * (sAm[lj - 1][li] / cp.Pr + sAt[lj - 1][li] / EC2D_PRT);
+        Amk1 = cp.Cp * (div_rcp(sAm[lj - 1][li], cp.Pr, yPr) + div_rcp(sAt[lj - 1][li], EC2D_PRT, yPrt));
       } else {
         Amu1 = (sAm[lj][li] + sAm[lj - 1][li] + sAt[lj][li] + sAt[lj - 1][li]) * 0.5;
-        Amk1 = cp.Cp * ((sAm[lj - 1][li] + sAm[lj][li]) / cp.Pr + (sAt[lj - 1][li] + sAt[lj][li]) / EC2D_PRT) * 0.5;
+        Amk1 = cp.Cp * (div_rcp(sAm[lj - 1][li] + sAm[lj][li], cp.Pr, yPr) + div_rcp(sAt[lj - 1][li] + sAt[lj][li], EC2D_PRT, yPrt)) * 0.5;
       }
       double Dix = (SX(lj - 1, li + 1) + SX(lj, li + 1) - SX(lj - 1, li - 1) - SX(lj, li - 1)) * 0.25;
       double Diy = (SY(lj - 1, li + 1) + SY(lj, li + 1) - SY(lj - 1, li - 1) - SY(lj, li - 1)) * 0.25;
@@ -445,7 +449,7 @@ __global__ __launch_bounds__(TX* TY) void k_residual(ResArgs a) {
         const double Cw1 = Cb1 / (SA_k * SA_k) + (1.0 + Cb2) / SA_sigma, Cw2 = 0.3, Ct3 = 1.3, Ct4 = 0.5;
         const double Cv1_3 = Cv1 * Cv1 * Cv1, Cw3_6 = 2.0 * 2.0 * 2.0 * 2.0 * 2.0 * 2.0;
         double vtc = svt[lj][li];
-        double v0 = (vtc + sAm[lj][li] / sd[lj][li]) * Cb2 / SA_sigma;
+        const double v0 = v0c;
         r5 = r5 + sFw[0][cj + 1][ci] - sFw[0][cj][ci] - v0 * (sFw[1][cj + 1][ci] - sFw[1][cj][ci]);
         double Dix = (SX(lj, li + 1) - SX(lj, li - 1)) * 0.5;
         double Diy = (SY(lj, li + 1) - SY(lj, li - 1)) * 0.5;
@@ -471,7 +475,7 @@ __global__ __launch_bounds__(TX* TY) void k_residual(ResArgs a) {
         double fw = gg * pow((1.0 + Cw3_6) / (gg * gg * gg * gg * gg * gg + Cw3_6), 1.0 / 6.0);
         double ft2 = Ct3 * exp(-(Ct4 * X * X));
         double qv = vtc / dwc;
-        double Q_SA = Cb1 * (1.0 - ft2) * S1 * vtc - (Cw1 * fw - Cb1 * ft2 / (SA_k * SA_k)) * (qv * qv);
+        double Q_SA = Cb1 * (1.0 - ft2) * S1 * vtc - (Cw1 * fw - div_rcp(Cb1 * ft2, SA_k * SA_k, yK2)) * (qv * qv);
         r5 = r5 + Q_SA * PL(pm.vol)[q];
         R[4] = r5;
       }
@@ -507,7 +511,7 @@ __global__ __launch_bounds__(TX* TY) void k_residual(ResArgs a) {
       double Lcj = (fabs(unj) + cc) * sj;
       double Lvi = 0.0, Lvj = 0.0;
       if constexpr (VISC) {
-        double tmp1 = g / sd[lj][li] * (sAm[lj][li] / cp.Pr + sAt[lj][li] / EC2D_PRT);
+        double tmp1 = g / sd[lj][li] * (div_rcp(sAm[lj][li], cp.Pr, yPr) + div_rcp(sAt[lj][li], EC2D_PRT, yPrt));
         Lvi = tmp1 * si * si / s0;
         Lvj = tmp1 * sj * sj / s0;
         PL(pm.Lvi)[q] = Lvi;
