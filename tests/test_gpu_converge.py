@@ -150,7 +150,7 @@ def test_run_to_drop_cold_cell_stops_only_its_case():
 
 
 def test_sa_converged_cl_cd_match_oracle():
-    """The north star's bar on converged forces: an SA-RANS C-grid case (65 x
+    """Converged forces to 1e-8: an SA-RANS C-grid case (65 x
     17, Re 1e5, M 0.5) run on the device to a 1e-8 density drop (about 12k
     passes) has Cl and Cd within 1e-8 relative of the oracle's run of the same
     number of updates (the device evaluates SA's exp / pow with its own math
