@@ -39,7 +39,9 @@ enum {
   EC2D_U = 1,      /* U(NV,-1:nx+1,-1:ny+1) */
   EC2D_RES = 2,    /* Res(NV,1:nx-1,1:ny-1) */
   EC2D_DT = 3,     /* dt(1:nx-1,1:ny-1) */
-  EC2D_DU = 4,     /* dU(NV,1:nx,1:ny) */
+  EC2D_DU = 4,     /* dU(NV,1:nx,1:ny): written by ec2d_du_lu_sgs_2d; a main-loop
+                      pass (ec2d_ns_time_advance, ec2d_run) on the fused LU-SGS
+                      path updates U without rewriting it (nothing reads it there) */
   EC2D_LCI = 5, EC2D_LCJ = 6, EC2D_LVI = 7, EC2D_LVJ = 8, /* (1:nx,1:ny) */
   EC2D_AMU = 9, EC2D_AMU_T = 10,                          /* (-1:nx+1,-1:ny+1); after a residual Amu holds
                                                              the interior, Amu_t also the layer 0:nx x 0:ny

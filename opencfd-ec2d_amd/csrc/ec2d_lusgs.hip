@@ -882,12 +882,13 @@ __global__ __launch_bounds__(MAXT) void k_lusgs_sweep(LusgsArgs a) {
 // k_lusgs_prep), into a ring of LDS slots (6 for NP = 3, 7 for NP = 6).  The
 // forward dU goes to the backward records' X pairs in global memory (the
 // backward sweep reads it there, after a fence).  The sweep wave puts the
-// backward dU (and U + dU) of four steps into an LDS buffer; wave NP + 1,
+// backward U + dU (with update; dU without) of four steps into an LDS
+// buffer; wave NP + 1,
 // the writer, takes each chunk and stores it to the block planes four
 // consecutive cells of a row at a time: this replaces k_lusgs_scatter
 // (stores along a hyperplane, one cache line per lane, hold up whichever
 // wave issues them).  NP = 3 or 6 (5 or 8 waves; with 6 the ring and the
-// buffer of a six-equation record take 157 KB of the CU's 160 KB).
+// buffer of a six-equation record take 145 KB of the CU's 160 KB).
 //
 // The writer waits for ob_ready > chunk, reads the chunk, then sets
 // ob_taken = chunk + 1; the sweep wave waits for ob_taken >= chunk before
@@ -906,7 +907,7 @@ struct FusedLds {
   static constexpr int NRING = NP == 3 ? 6 : 7;  // LDS record slots
   double sh[NRING][F::NSH][64];     // shared doubles of slot, lane
   double sd[NRING][2][F::NSD][64];  // side doubles
-  double ob[4][2 * NV][64];         // the sweep's backward dU and U + dU of four steps
+  double ob[4][NV][64];             // the sweep's backward U + dU (update) or dU of four steps
   int produced[NP];
   int consumed;
   int ob_ready, ob_taken;           // chunks of four backward steps put into / taken from ob
@@ -951,17 +952,20 @@ __global__ __launch_bounds__(64 * (2 + NP)) void k_lusgs_fused(LusgsArgs a) {
   bool ok = true;
   if (wave == FPROD + 1) {
     // ---- writer: each chunk of four backward steps from ob to the block
-    // planes (the stores of k_lusgs_scatter); lane = 16 rows x 4 steps, so
-    // that a store covers 4 consecutive cells of each of 16 rows
+    // planes; lane = 16 rows x 4 steps, so that a store covers 4 consecutive
+    // cells of each of 16 rows.  With update the planes are U (U + dU, the
+    // update of k_lusgs_scatter; nothing reads the dU planes after a
+    // main-loop pass, so they are not rewritten), without it dU.
     const int sub = lane & 3;
+    const int dst = a.update ? a.pm.U : a.pm.dU;
     for (int ck = 0; ck < npad / 4 && ok; ck++) {
       ok = wait_bounded(&L.ob_ready, ck + 1, a.err, a.spin_max);
       if (!ok) break;
-      double v[4][2 * NV];
+      double v[4][NV];
 #pragma unroll
       for (int rg = 0; rg < 4; rg++)
 #pragma unroll
-        for (int m = 0; m < 2 * NV; m++) v[rg][m] = L.ob[sub][m][rg * 16 + (lane >> 2)];
+        for (int m = 0; m < NV; m++) v[rg][m] = L.ob[sub][m][rg * 16 + (lane >> 2)];
       __atomic_signal_fence(__ATOMIC_SEQ_CST);
       __builtin_amdgcn_s_waitcnt(0xc07f);  // the buffer read before it is handed back
       lds_put(&L.ob_taken, ck + 1);
@@ -972,10 +976,7 @@ __global__ __launch_bounds__(64 * (2 + NP)) void k_lusgs_fused(LusgsArgs a) {
         if (tr < nrow && c >= 0 && c < ncol) {
           const int64_t q = pidx(B, ncol - c, nrow - tr);
 #pragma unroll
-          for (int m = 0; m < NV; m++) {
-            planes[(int64_t)(a.pm.dU + m) * B.plane + q] = v[rg][m];
-            if (a.update) planes[(int64_t)(a.pm.U + m) * B.plane + q] = v[rg][NV + m];
-          }
+          for (int m = 0; m < NV; m++) planes[(int64_t)(dst + m) * B.plane + q] = v[rg][m];
         }
       }
     }
@@ -1047,14 +1048,14 @@ __global__ __launch_bounds__(64 * (2 + NP)) void k_lusgs_fused(LusgsArgs a) {
   // that every register array keeps fixed indices.  The next step's record
   // is read from its slot while this step computes; in the backward sweep
   // each step's X (the forward dU) is loaded XD steps ahead, into one of XD
-  // register sets (the loop runs XD steps per turn: npad is a multiple of 4).
+  // register sets (the loop runs XD steps per turn; npad is a multiple of 4).
   auto run_dir = [&](auto fwd_c) {
     constexpr bool fwd = decltype(fwd_c)::value;
     constexpr int dir = fwd ? 0 : 1;
     const int gs0 = dir * npad;
     Ahead<NV> ah = {};
     Out<NV> own = {};
-    constexpr int XD = 4;
+    constexpr int XD = NV >= 6 ? 2 : 4;  // six-equation records leave no registers for four sets
     double xs[XD][2 * F::XP] = {};
     if (!fwd) {
       // the forward sweep's stores to the X pairs before these loads
@@ -1087,9 +1088,9 @@ __global__ __launch_bounds__(64 * (2 + NP)) void k_lusgs_fused(LusgsArgs a) {
         for (int m = 0; m < NV; m++) r.X[m] = x[m];
         load_x(x, st + XD);
       }
-      // forward: dU to the backward records' X pairs; backward: dU and
-      // U + dU (the record's U is the plane's value) into the output buffer,
-      // which the writer wave stores four steps at a time
+      // forward: dU to the backward records' X pairs; backward: U + dU (the
+      // record's U is the plane's value; with update) or dU into the output
+      // buffer, which the writer wave stores four steps at a time
       sweep_step<NV, VISC, fwd, 1, fwd>(r, own, rs, bw, n16, st, S, R, ncol, nrow, t, 0, lane, 0, false, false, g, ch,
                                         ah);
       // Slot gs is released after the step: its reads were issued a step ago
@@ -1103,10 +1104,12 @@ __global__ __launch_bounds__(64 * (2 + NP)) void k_lusgs_fused(LusgsArgs a) {
         // npad is a multiple of four: every chunk has four steps
         const int b4 = st & 3, ck = st >> 2;
         if (b4 == 0 && ck > 0 && ok) ok = wait_bounded(&L.ob_taken, ck, a.err, a.spin_max);
+        if (a.update) {
 #pragma unroll
-        for (int m = 0; m < NV; m++) {
-          L.ob[b4][m][t] = own.du[m];
-          L.ob[b4][NV + m][t] = own.W[m];
+          for (int m = 0; m < NV; m++) L.ob[b4][m][t] = own.W[m];
+        } else {
+#pragma unroll
+          for (int m = 0; m < NV; m++) L.ob[b4][m][t] = own.du[m];
         }
         if (b4 == 3) {
           __atomic_signal_fence(__ATOMIC_SEQ_CST);
