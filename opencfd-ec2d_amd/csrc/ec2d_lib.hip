@@ -527,7 +527,7 @@ int closure_infer() {
   return 0;
 }
 
-int launch_residual(int level, int clip, int force = 0, int set_un = 0) {
+int launch_residual(int level, int clip, int force = 0, int set_un = 0, double* sk_out = nullptr) {
   DLevel& L = R.lv[level];
   ResArgs a;
   a.tiles = L.dt;
@@ -541,6 +541,7 @@ int launch_residual(int level, int clip, int force = 0, int set_un = 0) {
   a.o = L.o;
   a.pm = L.pm;
   a.clip = clip;
+  a.sk_out = sk_out;
   a.write_visc = 1;
   a.force = force;
   a.set_un = set_un;
@@ -674,8 +675,10 @@ int reduce_norms(DLevel& L) {
 void prof_mark(int kind);
 
 // sk, gring: the record arena and split-sweep rings to use (null: the level's own)
+// skew_done: the residual before it wrote the changing inputs in hyperplane
+// order (fused path, the level's own record arena)
 int launch_lusgs(int level, const int* dlist, int nblk, double alfa1, int update, double* sk = nullptr,
-                 unsigned long long* gring = nullptr) {
+                 unsigned long long* gring = nullptr, bool skew_done = false) {
   DLevel& L = R.lv[level];
   LusgsArgs a;
   a.blocks = L.db;
@@ -720,7 +723,8 @@ int launch_lusgs(int level, const int* dlist, int nblk, double alfa1, int update
     prof_mark(4 + pk);                                                                      \
     if (static_skew)                                                                        \
       hipLaunchKernelGGL((k_lusgs_skew<NV, V, true>), gskew, dim3(256), 0, R.s, a);         \
-    hipLaunchKernelGGL((k_lusgs_skew<NV, V, false>), gskew, dim3(256), 0, R.s, a);          \
+    if (!skew_done)                                                                         \
+      hipLaunchKernelGGL((k_lusgs_skew<NV, V, false>), gskew, dim3(256), 0, R.s, a);        \
     prof_mark(-(4 + pk));                                                                   \
     prof_mark(5 + pk);                                                                      \
     if (L.fused_np == 3)                                                                    \
@@ -855,11 +859,12 @@ int flush_clip(int level) {
 // ------------------------------------------------------------ composite steps
 // force: 1 = comput_force_function, 2 = Add_force_function fused into the
 // residual's store (coarse levels; see ResArgs::force); set_un: Set_Un too
-int residual(int level, int force = 0, int set_un = 0) {
+// sk_out: see ResArgs (the fused LU-SGS that follows then skips its copy)
+int residual(int level, int force = 0, int set_un = 0, double* sk_out = nullptr) {
   int clip = R.clip_pending[level] ? 1 : 0;
   R.clip_pending[level] = false;
   prof_mark(1 + 10 * level);
-  int rc = launch_residual(level, clip, force, set_un);
+  int rc = launch_residual(level, clip, force, set_un, sk_out);
   if (!rc && R.cases[0].ctl.Time_Method == -1) {
     // dual-time source after the whole residual and before the norms:
     // Sfac = 1/(3 dt) on the first residual call, 1/(2 dt) afterwards
@@ -908,9 +913,13 @@ void tick(int level) {
 
 int advance_lusgs(int level, bool add_force) {
   DLevel& L = R.lv[level];
-  if (residual(level, add_force ? 2 : 0)) return 1;
+  // the fused launch's changing inputs in hyperplane order straight from the
+  // residual's epilogue (not on FD-WENO5 levels, whose interior Res a later
+  // launch writes), in place of its copy k_lusgs_skew
+  const bool rskew = L.lusgs_fused && !L.fdm;
+  if (residual(level, add_force ? 2 : 0, 0, rskew ? L.sk : nullptr)) return 1;
   prof_mark(2 + 10 * level);
-  if (launch_lusgs(level, L.dblocks_all, (int)L.hb.size(), 0.0, 1)) return 1;
+  if (launch_lusgs(level, L.dblocks_all, (int)L.hb.size(), 0.0, 1, nullptr, nullptr, rskew)) return 1;
   prof_mark(-(2 + 10 * level));
   prof_mark(3 + 10 * level);
   if (launch_ghost(level, 1, 1)) return 1;

@@ -33,6 +33,8 @@ struct ResArgs {
   MeshOpts o;
   PlaneMap pm;
   int clip;                   // apply force_vt_kw to interior nu_t on load
+  double* sk_out;             // or null: the changing inputs of the fused LU-SGS in hyperplane order (the
+                              // copy k_lusgs_skew makes), written here from the values the epilogue holds
   int write_visc;             // store Amu / Amu_t planes
   int force;                  // coarse multigrid levels, interior cells, variables 1..4, after the norms' values:
                               // 1: QF = QF - Res (comput_force_function), 2: Res = Res + QF (Add_force_function)
@@ -479,6 +481,7 @@ __global__ __launch_bounds__(TX* TY) void k_residual(ResArgs a) {
         r5 = r5 + Q_SA * PL(pm.vol)[q];
         R[4] = r5;
       }
+      double Rs[NV];  // Res as stored (with the forcing function on coarse levels)
       if (!fdm_cell) {
 #pragma unroll
         for (int m = 0; m < 4; m++) {
@@ -486,6 +489,7 @@ __global__ __launch_bounds__(TX* TY) void k_residual(ResArgs a) {
           if (a.force == 1) PL(pm.QF + m)[q] = PL(pm.QF + m)[q] - r;
           else if (a.force == 2) r = r + PL(pm.QF + m)[q];
           PL(pm.Res + m)[q] = r;
+          Rs[m] = r;
         }
       }
       if constexpr (SA) PL(pm.Res + 4)[q] = R[4];
@@ -494,6 +498,8 @@ __global__ __launch_bounds__(TX* TY) void k_residual(ResArgs a) {
         R[4] = PL(pm.Res + 4)[q];
         R[5] = PL(pm.Res + 5)[q];
       }
+#pragma unroll
+      for (int m = 4; m < NV; m++) Rs[m] = R[m];
       // comput_Lvc
       int64_t qi1 = pidx(B, i + 1, j), qj1 = pidx(B, i, j + 1);
       double s0 = PL(pm.vol)[q];
@@ -528,6 +534,30 @@ __global__ __launch_bounds__(TX* TY) void k_residual(ResArgs a) {
         if (dt < cp.dtmin) dt = cp.dtmin;
       }
       PL(pm.dt)[q] = dt;
+      if (a.sk_out) {
+        // logical plane k of cell (i, j) at k n + (i-1 + j-1) R + j-1 (the
+        // layout of k_lusgs_skew): U (nu_t as clipped in the staging), Res,
+        // dt, Lci, Lcj and, viscous, Lvi, Lvj
+        using I = RecIn;
+        double* so = a.sk_out + B.sk_base + (int64_t)(i - 1 + j - 1) * B.sk_R + (j - 1);
+        const int64_t n = B.sk_n;
+#pragma unroll
+        for (int m = 0; m < NV; m++) {
+          double u = U[m][q];
+          if constexpr (SA) {
+            if (m == 4) u = svt[lj][li];
+          }
+          so[(I::U + m) * n] = u;
+          so[(I::RES<NV> + m) * n] = Rs[m];
+        }
+        so[I::DT<NV> * n] = dt;
+        so[I::LCI<NV> * n] = Lci;
+        so[I::LCJ<NV> * n] = Lcj;
+        if constexpr (VISC) {
+          so[I::LVI<NV> * n] = Lvi;
+          so[I::LVJ<NV> * n] = Lvj;
+        }
+      }
       if (!fdm_cell) {
 #pragma unroll
         for (int m = 0; m < NV; m++) {
