@@ -299,12 +299,20 @@ __global__ __launch_bounds__(TX* TY) void k_residual(ResArgs a) {
   }
   __syncthreads();
 
+  // The cell of this thread in phases 3a and 3: row-major, or, when the
+  // epilogue also writes the hyperplane-order copy (sk_out), along the
+  // tile's hyperplanes: TY consecutive lanes take the TY rows of one
+  // hyperplane, so that the copy's stores form runs of TY positions.
+  const int kd = tid / TY, cjd = tid % TY;
+  const int cell_i = a.sk_out ? (kd - cjd + TX) % TX : tid % TX;
+  const int cell_j = a.sk_out ? cjd : tid / TX;
+
   // ---- 3a. each cell's i-face difference (the first terms of Res, in the
   // reference's order), before the j-faces take over the LDS --------------
   double R[NV];
   double r5 = 0.0, v0c = 0.0;  // SA: the cell's v0 of sub_turbulence_SA.f90, kept for phase 3
   {
-    const int ci = tid % TX, cj = tid / TX;
+    const int ci = cell_i, cj = cell_j;
     const int i = i0 + ci, j = j0 + cj;
     if (i <= nx - 1 && j <= ny - 1) {
       const bool fdm_cell = fdm && (i >= LF + 1 && i <= nx - LF - 1) && (j >= LF + 1 && j <= ny - LF - 1);
@@ -436,7 +444,7 @@ __global__ __launch_bounds__(TX* TY) void k_residual(ResArgs a) {
 #pragma unroll
   for (int m = 0; m < NV; m++) { rmax[m] = 0.0; rsum[m] = 0.0; }
   {
-    int ci = tid % TX, cj = tid / TX;
+    int ci = cell_i, cj = cell_j;
     int i = i0 + ci, j = j0 + cj;
     if (i <= nx - 1 && j <= ny - 1) {
       int64_t q = pidx(B, i, j);
