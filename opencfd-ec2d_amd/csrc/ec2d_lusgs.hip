@@ -882,13 +882,13 @@ __global__ __launch_bounds__(MAXT) void k_lusgs_sweep(LusgsArgs a) {
 // k_lusgs_prep), into a ring of LDS slots (6 for NP = 3, 7 for NP = 6).  The
 // forward dU goes to the backward records' X pairs in global memory (the
 // backward sweep reads it there, after a fence).  The sweep wave puts the
-// backward U + dU (with update; dU without) of four steps into an LDS
+// backward U + dU (with update; dU without) of eight steps into an LDS
 // buffer; wave NP + 1,
-// the writer, takes each chunk and stores it to the block planes four
+// the writer, takes each chunk and stores it to the block planes eight
 // consecutive cells of a row at a time: this replaces k_lusgs_scatter
 // (stores along a hyperplane, one cache line per lane, hold up whichever
 // wave issues them).  NP = 3 or 6 (5 or 8 waves; with 6 the ring and the
-// buffer of a six-equation record take 145 KB of the CU's 160 KB).
+// buffer of a six-equation record take 157 KB of the CU's 160 KB).
 //
 // The writer waits for ob_ready > chunk, reads the chunk, then sets
 // ob_taken = chunk + 1; the sweep wave waits for ob_taken >= chunk before
@@ -907,10 +907,10 @@ struct FusedLds {
   static constexpr int NRING = NP == 3 ? 6 : 7;  // LDS record slots
   double sh[NRING][F::NSH][64];     // shared doubles of slot, lane
   double sd[NRING][2][F::NSD][64];  // side doubles
-  double ob[4][NV][64];             // the sweep's backward U + dU (update) or dU of four steps
+  double ob[8][NV][64];             // the sweep's backward U + dU (update) or dU of eight steps
   int produced[NP];
   int consumed;
-  int ob_ready, ob_taken;           // chunks of four backward steps put into / taken from ob
+  int ob_ready, ob_taken;           // chunks of eight backward steps put into / taken from ob
 };
 
 __device__ __forceinline__ bool wait_bounded(const int* p, int v, int* err, int spin_max) {
@@ -951,28 +951,30 @@ __global__ __launch_bounds__(64 * (2 + NP)) void k_lusgs_fused(LusgsArgs a) {
   double* planes = a.arena + B.base;
   bool ok = true;
   if (wave == FPROD + 1) {
-    // ---- writer: each chunk of four backward steps from ob to the block
-    // planes; lane = 16 rows x 4 steps, so that a store covers 4 consecutive
-    // cells of each of 16 rows.  With update the planes are U (U + dU, the
-    // update of k_lusgs_scatter; nothing reads the dU planes after a
-    // main-loop pass, so they are not rewritten), without it dU.
-    const int sub = lane & 3;
+    // ---- writer: each chunk of eight backward steps from ob to the block
+    // planes; lane = 8 rows x 8 steps, so that a store covers 8 consecutive
+    // cells (64 bytes) of each of 8 rows.  The last chunk may hold four
+    // steps: the other four map to columns past the block and are skipped.
+    // With update the planes are U (U + dU, the update of k_lusgs_scatter;
+    // nothing reads the dU planes after a main-loop pass, so they are not
+    // rewritten), without it dU.
+    const int sub = lane & 7;
     const int dst = a.update ? a.pm.U : a.pm.dU;
-    for (int ck = 0; ck < npad / 4 && ok; ck++) {
+    for (int ck = 0; ck < (npad + 7) / 8 && ok; ck++) {
       ok = wait_bounded(&L.ob_ready, ck + 1, a.err, a.spin_max);
       if (!ok) break;
-      double v[4][NV];
+      double v[8][NV];
 #pragma unroll
-      for (int rg = 0; rg < 4; rg++)
+      for (int rg = 0; rg < 8; rg++)
 #pragma unroll
-        for (int m = 0; m < NV; m++) v[rg][m] = L.ob[sub][m][rg * 16 + (lane >> 2)];
+        for (int m = 0; m < NV; m++) v[rg][m] = L.ob[sub][m][rg * 8 + (lane >> 3)];
       __atomic_signal_fence(__ATOMIC_SEQ_CST);
       __builtin_amdgcn_s_waitcnt(0xc07f);  // the buffer read before it is handed back
       lds_put(&L.ob_taken, ck + 1);
-      const int sc = ck * 4 + sub;
+      const int sc = ck * 8 + sub;
 #pragma unroll
-      for (int rg = 0; rg < 4; rg++) {
-        const int tr = rg * 16 + (lane >> 2), c = sc - tr;
+      for (int rg = 0; rg < 8; rg++) {
+        const int tr = rg * 8 + (lane >> 3), c = sc - tr;
         if (tr < nrow && c >= 0 && c < ncol) {
           const int64_t q = pidx(B, ncol - c, nrow - tr);
 #pragma unroll
@@ -1090,7 +1092,7 @@ __global__ __launch_bounds__(64 * (2 + NP)) void k_lusgs_fused(LusgsArgs a) {
       }
       // forward: dU to the backward records' X pairs; backward: U + dU (the
       // record's U is the plane's value; with update) or dU into the output
-      // buffer, which the writer wave stores four steps at a time
+      // buffer, which the writer wave stores eight steps at a time
       sweep_step<NV, VISC, fwd, 1, fwd>(r, own, rs, bw, n16, st, S, R, ncol, nrow, t, 0, lane, 0, false, false, g, ch,
                                         ah);
       // Slot gs is released after the step: its reads were issued a step ago
@@ -1101,17 +1103,18 @@ __global__ __launch_bounds__(64 * (2 + NP)) void k_lusgs_fused(LusgsArgs a) {
       __builtin_amdgcn_s_waitcnt(0xc07f);
       lds_put(&L.consumed, gs + 1);
       if constexpr (!fwd) {
-        // npad is a multiple of four: every chunk has four steps
-        const int b4 = st & 3, ck = st >> 2;
-        if (b4 == 0 && ck > 0 && ok) ok = wait_bounded(&L.ob_taken, ck, a.err, a.spin_max);
+        // chunks of eight steps; npad is a multiple of four, so the last
+        // chunk may end after four
+        const int b8 = st & 7, ck = st >> 3;
+        if (b8 == 0 && ck > 0 && ok) ok = wait_bounded(&L.ob_taken, ck, a.err, a.spin_max);
         if (a.update) {
 #pragma unroll
-          for (int m = 0; m < NV; m++) L.ob[b4][m][t] = own.W[m];
+          for (int m = 0; m < NV; m++) L.ob[b8][m][t] = own.W[m];
         } else {
 #pragma unroll
-          for (int m = 0; m < NV; m++) L.ob[b4][m][t] = own.du[m];
+          for (int m = 0; m < NV; m++) L.ob[b8][m][t] = own.du[m];
         }
-        if (b4 == 3) {
+        if (b8 == 7 || st == npad - 1) {
           __atomic_signal_fence(__ATOMIC_SEQ_CST);
           __builtin_amdgcn_s_waitcnt(0xc07f);  // the chunk's writes done before it is handed over
           lds_put(&L.ob_ready, ck + 1);
