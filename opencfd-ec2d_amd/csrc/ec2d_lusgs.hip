@@ -431,7 +431,9 @@ __global__ __launch_bounds__(256) void k_lusgs_prep(LusgsArgs a) {
 // (RecIn) of cell (i, j) goes to sk + k n + (i-1 + j-1) R + j-1 (the
 // forward record region, which the fused launch leaves free).  STATIC: the
 // mesh planes (vol, normals, face areas), copied once per layout; else the
-// planes a residual and an update change.  A workgroup moves a tile of 16
+// planes a residual and an update change (in a main-loop pass the residual
+// before the update writes those itself, ResArgs::sk_out, and this copy is
+// left out).  A workgroup moves a tile of 16
 // steps x 16 rows through LDS: 16 lanes read 16 consecutive cells of a row,
 // and 16 lanes write 16 consecutive positions of a step (128 bytes each).
 // A row's 16 cells start at column s0 - r, so they straddle two cache lines
