@@ -37,7 +37,7 @@ struct BlockDesc {
   int64_t plane;       // doubles per plane = (ny+3)*P
   int fdm;             // 1: finite-difference interior (LFDM = 4 edge layers stay FVM)
   int bc_off, nsub;    // subfaces in the level's subface table
-  int sk_S, sk_R;      // LU-SGS records: steps (ncol + nrow - 1), rows rounded up to 32
+  int sk_S, sk_R;      // LU-SGS records: steps (ncol + nrow - 1), rows (whole sweep waves; the fused launch: nrow)
   int64_t sk_base;     // first double of the block's LU-SGS records in the level's record arena
   int64_t sk_n;        // stride of a record field in doubles: sk_S * sk_R positions + 64 trash slots
 };

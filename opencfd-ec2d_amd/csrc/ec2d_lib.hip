@@ -247,8 +247,11 @@ void ghost_phases(const BlockDesc& B, SubfaceDesc* s, int* units) {
 }
 
 // LU-SGS record layout of a level for lpc lanes per cell (see ec2d_lusgs.hip);
-// upload: also refresh the device block table (after ec2d_commit).
-int lusgs_layout(DLevel& L, int lpc, int split, bool upload) {
+// upload: also refresh the device block table (after ec2d_commit).  fused:
+// the layout of k_lusgs_fused, whose single sweep wave needs no whole waves
+// of rows, so a step holds exactly the block's rows (R = nrow): the rows of
+// consecutive steps share cache lines instead of padding each step to 64.
+int lusgs_layout(DLevel& L, int lpc, int split, bool upload, bool fused = false) {
   L.skew_static = false;  // the arena is cleared below
   int max_rows = 1;
   for (auto& d : L.hb) max_rows = std::max(max_rows, d.ny - 1);
@@ -273,7 +276,7 @@ int lusgs_layout(DLevel& L, int lpc, int split, bool upload) {
   for (auto& d : L.hb) {
     const int ncol = d.nx - 1, nrow = d.ny - 1;
     d.sk_S = ncol + nrow - 1;
-    d.sk_R = rq * split * ((nrow + rq * split - 1) / (rq * split));
+    d.sk_R = fused ? nrow : rq * split * ((nrow + rq * split - 1) / (rq * split));
     d.sk_n = (int64_t)d.sk_S * d.sk_R + 64;
     d.sk_base = sk_off;
     if (per * d.sk_n * 8 >= (int64_t)1 << 31) return fail("block too large for the LU-SGS record buffer (2 GiB)");
@@ -2318,7 +2321,7 @@ int ec2d_lusgs_shape(int nMesh, int lanes_per_cell, int workgroups_per_block) {
     for (auto& b : L.hb)
       if (b.ny - 1 > 64) return fail("the fused LU-SGS launch needs blocks of at most 64 rows of cells");
   }
-  if (lusgs_layout(L, lanes_per_cell, fused ? 1 : workgroups_per_block, true)) return 1;
+  if (lusgs_layout(L, lanes_per_cell, fused ? 1 : workgroups_per_block, true, fused)) return 1;
   L.lusgs_u = 2;
   L.lusgs_fused = fused;
   L.fused_np = workgroups_per_block == -3 ? 3 : 6;
@@ -2420,7 +2423,7 @@ int ec2d_autotune(const char* cache_path, int reps, char* report, int report_len
       // the fused variant: records made in the sweep workgroup (rows <= 64)
       bool fits = true;
       for (auto& b : L.hb) fits = fits && b.ny - 1 <= 64;
-      if (fits && !lusgs_layout(L, 1, 1, true)) {
+      if (fits && !lusgs_layout(L, 1, 1, true, true)) {
         L.lusgs_fused = true;
         for (int np : {3, 6}) {
           L.fused_np = np;
@@ -2452,7 +2455,7 @@ int ec2d_autotune(const char* cache_path, int reps, char* report, int report_len
     L.tile_cols = best_tr >= 100 ? 32 : 64;
     L.tile_rows = best_tr % 100;
     if (build_tiles(L)) return 1;
-    if (lusgs_layout(L, best_lp, best_sp == 0 ? 1 : best_sp, true)) return 1;
+    if (lusgs_layout(L, best_lp, best_sp == 0 ? 1 : best_sp, true, best_sp == 0)) return 1;
     L.lusgs_fused = best_sp == 0;
     L.lusgs_u = L.lusgs_fused ? 2 : best_u;
     if (L.lusgs_fused) L.fused_np = best_u == 3 ? 3 : 6;

@@ -37,7 +37,8 @@
 //
 // Record layout ("skewed").  A block's records are stored hyperplane by
 // hyperplane: the record of the cell that row t handles at step s sits at
-// position s*R + t, with R the row count rounded up to the rows of one wave,
+// position s*R + t, with R the row count rounded up to whole waves of the
+// sweep (the fused launch, one sweep wave: the row count itself),
 // S = ncol + nrow - 1 steps and n = S*R + 64 positions per field.  Fields go
 // in pairs of doubles, so that every load and store moves 16 bytes per lane:
 // the vector memory address path costs the same per lane for 8 and for 16
@@ -954,34 +955,58 @@ __global__ __launch_bounds__(64 * (2 + NP)) void k_lusgs_fused(LusgsArgs a) {
   bool ok = true;
   if (wave == FPROD + 1) {
     // ---- writer: each chunk of eight backward steps from ob to the block
-    // planes; lane = 8 rows x 8 steps, so that a store covers 8 consecutive
-    // cells (64 bytes) of each of 8 rows.  The last chunk may hold four
-    // steps: the other four map to columns past the block and are skipped.
-    // With update the planes are U (U + dU, the update of k_lusgs_scatter;
-    // nothing reads the dU planes after a main-loop pass, so they are not
-    // rewritten), without it dU.
-    const int sub = lane & 7;
+    // planes, every store one aligned 64-byte run (8 cells) of each of 8
+    // rows.  Row r's cells of a chunk (steps 8ck .. 8ck+7, columns c = step
+    // - r, plane offset ncol + 1 - c in the row, whose start is 128-byte
+    // aligned: the pitch is a multiple of 16) straddle two aligned runs.
+    // Lane = 8 rows x position p of the run (offset mod 8): it reads the one
+    // step whose cell sits at position p, always the same slot sub of the
+    // chunk.  Positions 0..q (q = (ncol + 1 + r) mod 8) complete the run they
+    // fall in and are stored at once; positions q+1..7 start the next run and
+    // are held one chunk, while the value held from the previous chunk
+    // completes this one.  A last turn stores the runs held after the final
+    // chunk.  Every cell is stored exactly once, cells outside the block
+    // (columns < 0 or >= ncol, rows >= nrow) are skipped.  With update the
+    // planes are U (U + dU, the update of k_lusgs_scatter; nothing reads the
+    // dU planes after a main-loop pass, so they are not rewritten), without
+    // it dU.
+    const int p = lane & 7, rl = lane >> 3;
+    const int q = (ncol + 1 + rl) & 7;   // rows rl + 8k share it: the pitch is a multiple of 8
+    const int sub = (q - p) & 7;
+    const bool held = p > q;             // this lane's cell starts the next run
     const int dst = a.update ? a.pm.U : a.pm.dU;
-    for (int ck = 0; ck < (npad + 7) / 8 && ok; ck++) {
-      ok = wait_bounded(&L.ob_ready, ck + 1, a.err, a.spin_max);
-      if (!ok) break;
+    const int nck = (npad + 7) / 8;
+    double keep[8][NV] = {};
+    for (int ck = 0; ck <= nck && ok; ck++) {
       double v[8][NV];
+      if (ck < nck) {
+        ok = wait_bounded(&L.ob_ready, ck + 1, a.err, a.spin_max);
+        if (!ok) break;
 #pragma unroll
-      for (int rg = 0; rg < 8; rg++)
+        for (int rg = 0; rg < 8; rg++)
 #pragma unroll
-        for (int m = 0; m < NV; m++) v[rg][m] = L.ob[sub][m][rg * 8 + (lane >> 3)];
-      __atomic_signal_fence(__ATOMIC_SEQ_CST);
-      __builtin_amdgcn_s_waitcnt(0xc07f);  // the buffer read before it is handed back
-      lds_put(&L.ob_taken, ck + 1);
-      const int sc = ck * 8 + sub;
+          for (int m = 0; m < NV; m++) v[rg][m] = L.ob[sub][m][rg * 8 + rl];
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __builtin_amdgcn_s_waitcnt(0xc07f);  // the buffer read before it is handed back
+        lds_put(&L.ob_taken, ck + 1);
+      } else {
+#pragma unroll
+        for (int rg = 0; rg < 8; rg++)
+#pragma unroll
+          for (int m = 0; m < NV; m++) v[rg][m] = 0.0;
+      }
 #pragma unroll
       for (int rg = 0; rg < 8; rg++) {
-        const int tr = rg * 8 + (lane >> 3), c = sc - tr;
-        if (tr < nrow && c >= 0 && c < ncol) {
-          const int64_t q = pidx(B, ncol - c, nrow - tr);
+        const int r = rg * 8 + rl;
+        // the run's offset in the row: 8 * floor((ncol + 1 + r) / 8) - 8ck + p
+        const int off = ((ncol + 1 + r) & ~7) - 8 * ck + p, c = ncol + 1 - off;
+        if (r < nrow && c >= 0 && c < ncol) {
+          const int64_t qi = (int64_t)(nrow - r + 1) * B.P + off;  // pidx(B, ncol - c, nrow - r)
 #pragma unroll
-          for (int m = 0; m < NV; m++) planes[(int64_t)(dst + m) * B.plane + q] = v[rg][m];
+          for (int m = 0; m < NV; m++) planes[(int64_t)(dst + m) * B.plane + qi] = held ? keep[rg][m] : v[rg][m];
         }
+#pragma unroll
+        for (int m = 0; m < NV; m++) keep[rg][m] = v[rg][m];
       }
     }
     return;

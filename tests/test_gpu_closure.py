@@ -388,6 +388,29 @@ def test_closure_model_residual_matches_oracle():
         assert diff is None, f"{a}: {diff}"
 
 
+def test_closure_model_autotune_keeps_relaxed_mut(tmp_path):
+    """Model 4 with mu_t under-relaxed (w = 0.3): each residual moves mu_t a
+    step towards the network's value, so the autotuner's timed residual
+    launches would carry extra relaxation steps into the run.  A tuned run
+    must equal an untuned run of the same passes bit for bit, U and mu_t."""
+    theta, norm = _trained_params("g_clo_tune_train")
+    d = _closure_case("g_clo_tune")
+    out = []
+    for tuned in (False, True):
+        with solver.Batch([solver.Case.from_dir(d)]) as b:
+            b.closure_set_params(theta, norm)
+            b.closure_relaxation(0.3)
+            b.init_flow()
+            b.run(4)
+            if tuned:
+                b.autotune(str(tmp_path / "tune.txt"), reps=2)
+            b.run(6)
+            out.append((b.get("U"), b.get("closure_mut")))
+    for k, name in enumerate(("U", "closure_mut")):
+        diff = first_diff(out[1][k], out[0][k])
+        assert diff is None, f"{name}: {diff}"
+
+
 def test_closure_model_runs_cheaper_than_sa():
     """The model-4 case advances 4 equations instead of SA's 5: per iteration
     it is cheaper than the SA case on the same 257 x 65 grid, stays finite,
