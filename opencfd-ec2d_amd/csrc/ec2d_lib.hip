@@ -82,6 +82,7 @@ struct DLevel {
   int lusgs_split = 1;        // LU-SGS sweep workgroups per block (2 or 4: a block's rows split between CUs)
   int lusgs_u = 2;            // LU-SGS sweep steps per register set (record prefetch depth), up to 4 waves
   bool lusgs_fused = false;   // LU-SGS records made inside the sweep workgroup (k_lusgs_fused)
+  bool planar_stale = false;  // Res, dt, Lc, Lv are current only in the hyperplane copy (planar_refresh)
   bool skew_static = false;   // the record arena holds the skewed copy of the mesh planes
   int fused_np = 6;           // its producer waves (3 or 6)
   unsigned long long* dgring = nullptr;  // split sweeps: one ring per boundary between workgroups and direction
@@ -109,6 +110,8 @@ struct Runtime {
   unsigned long long* derr = nullptr;
   int* dsweep_err = nullptr;  // a split LU-SGS sweep ran out of its wait budget
   int wait_budget = SPIN_MAX;  // polls of a bounded LU-SGS wait (ec2d_set_wait_budget)
+  long long sko_passes = 0;    // passes whose residual wrote the hyperplane copy alone (advance_lusgs)
+  bool graph_sko = false;      // the captured loop body is such a pass
   hipStream_t s = nullptr;
   std::string err;
   int num_mesh = 1;
@@ -430,9 +433,14 @@ inline int64_t pi_(int nx, int i, int j) { return (int64_t)(i - 1) + (int64_t)nx
 inline int64_t ci_(int nx, int i, int j) { return (int64_t)(i - 1) + (int64_t)(nx - 1) * (j - 1); } // (1:nx-1,...)
 inline int64_t wi_(int nx, int i, int j) { return (int64_t)i + (int64_t)(nx + 1) * j; }             // (0:nx,0:ny)
 
-int check_ready() {
+int planar_refresh();
+
+// refresh: restore the planes a main-loop pass left in the hyperplane copy
+// alone (planar_refresh); the main loop's own entry points and those that
+// read none of those planes pass false
+int check_ready(bool refresh = true) {
   if (!R.committed) return fail("ec2d_commit has not been called");
-  return 0;
+  return refresh ? planar_refresh() : 0;
 }
 
 // The device's two error words, read once the stream has drained.
@@ -530,7 +538,8 @@ int closure_infer() {
   return 0;
 }
 
-int launch_residual(int level, int clip, int force = 0, int set_un = 0, double* sk_out = nullptr) {
+int launch_residual(int level, int clip, int force = 0, int set_un = 0, double* sk_out = nullptr,
+                    bool sk_only = false) {
   DLevel& L = R.lv[level];
   ResArgs a;
   a.tiles = L.dt;
@@ -545,6 +554,7 @@ int launch_residual(int level, int clip, int force = 0, int set_un = 0, double* 
   a.pm = L.pm;
   a.clip = clip;
   a.sk_out = sk_out;
+  a.sk_only = sk_out != nullptr && sk_only ? 1 : 0;
   a.write_visc = 1;
   a.force = force;
   a.set_un = set_un;
@@ -771,6 +781,35 @@ int launch_lusgs(int level, const int* dlist, int nblk, double alfa1, int update
   return 0;
 }
 
+// Res, dt, Lci, Lcj, Lvi and Lvj of a level back into their planes from the
+// fused LU-SGS launch's hyperplane copy, after passes whose residual wrote
+// them there alone (advance_lusgs)
+int planar_refresh() {
+  for (int l = 0; l < R.num_mesh; l++) {
+    DLevel& L = R.lv[l];
+    if (!L.planar_stale) continue;
+    L.planar_stale = false;
+    LusgsArgs a{};
+    a.blocks = L.db;
+    a.block_list = L.dblocks_all;
+    a.arena = L.arena;
+    a.sk = L.sk;
+    a.pm = L.pm;
+    a.nblk = (int)L.hb.size();
+    a.sk_tiles = std::max(L.skew_tiles, 1);
+    const dim3 g((unsigned)(8 * (((int64_t)a.sk_tiles * a.nblk + 7) / 8)));
+    const int nv = L.o.nvar;
+    const bool visc = R.cases[0].cp.If_viscous == 1;
+    if (nv == 4 && !visc) hipLaunchKernelGGL((k_lusgs_unskew<4, false>), g, dim3(256), 0, R.s, a);
+    else if (nv == 4) hipLaunchKernelGGL((k_lusgs_unskew<4, true>), g, dim3(256), 0, R.s, a);
+    else if (nv == 5) hipLaunchKernelGGL((k_lusgs_unskew<5, true>), g, dim3(256), 0, R.s, a);
+    else if (nv == 6) hipLaunchKernelGGL((k_lusgs_unskew<6, true>), g, dim3(256), 0, R.s, a);
+    else return fail("bad equation count");
+    HIPCHK(hipGetLastError());
+  }
+  return 0;
+}
+
 int launch_ghost(int level, int do_bc, int do_exchange) {
   DLevel& L = R.lv[level];
   GhostArgs a;
@@ -863,11 +902,12 @@ int flush_clip(int level) {
 // force: 1 = comput_force_function, 2 = Add_force_function fused into the
 // residual's store (coarse levels; see ResArgs::force); set_un: Set_Un too
 // sk_out: see ResArgs (the fused LU-SGS that follows then skips its copy)
-int residual(int level, int force = 0, int set_un = 0, double* sk_out = nullptr) {
+int residual(int level, int force = 0, int set_un = 0, double* sk_out = nullptr, bool sk_only = false) {
   int clip = R.clip_pending[level] ? 1 : 0;
   R.clip_pending[level] = false;
   prof_mark(1 + 10 * level);
-  int rc = launch_residual(level, clip, force, set_un, sk_out);
+  int rc = launch_residual(level, clip, force, set_un, sk_out, sk_only);
+  if (!(sk_out && sk_only)) R.lv[level].planar_stale = false;  // the planes hold this residual's values
   if (!rc && R.cases[0].ctl.Time_Method == -1) {
     // dual-time source after the whole residual and before the norms:
     // Sfac = 1/(3 dt) on the first residual call, 1/(2 dt) afterwards
@@ -920,7 +960,17 @@ int advance_lusgs(int level, bool add_force) {
   // residual's epilogue (not on FD-WENO5 levels, whose interior Res a later
   // launch writes), in place of its copy k_lusgs_skew
   const bool rskew = L.lusgs_fused && !L.fdm;
-  if (residual(level, add_force ? 2 : 0, 0, rskew ? L.sk : nullptr)) return 1;
+  // On a single level (steady LU-SGS) nothing reads Res, dt, Lc or Lv of
+  // the planes before the next pass's residual rewrites them, so they go to
+  // the hyperplane copy alone; an entry point that may read them restores
+  // the planes first (planar_refresh, from check_ready).  A V-cycle's
+  // restriction reads Res: there every residual writes the planes.
+  const bool sko = rskew && R.num_mesh == 1;
+  if (residual(level, add_force ? 2 : 0, 0, rskew ? L.sk : nullptr, sko)) return 1;
+  if (sko) {
+    L.planar_stale = true;
+    R.sko_passes++;
+  }
   prof_mark(2 + 10 * level);
   if (launch_lusgs(level, L.dblocks_all, (int)L.hb.size(), 0.0, 1, nullptr, nullptr, rskew)) return 1;
   prof_mark(-(2 + 10 * level));
@@ -1908,7 +1958,7 @@ int ec2d_put_array(int case_id, int nMesh, int mBlock, int which, const double* 
 }
 
 int ec2d_get_array(int case_id, int nMesh, int mBlock, int which, double* dst) {
-  if (check_ready()) return 1;
+  if (check_ready(which != EC2D_U)) return 1;
   if (case_id < 0 || case_id >= (int)R.cases.size() || nMesh < 1 || nMesh > R.num_mesh) return fail("bad case / mesh");
   int gb = global_block(case_id, nMesh - 1, mBlock);
   if (gb < 0) return fail("bad block");
@@ -2119,7 +2169,7 @@ int ec2d_comput_new_u(int n) {
 }
 
 int ec2d_run(int n_iter) {
-  if (check_ready()) return 1;
+  if (check_ready(false)) return 1;
   if (n_iter <= 0) return 0;
   R.res_ms = R.lusgs_ms = R.ghost_ms = 0.0;
   HIPCHK(hipEventRecord(R.ev0, R.s));
@@ -2136,6 +2186,7 @@ int ec2d_run(int n_iter) {
     for (int k = 0; k < n_iter; k++)
       for (int l = 0; l < 3; l++)
         for (int t = 0; t < R.graph_ticks[l]; t++) tick(l);
+    if (R.graph_sko) R.lv[0].planar_stale = true;  // as advance_lusgs marks it in an eager pass
   } else {
     // capture one pass of the loop body; the clip flag must be in a steady
     // state (the body leaves it as it found it from the second pass on)
@@ -2146,8 +2197,10 @@ int ec2d_run(int n_iter) {
       std::memcpy(pend_before, R.clip_pending, sizeof(pend_before));
       hipGraph_t g;
       int before[3] = {tick_count[0], tick_count[1], tick_count[2]};
+      const long long sko_before = R.sko_passes;
       HIPCHK(hipStreamBeginCapture(R.s, hipStreamCaptureModeThreadLocal));
       int rc = main_step();
+      const bool body_sko = R.sko_passes != sko_before;
       int pass_ticks[3] = {tick_count[0] - before[0], tick_count[1] - before[1], tick_count[2] - before[2]};
       hipError_t e = hipStreamEndCapture(R.s, &g);
       if (rc) return rc;
@@ -2168,6 +2221,7 @@ int ec2d_run(int n_iter) {
       R.graph = ge;
       std::memcpy(R.graph_pend, pend_before, sizeof(R.graph_pend));
       std::memcpy(R.graph_ticks, pass_ticks, sizeof(R.graph_ticks));
+      R.graph_sko = body_sko;
     }
   }
   HIPCHK(hipEventRecord(R.ev1, R.s));
@@ -2219,7 +2273,7 @@ int ec2d_set_profile(int on) {
 }
 
 int ec2d_run_to_drop(double drop, int max_iter, int check_every, int* iters, int* status, double* ms) {
-  if (check_ready()) return 1;
+  if (check_ready(false)) return 1;
   if (!(drop > 0.0 && drop < 1.0) || max_iter < 1 || check_every < 1)
     return fail("ec2d_run_to_drop: need 0 < drop < 1, max_iter >= 1 and check_every >= 1");
   if (R.cases[0].ctl.Time_Method == -1) return fail("ec2d_run_to_drop: dual-time stepping runs a fixed physical time");
@@ -2774,7 +2828,7 @@ int ec2d_synchronize(void) {
 }
 
 int ec2d_res_norms(int case_id, int nMesh, double* res_max, double* res_rms) {
-  if (check_ready()) return 1;
+  if (check_ready(false)) return 1;
   if (case_id < 0 || case_id >= (int)R.cases.size() || nMesh < 1 || nMesh > R.num_mesh) return fail("bad case / mesh");
   DLevel& L = R.lv[nMesh - 1];
   double buf[12];

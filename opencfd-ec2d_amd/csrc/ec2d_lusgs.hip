@@ -498,6 +498,62 @@ __global__ __launch_bounds__(256) void k_lusgs_skew(LusgsArgs a) {
   }
 }
 
+// The reverse of k_lusgs_skew<..., false> for the planes a residual writes
+// (Res, dt, Lci, Lcj and, viscous, Lvi, Lvj; not U, whose planes the update
+// keeps): after a main-loop pass whose residual wrote them to the hyperplane
+// copy alone (ResArgs::sk_only), this puts them back into their planes.  The
+// same tiles, the two phases swapped: 16 lanes of a step read 16 consecutive
+// positions, 16 lanes of a row write 16 consecutive cells.  Every block is
+// copied, frozen cases included (their copy holds their last residual).
+template <int NV, bool VISC>
+__host__ __device__ constexpr bool unskew_plane(int k) {
+  return skew_plane<NV, VISC>(k, false) && (k < RecIn::U || k >= RecIn::U + NV);
+}
+
+template <int NV, bool VISC>
+__global__ __launch_bounds__(256) void k_lusgs_unskew(LusgsArgs a) {
+  constexpr int NK = RecIn::COUNT<NV>;
+  constexpr int NC = [] {
+    int n = 0;
+    for (int k = 0; k < NK; k++) n += unskew_plane<NV, VISC>(k) ? 1 : 0;
+    return n;
+  }();
+  __shared__ double tile[NC][16][17];
+  const int chunk = (int)gridDim.x / 8;
+  const int w = ((int)blockIdx.x % 8) * chunk + (int)blockIdx.x / 8;
+  const int bi = w / a.sk_tiles, ti = w % a.sk_tiles;
+  if (bi >= a.nblk) return;
+  const BlockDesc B = a.blocks[a.block_list[bi]];
+  const int ncol = B.nx - 1, nrow = B.ny - 1;
+  const int S = B.sk_S;
+  const int tiles_s = (S + 15) / 16;
+  if (ti >= tiles_s * ((nrow + 15) / 16)) return;
+  const int s0 = (ti % tiles_s) * 16, r0 = (ti / tiles_s) * 16;
+  const int tid = threadIdx.x;
+  {
+    const int ss = tid >> 4, rr = tid & 15, r = r0 + rr, s = s0 + ss, c = s - r;
+    if (s < S && r < nrow && c >= 0 && c < ncol) {
+      const double* __restrict__ sk = a.sk + B.sk_base + (int64_t)s * B.sk_R + r;
+      int j = 0;
+#pragma unroll
+      for (int k = 0; k < NK; k++)
+        if (unskew_plane<NV, VISC>(k)) tile[j++][rr][ss] = sk[(int64_t)k * B.sk_n];
+    }
+  }
+  __syncthreads();
+  {
+    const int rr = tid >> 4, ss = tid & 15, r = r0 + rr, s = s0 + ss, c = s - r;
+    if (s < S && r < nrow && c >= 0 && c < ncol) {
+      double* __restrict__ base = a.arena + B.base;
+      const int64_t q = pidx(B, c + 1, r + 1);
+      int j = 0;
+#pragma unroll
+      for (int k = 0; k < NK; k++)
+        if (unskew_plane<NV, VISC>(k)) base[(int64_t)rec_in_slot<NV>(a.pm, k) * B.plane + q] = tile[j++][rr][ss];
+    }
+  }
+}
+
 // ----------------------------------------------------------------- sweeps
 // One side's share of a cell, dui (the i-neighbour) or duj (the
 // j-neighbour), of sub_LU_SGS.f90:42-58 (forward) or :85-100 (backward);

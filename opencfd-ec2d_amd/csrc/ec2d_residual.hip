@@ -35,6 +35,8 @@ struct ResArgs {
   int clip;                   // apply force_vt_kw to interior nu_t on load
   double* sk_out;             // or null: the changing inputs of the fused LU-SGS in hyperplane order (the
                               // copy k_lusgs_skew makes), written here from the values the epilogue holds
+  int sk_only;                // with sk_out: Res, dt, Lci, Lcj, Lvi, Lvj go to the hyperplane copy alone,
+                              // not to their planes (k_lusgs_skew's reverse copy restores the planes)
   int write_visc;             // store Amu / Amu_t planes
   int force;                  // coarse multigrid levels, interior cells, variables 1..4, after the norms' values:
                               // 1: QF = QF - Res (comput_force_function), 2: Res = Res + QF (Add_force_function)
@@ -496,11 +498,13 @@ __global__ __launch_bounds__(TX* TY) void k_residual(ResArgs a) {
           double r = R[m];
           if (a.force == 1) PL(pm.QF + m)[q] = PL(pm.QF + m)[q] - r;
           else if (a.force == 2) r = r + PL(pm.QF + m)[q];
-          PL(pm.Res + m)[q] = r;
+          if (!a.sk_only) PL(pm.Res + m)[q] = r;
           Rs[m] = r;
         }
       }
-      if constexpr (SA) PL(pm.Res + 4)[q] = R[4];
+      if constexpr (SA) {
+        if (!a.sk_only) PL(pm.Res + 4)[q] = R[4];
+      }
       if constexpr (NV == 6) {
         // Res(5:6) came from k_sst_res; read for the norms
         R[4] = PL(pm.Res + 4)[q];
@@ -528,11 +532,15 @@ __global__ __launch_bounds__(TX* TY) void k_residual(ResArgs a) {
         double tmp1 = g / sd[lj][li] * (div_rcp(sAm[lj][li], cp.Pr, yPr) + div_rcp(sAt[lj][li], EC2D_PRT, yPrt));
         Lvi = tmp1 * si * si / s0;
         Lvj = tmp1 * sj * sj / s0;
-        PL(pm.Lvi)[q] = Lvi;
-        PL(pm.Lvj)[q] = Lvj;
       }
-      PL(pm.Lci)[q] = Lci;
-      PL(pm.Lcj)[q] = Lcj;
+      if (!a.sk_only) {
+        if constexpr (VISC) {
+          PL(pm.Lvi)[q] = Lvi;
+          PL(pm.Lvj)[q] = Lvj;
+        }
+        PL(pm.Lci)[q] = Lci;
+        PL(pm.Lcj)[q] = Lcj;
+      }
       double dt;
       if (cp.Iflag_local_dt == 0) {
         dt = cp.dt_global;
@@ -541,7 +549,7 @@ __global__ __launch_bounds__(TX* TY) void k_residual(ResArgs a) {
         if (dt > cp.dtmax) dt = cp.dtmax;
         if (dt < cp.dtmin) dt = cp.dtmin;
       }
-      PL(pm.dt)[q] = dt;
+      if (!a.sk_only) PL(pm.dt)[q] = dt;
       if (a.sk_out) {
         // logical plane k of cell (i, j) at k n + (i-1 + j-1) R + j-1 (the
         // layout of k_lusgs_skew): U (nu_t as clipped in the staging), Res,

@@ -82,3 +82,45 @@ def test_ensemble_uneven_sa_cases(tuned, tmp_path):
             gmax, grms = b.res_norms(ci)
             rmax, rrms = r.norms()
             np.testing.assert_allclose(grms[:5], rrms[:5], rtol=1e-11, atol=0)
+
+
+def test_fused_run_restores_residual_planes():
+    """Single-level fused LU-SGS passes write Res, dt and the spectral radii
+    to the update's hyperplane copy alone; reading them (or any entry point
+    that may) puts them back into their planes first.  Through graph capture,
+    a later replay of the captured pass and a read of U alone in between, the
+    planes must equal the oracle's after every run: 1e-11 relative to each
+    plane's largest value (SA's device exp / pow differ from the oracle's at
+    about 1e-16 absolute per step, so entries near zero have no relative
+    bound after eight passes)."""
+    prms = cases.ensemble_params(seed=5, n=3)
+    dirs = [cases.write_ensemble_case(case_dir(f"g_sko_{k}"), p) for k, p in enumerate(prms)]
+    refs = []
+    for d in dirs:
+        r = O.OracleCase(d)
+        r.init_flow()
+        refs.append(r)
+    planes = ("Res", "dt", "Lci", "Lcj", "Lvi", "Lvj")
+    with solver.Batch([solver.Case.from_dir(d) for d in dirs]) as b:
+        b.lusgs_shape(1, 1, 0)
+        b.init_flow()
+        for n, u_only in ((4, False), (3, True), (2, False)):
+            b.run(n)
+            for r in refs:
+                for _ in range(n):
+                    r.main_step()
+            if u_only:
+                _check(b, refs, ("U",), False, 1e-11)
+                b.run(1)
+                for r in refs:
+                    r.main_step()
+            for ci, r in enumerate(refs):
+                rb = r.block(1, 1)
+                np.testing.assert_allclose(b.get("U", ci), rb.U, rtol=1e-11, atol=1e-300, err_msg=f"case {ci} U")
+                for a in planes:
+                    g, want = b.get(a, ci), getattr(rb, a)
+                    if a.startswith("L"):
+                        g, want = g[:-1, :-1], want[:-1, :-1]
+                    scale = float(np.abs(want).max())
+                    np.testing.assert_allclose(g, want, rtol=1e-11, atol=1e-11 * scale,
+                                               err_msg=f"case {ci} {a} after run({n})")
