@@ -157,7 +157,11 @@ int ec2d_comput_new_u(int nMesh);
 
 /* n passes of the main-loop body (opencfd_ec2d_v1.5.4.f90:172-186) as one
  * HIP graph replayed n times; no host synchronisation inside, one at the end,
- * where the device error words are checked as in ec2d_synchronize. */
+ * where the device error words are checked as in ec2d_synchronize.  On a
+ * single level with the fused LU-SGS launch, the passes leave Res, dt and the
+ * spectral radii in the update's hyperplane copy only.  The next call that
+ * may read them restores their planes first, so every array reads as the
+ * reference's. */
 int ec2d_run(int n_iter);
 
 /* Runs main-loop passes (as ec2d_run, graph replays of check_every passes)
