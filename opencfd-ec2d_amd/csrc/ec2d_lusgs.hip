@@ -176,6 +176,16 @@ __device__ __forceinline__ void bstore2(double x, double y, __amdgpu_buffer_rsrc
   __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, v), r, voff, soff, 0);
 }
 
+// one double (the odd equation's field of the fused launch's X, 8 bytes a position)
+__device__ __forceinline__ double bload1(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+
+__device__ __forceinline__ void bstore1(double x, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, x), r, voff, soff, 0);
+}
+
 // v16: byte offset 16*pos of the lane's position, vx16 that of its X pairs
 // (its own, or a trash-pad slot); the field offsets are scalar and fixed.
 // base: byte offset of the record set in the block's record region, n16 =
@@ -631,7 +641,9 @@ __device__ __forceinline__ void wait_at_least(const int* p, int v) {
 // LPC = 2: lanes 2t (k = 0, the j-neighbour's term) and 2t+1 (k = 1, the
 // i-neighbour's term) share the cell and swap their halves; LPC = 1: lane t
 // computes both.
-template <int NV, bool VISC, bool FWD, int LPC, bool STORE = true>
+// ODD1: with an odd equation count the last X field holds single doubles
+// (the fused launch's layout), not a pair padded with zero
+template <int NV, bool VISC, bool FWD, int LPC, bool STORE = true, bool ODD1 = false>
 __device__ __forceinline__ void sweep_step(const Rec<NV, 3 - LPC>& p, Out<NV>& own, __amdgpu_buffer_rsrc_t rs,
                                            uint32_t dst, uint32_t n16, int s, int S, int R, int ncol, int nrow, int t,
                                            int k, int lane, int wave, bool cin, bool cout, double g,
@@ -714,8 +726,10 @@ __device__ __forceinline__ void sweep_step(const Rec<NV, 3 - LPC>& p, Out<NV>& o
     const uint32_t w = (act && k == 0) ? (FWD ? (uint32_t)((S - 1 - s) * R + (nrow - 1 - t)) : (uint32_t)(s * R + t))
                                        : (uint32_t)(S * R + lane);
 #pragma unroll
-    for (int j = 0; j < (NV + 1) / 2; j++)
-      bstore2(o.du[2 * j], 2 * j + 1 < NV ? o.du[2 * j + 1] : 0.0, rs, w * 16u, dst + (uint32_t)j * n16);
+    for (int j = 0; j < (NV + 1) / 2; j++) {
+      if (ODD1 && 2 * j + 1 == NV) bstore1(o.du[2 * j], rs, w * 8u, dst + (uint32_t)j * n16);
+      else bstore2(o.du[2 * j], 2 * j + 1 < NV ? o.du[2 * j + 1] : 0.0, rs, w * 16u, dst + (uint32_t)j * n16);
+    }
   }
   if (cout) {
     // slot s % RING is free once the next wave has read step s - RING's result
@@ -1090,7 +1104,6 @@ __global__ __launch_bounds__(64 * (2 + NP)) void k_lusgs_fused(LusgsArgs a) {
   // ---- the sweep wave: lane t = row t; it issues first on its SIMD
   __builtin_amdgcn_s_setprio(3);
   const int t = lane;
-  const int tl = min(t, nrow - 1);
   Chain<NV>& ch = *reinterpret_cast<Chain<NV>*>(&L);  // no other compute wave: never touched
   auto load_lds = [&](Rec<NV, 2>& r, int slot, bool fwd) {
 #pragma unroll
@@ -1120,13 +1133,24 @@ __global__ __launch_bounds__(64 * (2 + NP)) void k_lusgs_fused(LusgsArgs a) {
     }
   };
   // the backward sweep's X (the forward dU) from the backward records' X pairs
+  // (pairs, and with an odd equation count the last field as single doubles).
+  // A lane whose row has no cell at this step reads the nearest row that has
+  // one: positions without a cell (about (nrow-1)/ncol of the field) are
+  // never written, and a wave reading them would fetch lines nobody needs;
+  // such a lane's value never reaches a cell.
   auto load_x = [&](double* x, int st) {
-    const uint32_t v16 = (uint32_t)(min(st, S - 1) * R + tl) * 16u;
+    const int sc = min(st, S - 1);
+    const int tr = min(max(t, sc - (ncol - 1)), min(sc, nrow - 1));
+    const uint32_t pos = (uint32_t)(sc * R + tr);
 #pragma unroll
     for (int j = 0; j < F::XP; j++) {
-      const D2 v = bload2(rs, v16, bw + (uint32_t)j * n16);
-      x[2 * j] = v.x;
-      if (2 * j + 1 < NV) x[2 * j + 1] = v.y;
+      if (2 * j + 1 == NV) {
+        x[2 * j] = bload1(rs, pos * 8u, bw + (uint32_t)j * n16);
+      } else {
+        const D2 v = bload2(rs, pos * 16u, bw + (uint32_t)j * n16);
+        x[2 * j] = v.x;
+        x[2 * j + 1] = v.y;
+      }
     }
   };
   // one direction of the sweep; the direction is a compile-time constant so
@@ -1176,7 +1200,7 @@ __global__ __launch_bounds__(64 * (2 + NP)) void k_lusgs_fused(LusgsArgs a) {
       // forward: dU to the backward records' X pairs; backward: U + dU (the
       // record's U is the plane's value; with update) or dU into the output
       // buffer, which the writer wave stores eight steps at a time
-      sweep_step<NV, VISC, fwd, 1, fwd>(r, own, rs, bw, n16, st, S, R, ncol, nrow, t, 0, lane, 0, false, false, g, ch,
+      sweep_step<NV, VISC, fwd, 1, fwd, true>(r, own, rs, bw, n16, st, S, R, ncol, nrow, t, 0, lane, 0, false, false, g, ch,
                                         ah);
       // Slot gs is released after the step: its reads were issued a step ago
       // and the next slot's at this step's start, so the wait costs nothing
