@@ -641,8 +641,9 @@ __device__ __forceinline__ void wait_at_least(const int* p, int v) {
 // LPC = 2: lanes 2t (k = 0, the j-neighbour's term) and 2t+1 (k = 1, the
 // i-neighbour's term) share the cell and swap their halves; LPC = 1: lane t
 // computes both.
-// ODD1: with an odd equation count the last X field holds single doubles
-// (the fused launch's layout), not a pair padded with zero
+// ODD1 (the fused launch's layout): with an odd equation count the last X
+// field holds single doubles, not a pair padded with zero, and lanes without
+// a cell store nothing (no trash-pad slot)
 template <int NV, bool VISC, bool FWD, int LPC, bool STORE = true, bool ODD1 = false>
 __device__ __forceinline__ void sweep_step(const Rec<NV, 3 - LPC>& p, Out<NV>& own, __amdgpu_buffer_rsrc_t rs,
                                            uint32_t dst, uint32_t n16, int s, int S, int R, int ncol, int nrow, int t,
@@ -725,10 +726,12 @@ __device__ __forceinline__ void sweep_step(const Rec<NV, 3 - LPC>& p, Out<NV>& o
     // slot of the trash pad (positions S*R .. S*R+63)
     const uint32_t w = (act && k == 0) ? (FWD ? (uint32_t)((S - 1 - s) * R + (nrow - 1 - t)) : (uint32_t)(s * R + t))
                                        : (uint32_t)(S * R + lane);
+    if (!ODD1 || (act && k == 0)) {
 #pragma unroll
-    for (int j = 0; j < (NV + 1) / 2; j++) {
-      if (ODD1 && 2 * j + 1 == NV) bstore1(o.du[2 * j], rs, w * 8u, dst + (uint32_t)j * n16);
-      else bstore2(o.du[2 * j], 2 * j + 1 < NV ? o.du[2 * j + 1] : 0.0, rs, w * 16u, dst + (uint32_t)j * n16);
+      for (int j = 0; j < (NV + 1) / 2; j++) {
+        if (ODD1 && 2 * j + 1 == NV) bstore1(o.du[2 * j], rs, w * 8u, dst + (uint32_t)j * n16);
+        else bstore2(o.du[2 * j], 2 * j + 1 < NV ? o.du[2 * j + 1] : 0.0, rs, w * 16u, dst + (uint32_t)j * n16);
+      }
     }
   }
   if (cout) {
